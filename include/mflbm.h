@@ -1,0 +1,170 @@
+/*
+ * mflbm.h -- C ABI of the MI355X (gfx950) implementation of MF-LBM's per-time-step lattice update.
+ *
+ * The reference drives its hot path through Fortran subroutines without field arguments that operate
+ * on module globals inside one OpenACC data region.  This ABI replaces that boundary with a context
+ * handle that owns the device copies of the same state; every array crossing the boundary keeps the
+ * reference's host layout (Fortran column-major, x fastest, with the reference's halo widths):
+ *
+ *   PDF arrays f0..f18, g0..g18, curv, u, v, w, rho : (0:nx+1, 0:ny+1, 0:nz+1)   halo 1
+ *   phi                                            : (-3:nx+4, -3:ny+4, -3:nz+4) halo 4
+ *   cn_x, cn_y, cn_z, c_norm                       : (-1:nx+2, -1:ny+2, -1:nz+2) halo 2
+ *   walls (int8, 1 = solid)                        : (-1:nx+2, -1:ny+2, -1:nz+2) halo 2
+ *   w_in, phi_convec_bc                            : (0:nx+1, 0:ny+1)
+ *   f_convec_bc, g_convec_bc                       : (0:nx+1, 0:ny+1, 0:18)
+ *   (multiphase_3D/0.src/Init_multiphase.F90:583-700, Module.F90:263)
+ *
+ * Conventions: every entry point returns 0 (MFLBM_OK) on success and a nonzero MFLBM_ERR_* code
+ * otherwise; mflbm_last_error() gives the message.  A context is used by one host thread.  Device work
+ * is stream ordered; the download and monitor calls synchronise.  The reference aborts the MPI job on
+ * fatal errors (e.g. Init_multiphase.F90:316-318); here the caller decides.
+ */
+#ifndef MFLBM_H
+#define MFLBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFLBM_OK 0
+#define MFLBM_ERR_ARG 1      /* invalid argument or size */
+#define MFLBM_ERR_HIP 2      /* HIP runtime error (message names the call) */
+#define MFLBM_ERR_STATE 3    /* call out of order (e.g. step before upload) */
+#define MFLBM_ERR_NOMEM 4    /* allocation failed */
+#define MFLBM_ERR_NODEV 5    /* no GPU visible */
+
+/* Per-run constants.  Field meanings follow the reference's module variables
+ * (multiphase_3D/0.src/Module.F90:4-241) after initialization_basic_multi (Init_multiphase.F90:5-191). */
+typedef struct mflbm_params {
+    int32_t nx, ny, nz;                 /* interior lattice (nxGlobal, nyGlobal, nzGlobal; one domain) */
+    int32_t jper, kper;                 /* periodic y / z (x is never periodic, IO_multiphase.F90:471) */
+    int32_t wall_x_min, wall_x_max;     /* domain_wall_status_* : 1 = solid wall plane */
+    int32_t wall_y_min, wall_y_max;
+    int32_t wall_z_min, wall_z_max;
+    int32_t inlet_bc;                   /* 1 velocity bounce-back, 2 Zou-He pressure */
+    int32_t outlet_bc;                  /* 1 convective, 2 Zou-He pressure */
+    int32_t porous_plate_cmd;           /* 0 off, 1 block fluid 1, 2 block fluid 2 */
+    int32_t z_porous_plate;             /* plate position (1-based z) */
+    int32_t n_exclude_inlet, n_exclude_outlet;
+    double la_nui1, la_nui2;            /* 1/nu of fluid 1 and fluid 2 */
+    double gamma;                       /* surface tension */
+    double beta;                        /* recolouring parameter */
+    double force_z;                     /* body force along z */
+    double sa_inject;                   /* fluid-1 fraction of the injected fluid */
+    double phi_inlet;                   /* 2*sa_inject-1 */
+    double rho_in, rho_out;             /* Zou-He densities */
+    double uin_avg;                     /* convective outlet velocity (= mean inlet velocity) */
+    double relaxation;                  /* inlet ramp factor, 1 in the reference */
+} mflbm_params;
+
+/* Monitor sums of one call (Monitor.F90:32-85 and :512-550), all FP64.
+ * layers: 7*nz values, per z layer k (0-based) at [m*nz + k] for m = 0..6:
+ *   fl1, fl2 (flux of each fluid), vol1, vol2 (volume), mass1, mass2, pre (sum of rho). */
+typedef struct mflbm_monitor_sums {
+    double umax2;          /* max over fluid nodes of u^2+v^2+w^2 */
+    double usq1, usq2;     /* sum of u^2 over nodes with phi > 0.999 / phi < -0.999 */
+    double vol1_full, vol2_full;   /* cal_saturation sums over the whole interior */
+    double dphi_max;       /* max |phi - phi_old| over fluid nodes (steady phase-field monitor) */
+    double pre_w, pre_nw;  /* sums of rho over fluid nodes with phi < -0.99 / phi > 0.99 */
+    int64_t n_w, n_nw;     /* counts of those nodes */
+    int64_t outlet_phase1; /* fluid nodes at z = nz-1 with phi > 0 (breakthrough monitor) */
+} mflbm_monitor_sums;
+
+typedef struct mflbm_ctx mflbm_ctx;
+typedef struct mflbm_geometry mflbm_geometry;
+
+/* ---------------------------------------------------------------- context -------------------------- */
+
+/* Create a context on the current HIP device and allocate all device state.
+ * Replaces MemAllocate_multi (Init_multiphase.F90:583-828) and the `!$acc data` region
+ * (Main_multiphase.F90:104-115). */
+int mflbm_create(const mflbm_params* params, mflbm_ctx** out);
+void mflbm_destroy(mflbm_ctx* ctx);
+const char* mflbm_last_error(const mflbm_ctx* ctx);
+/* Message of the last failed mflbm_create (no context exists then). */
+const char* mflbm_create_error(void);
+
+/* Update run scalars between steps (force_z, rho_in, uin_avg, relaxation); the reference changes them
+ * from the driver (e.g. IO_multiphase.F90:586 checkpoint header). */
+int mflbm_set_scalars(mflbm_ctx* ctx, double force_z, double rho_in, double uin_avg, double relaxation);
+
+/* Upload the geometry: walls with halo 2, the solid-boundary list (Module.F90:90-96: ix,iy,iz,
+ * i_fluid_num, neighbor_list(18), la_weight) and the fluid-boundary list (Module.F90:98-103:
+ * ix,iy,iz,nwx,nwy,nwz,theta) as structure-of-arrays, and the inlet profile w_in.  Coordinates are the
+ * reference's 1-based local indices.  s_ijk/f_ijk hold 3 ints per node, s_nbr 18 ints per node,
+ * f_nw 3 doubles per node.  theta is in radians (already converted as IO_multiphase.F90:467-468). */
+int mflbm_upload_geometry(mflbm_ctx* ctx, const int8_t* walls_h2,
+                          int32_t n_solid, const int32_t* s_ijk, const int32_t* s_nfluid,
+                          const int32_t* s_nbr, const double* s_weight,
+                          int32_t n_fluid, const int32_t* f_ijk, const double* f_nw, const double* f_theta,
+                          const double* w_in);
+
+/* Upload the full solver state (checkpoint content, IO_multiphase.F90:586-631): 19 pointers each for
+ * f and g, phi, and the convective-outlet history (may be NULL when outlet_bc != 1). */
+int mflbm_upload_state(mflbm_ctx* ctx, const double* const* f, const double* const* g, const double* phi,
+                       const double* f_convec, const double* g_convec, const double* phi_convec);
+int mflbm_download_state(mflbm_ctx* ctx, double* const* f, double* const* g, double* phi,
+                         double* f_convec, double* g_convec, double* phi_convec);
+
+/* ---------------------------------------------------------------- hot path ------------------------- */
+
+/* One time step (main_iteration_kernel, Main_multiphase.F90:341-486): even ntime -> in-place collision
+ * (kernel_even_color), odd ntime -> pull/push collision (kernel_odd_color); periodic wrap of PDFs and
+ * phi (replaces the MPI self-exchange of Mpi.F90); inlet/outlet/porous-plate boundaries; colour
+ * gradient chain (color_gradient, Phase_gradient.F90:5-204). */
+int mflbm_step(mflbm_ctx* ctx, int32_t ntime);
+/* nsteps consecutive steps ntime0 .. ntime0+nsteps-1, launched back to back. */
+int mflbm_run(mflbm_ctx* ctx, int32_t ntime0, int32_t nsteps);
+/* The colour-gradient chain alone (Phase_gradient.F90:5-204), as the driver calls it after
+ * initialisation (Main_multiphase.F90:120). */
+int mflbm_color_gradient(mflbm_ctx* ctx);
+
+/* u, v, w, rho from the PDFs (compute_macro_vars, Misc.F90:372-430; meaningful after even steps), also
+ * zeroing phi inside solids as the reference does. Stays on the device. */
+int mflbm_compute_macro(mflbm_ctx* ctx);
+/* Monitor reductions on the device (Monitor.F90); calls compute_macro first.  layers: 7*nz doubles.
+ * update_phi_old != 0 copies phi into phi_old afterwards (monitor_multiphase_steady_phasefield). */
+int mflbm_monitor(mflbm_ctx* ctx, double* layers, mflbm_monitor_sums* sums, int32_t update_phi_old);
+
+int mflbm_download_macro(mflbm_ctx* ctx, double* u, double* v, double* w, double* rho, double* phi);
+int mflbm_download_gradient(mflbm_ctx* ctx, double* cn_x, double* cn_y, double* cn_z, double* c_norm,
+                            double* curv);
+int mflbm_synchronize(mflbm_ctx* ctx);
+/* Mean duration (ms) of the step kernels recorded with HIP events since the last reset, measured on the
+ * context's stream: which = 0 collision-streaming kernel, 1 gradient chain, 2 whole step. */
+int mflbm_timing(mflbm_ctx* ctx, int32_t which, double* mean_ms, int64_t* count);
+int mflbm_timing_enable(mflbm_ctx* ctx, int32_t enable);
+
+/* ---------------------------------------------------------------- host set-up ---------------------- */
+
+/* Local wall array with halo 2 as set_walls builds it for one rank (Misc.F90:137-210, including the
+ * periodic wall transport Mpi_misc.F90:337-502).  walls_global: nx*ny*nz int8, x fastest. */
+int mflbm_setup_walls(const mflbm_params* params, const int8_t* walls_global, int8_t* walls_h2);
+
+/* Boundary-node classification, wall normals and extrapolation weights
+ * (geometry_preprocessing_new, Geometry_preprocessing.F90:9-512). */
+int mflbm_geometry_new(const mflbm_params* params, const int8_t* walls_global, double theta,
+                       mflbm_geometry** out);
+int mflbm_geometry_counts(const mflbm_geometry* geom, int32_t* n_solid, int32_t* n_fluid);
+int mflbm_geometry_export(const mflbm_geometry* geom, int32_t* s_ijk, int32_t* s_nfluid, int32_t* s_nbr,
+                          double* s_weight, int32_t* f_ijk, double* f_nw, double* f_theta);
+void mflbm_geometry_free(mflbm_geometry* geom);
+
+/* Initial fields of a new run (initialization_new_multi / _pdf, Init_multiphase.F90:243-470):
+ * phi by initial_fluid_distribution_option 1..5, equilibrium PDFs from (u, v, w) (NULL = at rest)
+ * and rho = 1, convective-outlet history.  f, g: 19 pointers each. */
+int mflbm_init_fields(const mflbm_params* params, int32_t option, double interface_z0,
+                      const double* u0, const double* v0, const double* w0,
+                      double* const* f, double* const* g, double* phi,
+                      double* f_convec, double* g_convec, double* phi_convec);
+
+/* Inlet velocity profile w_in (initialization_open_velocity_inlet_BC, Init_multiphase.F90:194-236 and
+ * inlet_vel_profile_rectangular, Misc.F90:625-665). */
+int mflbm_inlet_profile(const mflbm_params* params, double uin_avg, double la_x, double la_y, double* w_in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFLBM_H */

@@ -1,0 +1,407 @@
+// Host-side set-up of a run: local wall array, boundary-node lists with wall normals, initial fields and
+// the inlet velocity profile.  These run once per run on the host and feed the device state; their
+// arithmetic follows the reference expression by expression so the device path starts from the same bits.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "lattice.h"
+#include "mflbm.h"
+
+namespace {
+
+// (i, j, k) -> offset in an array with halo h in every direction, x fastest.
+struct Grid {
+    int nx, ny, nz, h;
+    long sx, sy;
+    Grid(int nx_, int ny_, int nz_, int h_) : nx(nx_), ny(ny_), nz(nz_), h(h_) {
+        sx = nx + 2 * h;
+        sy = (long)(ny + 2 * h) * sx;
+    }
+    long at(int i, int j, int k) const { return (long)(i + h - 1) + (long)(j + h - 1) * sx + (long)(k + h - 1) * sy; }
+    long size() const { return sy * (nz + 2 * h); }
+};
+
+bool bad_params(const mflbm_params* p) {
+    return !p || p->nx < 3 || p->ny < 3 || p->nz < 3;
+}
+
+}  // namespace
+
+struct mflbm_geometry {
+    std::vector<int32_t> s_ijk, s_nfl, s_nbr;
+    std::vector<double> s_w;
+    std::vector<int32_t> f_ijk;
+    std::vector<double> f_nw, f_theta;
+};
+
+extern "C" int mflbm_setup_walls(const mflbm_params* p, const int8_t* wg, int8_t* walls_h2) {
+    // set_walls for a single rank (Misc.F90:24-36, 130-210) followed by the periodic transport of the
+    // wall halo (ztransport_walls(0,0,2), ytransport_walls(0,2,2), xtransport_walls(2,2,2); x is never
+    // periodic so the x transport changes nothing).
+    if (bad_params(p) || !wg || !walls_h2) return MFLBM_ERR_ARG;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    Grid G(nx, ny, nz, 2);
+    std::memset(walls_h2, 0, (size_t)G.size());
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                walls_h2[G.at(i, j, k)] = wg[(long)(i - 1) + (long)(j - 1) * nx + (long)(k - 1) * nx * ny];
+    for (int k = -1; k <= nz + 2; ++k)
+        for (int j = -1; j <= ny + 2; ++j)
+            for (int i = -1; i <= nx + 2; ++i) {
+                int8_t& w = walls_h2[G.at(i, j, k)];
+                if (k <= 1 && p->wall_z_min == 1) w = 1;
+                if (k >= nz && p->wall_z_max == 1) w = 1;
+                if (i <= 1 && p->wall_x_min == 1) w = 1;
+                if (i >= nx && p->wall_x_max == 1) w = 1;
+                if (j <= 1 && p->wall_y_min == 1) w = 1;
+                if (j >= ny && p->wall_y_max == 1) w = 1;
+            }
+    if (p->kper == 1) {  // z halo of the interior x/y range from the periodic image
+        for (int l = 1; l <= 2; ++l)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    walls_h2[G.at(i, j, l - 2)] = walls_h2[G.at(i, j, nz + l - 2)];
+                    walls_h2[G.at(i, j, nz + l)] = walls_h2[G.at(i, j, l)];
+                }
+    }
+    if (p->jper == 1) {  // y halo for k over the full z range including its halo
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int l = 1; l <= 2; ++l)
+                for (int i = 1; i <= nx; ++i) {
+                    walls_h2[G.at(i, l - 2, k)] = walls_h2[G.at(i, ny + l - 2, k)];
+                    walls_h2[G.at(i, ny + l, k)] = walls_h2[G.at(i, l, k)];
+                }
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_geometry_new(const mflbm_params* p, const int8_t* wg, double theta, mflbm_geometry** out) {
+    // geometry_preprocessing_new (Geometry_preprocessing.F90:9-512), single rank: extended wall copy with
+    // 10 ghost layers, 1/2 (solid) and 0/-1 (fluid) boundary marks, four passes of 27-point smoothing, then
+    // the solid list (with fluid neighbours and weights) and the fluid list (ISO8 normals) over the range
+    // that covers the phi halo, finally restricted to 3 (solid) / 2 (fluid) ghost layers.
+    if (bad_params(p) || !wg || !out) return MFLBM_ERR_ARG;
+    *out = nullptr;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const int GL = 10;
+    if ((p->kper == 1 && nz < GL) || (p->jper == 1 && ny < GL)) return MFLBM_ERR_ARG;
+    Grid G(nx, ny, nz, GL);
+    std::vector<int8_t> t(G.size(), 0);
+    std::vector<double> s1(G.size()), s2(G.size());
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                t[G.at(i, j, k)] = wg[(long)(i - 1) + (long)(j - 1) * nx + (long)(k - 1) * nx * ny];
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int l = 1 - GL; l <= 0; ++l) {
+                if (p->kper == 0) {
+                    t[G.at(i, j, l)] = t[G.at(i, j, 1)];
+                    t[G.at(i, j, nz + GL + l)] = t[G.at(i, j, nz)];
+                } else {
+                    t[G.at(i, j, l)] = t[G.at(i, j, nz + l)];
+                    t[G.at(i, j, nz + GL + l)] = t[G.at(i, j, GL + l)];
+                }
+            }
+    for (int k = 1 - GL; k <= nz + GL; ++k)
+        for (int i = 1; i <= nx; ++i)
+            for (int l = 1 - GL; l <= 0; ++l) {
+                if (p->jper == 0) {
+                    t[G.at(i, l, k)] = t[G.at(i, 1, k)];
+                    t[G.at(i, ny + GL + l, k)] = t[G.at(i, ny, k)];
+                } else {
+                    t[G.at(i, l, k)] = t[G.at(i, ny + l, k)];
+                    t[G.at(i, ny + GL + l, k)] = t[G.at(i, GL + l, k)];
+                }
+            }
+    for (int k = 1 - GL; k <= nz + GL; ++k)
+        for (int j = 1 - GL; j <= ny + GL; ++j)
+            for (int l = 1 - GL; l <= 0; ++l) {  // x is never periodic
+                t[G.at(l, j, k)] = t[G.at(1, j, k)];
+                t[G.at(nx + GL + l, j, k)] = t[G.at(nx, j, k)];
+            }
+    for (long n = 0; n < G.size(); ++n) s1[n] = s2[n] = (double)t[n];
+    // boundary marks; the fluid/solid class of every node is unchanged by the marks, so the in-place sweep
+    // gives the same result in any order
+    for (int k = 2 - GL; k <= nz + GL - 1; ++k)
+        for (int j = 2 - GL; j <= ny + GL - 1; ++j)
+            for (int i = 2 - GL; i <= nx + GL - 1; ++i) {
+                int8_t& c = t[G.at(i, j, k)];
+                if (c == 1) {
+                    for (int q = 1; q <= 18; ++q)
+                        if (t[G.at(i + lat::EX[q], j + lat::EY[q], k + lat::EZ[q])] <= 0) { c = 2; break; }
+                }
+                if (c == 0) {
+                    for (int q = 1; q <= 18; ++q)
+                        if (t[G.at(i + lat::EX[q], j + lat::EY[q], k + lat::EZ[q])] >= 1) { c = -1; break; }
+                }
+            }
+    static const int IEX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+    static const int IEY[27] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1};
+    static const int IEZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, 1, 1, -1};
+    const double WE[4] = {8.0 / 27.0, 2.0 / 27.0, 1.0 / 54.0, 1.0 / 216.0};
+    for (int it = 0; it < 4; ++it) {
+        for (int k = 2 - GL; k <= nz + GL - 1; ++k)
+            for (int j = 2 - GL; j <= ny + GL - 1; ++j)
+                for (int i = 2 - GL; i <= nx + GL - 1; ++i) {
+                    double acc = 0.0;
+                    for (int n = 0; n < 27; ++n) {
+                        int m = IEX[n] * IEX[n] + IEY[n] * IEY[n] + IEZ[n] * IEZ[n];
+                        acc = acc + s1[G.at(i + IEX[n], j + IEY[n], k + IEZ[n])] * WE[m];
+                    }
+                    s2[G.at(i, j, k)] = acc;
+                }
+        for (int k = 2 - GL; k <= nz + GL - 1; ++k)
+            for (int j = 2 - GL; j <= ny + GL - 1; ++j)
+                for (int i = 2 - GL; i <= nx + GL - 1; ++i) s1[G.at(i, j, k)] = s2[G.at(i, j, k)];
+    }
+    mflbm_geometry* geo = new (std::nothrow) mflbm_geometry();
+    if (!geo) return MFLBM_ERR_NOMEM;
+    const double* W = s2.data();
+    auto S = [&](int i, int j, int k) { return W[G.at(i, j, k)]; };
+    const double I1 = 4.0 / 45.0, I2 = 1.0 / 21.0, I3 = 2.0 / 105.0, I4 = 5.0 / 504.0, I5 = 1.0 / 315.0,
+                 I6 = 1.0 / 630.0, I7 = 1.0 / 5040.0;
+    for (int k = -3; k <= nz + 4; ++k)
+        for (int j = -3; j <= ny + 4; ++j)
+            for (int i = -3; i <= nx + 4; ++i) {
+                const int8_t c = t[G.at(i, j, k)];
+                if (c == 2) {
+                    if (i < -2 || i > nx + 3 || j < -2 || j > ny + 3 || k < -2 || k > nz + 3) continue;
+                    int nbr[18] = {0};
+                    int cnt = 0;
+                    double w = 0.0;
+                    for (int q = 1; q <= 18; ++q)
+                        if (t[G.at(i + lat::EX[q], j + lat::EY[q], k + lat::EZ[q])] <= 0) {
+                            w = w + lat::W[q];
+                            nbr[cnt++] = q;
+                        }
+                    geo->s_ijk.push_back(i);
+                    geo->s_ijk.push_back(j);
+                    geo->s_ijk.push_back(k);
+                    geo->s_nfl.push_back(cnt);
+                    for (int q = 0; q < 18; ++q) geo->s_nbr.push_back(nbr[q]);
+                    geo->s_w.push_back(w);
+                } else if (c == -1) {
+                    if (i < -1 || i > nx + 2 || j < -1 || j > ny + 2 || k < -1 || k > nz + 2) continue;
+                    // ISO8 gradient of the smoothed wall field (Geometry_preprocessing.F90:230-368)
+                    double nwx = I1 * (S(i + 1, j, k) - S(i - 1, j, k)) +
+                        I2 * (S(i + 1, j + 1, k) - S(i - 1, j - 1, k) + S(i + 1, j - 1, k) - S(i - 1, j + 1, k) +
+                              S(i + 1, j, k + 1) - S(i - 1, j, k - 1) + S(i + 1, j, k - 1) - S(i - 1, j, k + 1)) +
+                        I3 * (S(i + 1, j + 1, k + 1) - S(i - 1, j - 1, k - 1) + S(i + 1, j + 1, k - 1) - S(i - 1, j - 1, k + 1) +
+                              S(i + 1, j - 1, k + 1) - S(i - 1, j + 1, k - 1) + S(i + 1, j - 1, k - 1) - S(i - 1, j + 1, k + 1)) +
+                        I4 * (S(i + 2, j, k) - S(i - 2, j, k)) +
+                        I5 * (S(i + 2, j + 1, k) - S(i - 2, j - 1, k) + S(i + 2, j - 1, k) - S(i - 2, j + 1, k) +
+                              S(i + 2, j, k + 1) - S(i - 2, j, k - 1) + S(i + 2, j, k - 1) - S(i - 2, j, k + 1) +
+                              S(i + 1, j + 2, k) - S(i - 1, j - 2, k) + S(i + 1, j - 2, k) - S(i - 1, j + 2, k) +
+                              S(i + 1, j, k + 2) - S(i - 1, j, k - 2) + S(i + 1, j, k - 2) - S(i - 1, j, k + 2)) +
+                        I6 * (S(i + 2, j + 1, k + 1) - S(i - 2, j - 1, k - 1) + S(i + 2, j + 1, k - 1) - S(i - 2, j - 1, k + 1) +
+                              S(i + 2, j - 1, k + 1) - S(i - 2, j + 1, k - 1) + S(i + 2, j - 1, k - 1) - S(i - 2, j + 1, k + 1) +
+                              S(i + 1, j + 2, k + 1) - S(i - 1, j - 2, k - 1) + S(i + 1, j + 2, k - 1) - S(i - 1, j - 2, k + 1) +
+                              S(i + 1, j - 2, k + 1) - S(i - 1, j + 2, k - 1) + S(i + 1, j - 2, k - 1) - S(i - 1, j + 2, k + 1) +
+                              S(i + 1, j + 1, k + 2) - S(i - 1, j - 1, k - 2) + S(i + 1, j + 1, k - 2) - S(i - 1, j - 1, k + 2) +
+                              S(i + 1, j - 1, k + 2) - S(i - 1, j + 1, k - 2) + S(i + 1, j - 1, k - 2) - S(i - 1, j + 1, k + 2)) +
+                        I7 * (S(i + 2, j + 2, k) - S(i - 2, j - 2, k) + S(i + 2, j - 2, k) - S(i - 2, j + 2, k) +
+                              S(i + 2, j, k + 2) - S(i - 2, j, k - 2) + S(i + 2, j, k - 2) - S(i - 2, j, k + 2));
+                    double nwy = I1 * (S(i, j + 1, k) - S(i, j - 1, k)) +
+                        I2 * (S(i + 1, j + 1, k) - S(i - 1, j - 1, k) + S(i - 1, j + 1, k) - S(i + 1, j - 1, k) +
+                              S(i, j + 1, k + 1) - S(i, j - 1, k - 1) + S(i, j + 1, k - 1) - S(i, j - 1, k + 1)) +
+                        I3 * (S(i + 1, j + 1, k + 1) - S(i - 1, j - 1, k - 1) + S(i + 1, j + 1, k - 1) - S(i - 1, j - 1, k + 1) +
+                              S(i - 1, j + 1, k - 1) - S(i + 1, j - 1, k + 1) + S(i - 1, j + 1, k + 1) - S(i + 1, j - 1, k - 1)) +
+                        I4 * (S(i, j + 2, k) - S(i, j - 2, k)) +
+                        I5 * (S(i + 2, j + 1, k) - S(i - 2, j - 1, k) + S(i - 2, j + 1, k) - S(i + 2, j - 1, k) +
+                              S(i, j + 2, k + 1) - S(i, j - 2, k - 1) + S(i, j + 2, k - 1) - S(i, j - 2, k + 1) +
+                              S(i + 1, j + 2, k) - S(i - 1, j - 2, k) + S(i - 1, j + 2, k) - S(i + 1, j - 2, k) +
+                              S(i, j + 1, k + 2) - S(i, j - 1, k - 2) + S(i, j + 1, k - 2) - S(i, j - 1, k + 2)) +
+                        I6 * (S(i + 2, j + 1, k + 1) - S(i - 2, j - 1, k - 1) + S(i + 2, j + 1, k - 1) - S(i - 2, j - 1, k + 1) +
+                              S(i - 2, j + 1, k + 1) - S(i + 2, j - 1, k - 1) + S(i - 2, j + 1, k - 1) - S(i + 2, j - 1, k + 1) +
+                              S(i + 1, j + 2, k + 1) - S(i - 1, j - 2, k - 1) + S(i + 1, j + 2, k - 1) - S(i - 1, j - 2, k + 1) +
+                              S(i - 1, j + 2, k + 1) - S(i + 1, j - 2, k - 1) + S(i - 1, j + 2, k - 1) - S(i + 1, j - 2, k + 1) +
+                              S(i + 1, j + 1, k + 2) - S(i - 1, j - 1, k - 2) + S(i + 1, j + 1, k - 2) - S(i - 1, j - 1, k + 2) +
+                              S(i - 1, j + 1, k + 2) - S(i + 1, j - 1, k - 2) + S(i - 1, j + 1, k - 2) - S(i + 1, j - 1, k + 2)) +
+                        I7 * (S(i + 2, j + 2, k) - S(i - 2, j - 2, k) + S(i - 2, j + 2, k) - S(i + 2, j - 2, k) +
+                              S(i, j + 2, k + 2) - S(i, j - 2, k - 2) + S(i, j + 2, k - 2) - S(i, j - 2, k + 2));
+                    double nwz = I1 * (S(i, j, k + 1) - S(i, j, k - 1)) +
+                        I2 * (S(i, j + 1, k + 1) - S(i, j - 1, k - 1) + S(i, j - 1, k + 1) - S(i, j + 1, k - 1) +
+                              S(i + 1, j, k + 1) - S(i - 1, j, k - 1) + S(i - 1, j, k + 1) - S(i + 1, j, k - 1)) +
+                        I3 * (S(i + 1, j + 1, k + 1) - S(i - 1, j - 1, k - 1) + S(i + 1, j - 1, k + 1) - S(i - 1, j + 1, k - 1) +
+                              S(i - 1, j + 1, k + 1) - S(i + 1, j - 1, k - 1) + S(i - 1, j - 1, k + 1) - S(i + 1, j + 1, k - 1)) +
+                        I4 * (S(i, j, k + 2) - S(i, j, k - 2)) +
+                        I5 * (S(i, j + 1, k + 2) - S(i, j - 1, k - 2) + S(i, j - 1, k + 2) - S(i, j + 1, k - 2) +
+                              S(i + 2, j, k + 1) - S(i - 2, j, k - 1) + S(i - 2, j, k + 1) - S(i + 2, j, k - 1) +
+                              S(i, j + 2, k + 1) - S(i, j - 2, k - 1) + S(i, j - 2, k + 1) - S(i, j + 2, k - 1) +
+                              S(i + 1, j, k + 2) - S(i - 1, j, k - 2) + S(i - 1, j, k + 2) - S(i + 1, j, k - 2)) +
+                        I6 * (S(i + 2, j + 1, k + 1) - S(i - 2, j - 1, k - 1) + S(i + 2, j - 1, k + 1) - S(i - 2, j + 1, k - 1) +
+                              S(i - 2, j + 1, k + 1) - S(i + 2, j - 1, k - 1) + S(i - 2, j - 1, k + 1) - S(i + 2, j + 1, k - 1) +
+                              S(i + 1, j + 2, k + 1) - S(i - 1, j - 2, k - 1) + S(i + 1, j - 2, k + 1) - S(i - 1, j + 2, k - 1) +
+                              S(i - 1, j + 2, k + 1) - S(i + 1, j - 2, k - 1) + S(i - 1, j - 2, k + 1) - S(i + 1, j + 2, k - 1) +
+                              S(i + 1, j + 1, k + 2) - S(i - 1, j - 1, k - 2) + S(i + 1, j - 1, k + 2) - S(i - 1, j + 1, k - 2) +
+                              S(i - 1, j + 1, k + 2) - S(i + 1, j - 1, k - 2) + S(i - 1, j - 1, k + 2) - S(i + 1, j + 1, k - 2)) +
+                        I7 * (S(i, j + 2, k + 2) - S(i, j - 2, k - 2) + S(i, j - 2, k + 2) - S(i, j + 2, k - 2) +
+                              S(i + 2, j, k + 2) - S(i - 2, j, k - 2) + S(i - 2, j, k + 2) - S(i + 2, j, k - 2));
+                    double tmp = 1.0 / (std::sqrt(nwx * nwx + nwy * nwy + nwz * nwz) + lat::EPS);
+                    geo->f_ijk.push_back(i);
+                    geo->f_ijk.push_back(j);
+                    geo->f_ijk.push_back(k);
+                    geo->f_nw.push_back(nwx * tmp);
+                    geo->f_nw.push_back(nwy * tmp);
+                    geo->f_nw.push_back(nwz * tmp);
+                    geo->f_theta.push_back(theta);
+                }
+            }
+    *out = geo;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_geometry_counts(const mflbm_geometry* g, int32_t* ns, int32_t* nf) {
+    if (!g || !ns || !nf) return MFLBM_ERR_ARG;
+    *ns = (int32_t)g->s_nfl.size();
+    *nf = (int32_t)g->f_theta.size();
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_geometry_export(const mflbm_geometry* g, int32_t* s_ijk, int32_t* s_nfl, int32_t* s_nbr,
+                                     double* s_w, int32_t* f_ijk, double* f_nw, double* f_theta) {
+    if (!g) return MFLBM_ERR_ARG;
+    auto cp = [](auto* dst, const auto& src) {
+        if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    cp(s_ijk, g->s_ijk);
+    cp(s_nfl, g->s_nfl);
+    cp(s_nbr, g->s_nbr);
+    cp(s_w, g->s_w);
+    cp(f_ijk, g->f_ijk);
+    cp(f_nw, g->f_nw);
+    cp(f_theta, g->f_theta);
+    return MFLBM_OK;
+}
+
+extern "C" void mflbm_geometry_free(mflbm_geometry* g) { delete g; }
+
+extern "C" int mflbm_init_fields(const mflbm_params* p, int32_t option, double z0, const double* u0,
+                                 const double* v0, const double* w0, double* const* f, double* const* g,
+                                 double* phi, double* f_convec, double* g_convec, double* phi_convec) {
+    // initialization_new_multi (Init_multiphase.F90:243-354) and initialization_new_multi_pdf (:357-470)
+    if (bad_params(p) || !f || !g || !phi) return MFLBM_ERR_ARG;
+    if (option < 1 || option > 5) return MFLBM_ERR_ARG;  // option 6 draws unseeded random numbers
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    Grid P(nx, ny, nz, 4), F(nx, ny, nz, 1);
+    const bool open_z = p->kper == 0 && p->wall_z_min == 0 && p->wall_z_max == 0;
+    for (int k = -3; k <= nz + 4; ++k)
+        for (int j = -3; j <= ny + 4; ++j)
+            for (int i = -3; i <= nx + 4; ++i) {
+                const double x = i, y = j, z = k;
+                double v;
+                if (option == 1) {
+                    v = -1.0;
+                    if (z <= z0) v = 1.0;
+                } else if (option == 2) {
+                    v = 1.0;
+                    if (z <= z0) v = -1.0;
+                } else if (option == 3 || option == 4) {
+                    const double dx = x - (nx + 1) * 0.0, dz = z - (nz + 1) * 0.5, dy = y - (ny + 1) * 0.5;
+                    const bool in = dx * dx + dz * dz + dy * dy <= z0 * z0;
+                    v = option == 3 ? (in ? 1.0 : -1.0) : (in ? -1.0 : 1.0);
+                } else {
+                    const double dx = x - (nx + 1) * 0.5, dz = z - (nz + 1) * 0.5, dy = y - (ny + 1) * 0.5;
+                    v = dx * dx + dz * dz + dy * dy <= z0 * z0 ? 1.0 : -1.0;
+                }
+                if (open_z && z <= 0) v = p->phi_inlet;
+                phi[P.at(i, j, k)] = v;
+            }
+    const double W0 = lat::W[0], W1 = lat::W[1], W2 = lat::W[7];
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i) {
+                const long n = F.at(i, j, k);
+                const double u = u0 ? u0[n] : 0.0, v = v0 ? v0[n] : 0.0, w = w0 ? w0[n] : 0.0;
+                const double rho = 1.0;
+                const double ph = phi[P.at(i, j, k)];
+                const double usq = u * u + v * v + w * w;
+                const double r[2] = {rho * (1.0 + ph) * 0.5, rho * (1.0 - ph) * 0.5};
+                for (int c = 0; c < 2; ++c) {
+                    double* const* d = c == 0 ? f : g;
+                    const double rr = r[c];
+                    auto eq1 = [&](double eu, double uu) { return rr * W1 + rr * W1 * (3.0 * eu + 4.5 * uu * uu - 1.5 * usq); };
+                    auto eq2 = [&](double eu) { return rr * W2 + rr * W2 * (3.0 * eu + 4.5 * eu * eu - 1.5 * usq); };
+                    d[0][n] = rr * W0 + rr * W0 * (-1.5 * usq);
+                    d[1][n] = eq1(u, u);
+                    d[2][n] = eq1(-u, u);
+                    d[3][n] = eq1(v, v);
+                    d[4][n] = eq1(-v, v);
+                    d[5][n] = eq1(w, w);
+                    d[6][n] = eq1(-w, w);
+                    d[7][n] = eq2(u + v);
+                    d[8][n] = eq2(-u + v);
+                    d[9][n] = eq2(u - v);
+                    d[10][n] = eq2(-u - v);
+                    d[11][n] = eq2(u + w);
+                    d[12][n] = eq2(-u + w);
+                    d[13][n] = eq2(u - w);
+                    d[14][n] = eq2(-u - w);
+                    d[15][n] = eq2(v + w);
+                    d[16][n] = eq2(-v + w);
+                    d[17][n] = eq2(v - w);
+                    d[18][n] = eq2(-v - w);
+                }
+            }
+    if (p->outlet_bc == 1 && f_convec && g_convec && phi_convec) {
+        const long plane = (long)(nx + 2) * (ny + 2);
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx + 1; ++i) {
+                const long c = i + (long)j * (nx + 2);
+                for (int q = 0; q < 19; ++q) {
+                    f_convec[c + q * plane] = f[q][F.at(i, j, nz)];
+                    g_convec[c + q * plane] = g[q][F.at(i, j, nz)];
+                }
+                phi_convec[c] = phi[P.at(i, j, nz)];
+            }
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_inlet_profile(const mflbm_params* p, double uin_avg, double la_x, double la_y, double* w_in) {
+    // initialization_open_velocity_inlet_BC (Init_multiphase.F90:194-236) with the rectangular-duct series
+    // of inlet_vel_profile_rectangular (Misc.F90:625-665), 1000 terms.  The series divides by n**5 in
+    // default (32-bit) integer arithmetic, which wraps for n >= 75; the wrapped divisor is reproduced.
+    if (bad_params(p) || !w_in) return MFLBM_ERR_ARG;
+    const int nx = p->nx, ny = p->ny;
+    const int nterms = 1000;
+    const double pi = lat::PI;
+    const double pi5 = std::pow(pi, 5.0);          // the reference's compile-time pi**5
+    const double pim3 = 1.0 / (pi * pi * pi);      // and pi**(-3)
+    for (int j = 0; j <= ny + 1; ++j)
+        for (int i = 0; i <= nx + 1; ++i) w_in[i + (long)j * (nx + 2)] = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            if (i > 1 && i < nx && j > 1 && j < ny) w_in[i + (long)j * (nx + 2)] = uin_avg;
+    const double a = 0.5 * la_x, b = 0.5 * la_y;
+    double tmp1 = 0.0;
+    for (int n = 1; n <= nterms; n += 2) {
+        uint32_t n5 = (uint32_t)n;
+        n5 = n5 * (uint32_t)n * (uint32_t)n * (uint32_t)n * (uint32_t)n;
+        tmp1 = tmp1 + std::tanh(0.5 * (double)n * pi * b / a) / (double)(int32_t)n5;
+    }
+    double tmp2 = 1.0 - 192.0 / pi5 * (a / b) * tmp1;
+    tmp2 = -3.0 * uin_avg / (tmp2 * (a * a));
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const int x = i, y = j;
+            if (x > 1 && x < nx && y > 1 && y < ny) {
+                const double xx = x - 1.5 - a;
+                const double yy = y - 1.5 - b;
+                double tmp3 = 0.0;
+                for (int n = 1; n <= nterms; n += 2) {
+                    const double n3 = (double)(n * n * n);
+                    tmp3 = tmp3 + std::pow(-1.0, 0.5 * (double)(n - 1)) * std::cos(0.5 * n * pi * xx / a) / n3 *
+                                      (1.0 - (std::exp(0.5 * n * pi * (yy - b) / a) + std::exp(0.5 * n * pi * (-yy - b) / a)) /
+                                                 (1.0 + std::exp(0.5 * n * pi * (-b - b) / a)));
+                }
+                w_in[i + (long)j * (nx + 2)] = tmp3 * (-16.0 * tmp2 * (a * a) * pim3);
+            }
+        }
+    return MFLBM_OK;
+}

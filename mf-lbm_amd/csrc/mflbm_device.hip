@@ -1,0 +1,1320 @@
+// gfx950 implementation of MF-LBM's per-time-step lattice update (two-colour D3Q19 MRT, FP64) and its
+// C ABI.  One context owns the device state of one run; all fields share one padded structure-of-arrays
+// grid so a node has the same offset in every array:
+//
+//   offset(i, j, k) = (i + XO) + (j + YO) * px + (k + ZO) * px * py      (1-based interior indices)
+//
+// with XO = 15 so that i = 1 sits on a 128-byte boundary, px = round_up(nx + 20, 16) (row pitch, a
+// multiple of one 128-byte line), py = ny + 8, pz = nz + 8: every array carries at least the 4-layer halo
+// the phase field needs, and the population arrays keep the reference's 1-layer halo semantics inside it.
+// The 38 population arrays (f0..f18 fluid 1, g0..g18 fluid 2) are slices of one allocation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "collide.h"
+#include "lattice.h"
+#include "mflbm.h"
+
+namespace {
+
+constexpr int XO = 15, YO = 3, ZO = 3;
+constexpr int BLOCK = 256;
+
+struct Layout {
+    int nx, ny, nz;
+    int px, pxy;   // row and plane pitch (elements)
+    long total;    // elements per array
+    __host__ __device__ int at(int i, int j, int k) const { return (i + XO) + (j + YO) * px + (k + ZO) * pxy; }
+};
+
+// device view of the state passed to kernels by value
+struct Dev {
+    Layout L;
+    double* f;       // 19 population arrays of fluid 1, stride ps
+    double* g;       // 19 population arrays of fluid 2
+    long ps;
+    double* phi;
+    double* cnx;
+    double* cny;
+    double* cnz;
+    double* cnorm;
+    double* curv;
+    const int8_t* walls;
+    double* w_in;    // (nx+2)(ny+2), reference layout
+    double* cf;      // convective-outlet history f: 19 planes (nx+2)(ny+2)
+    double* cg;
+    double* cphi;    // (nx+2)(ny+2)
+};
+
+__device__ __forceinline__ double* F(const Dev& d, int q) { return d.f + q * d.ps; }
+__device__ __forceinline__ double* Gp(const Dev& d, int q) { return d.g + q * d.ps; }
+__device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
+
+// ------------------------------------------------------------------------------------------------------
+// lattice update: collision + recolouring + AA-pattern streaming of both colours
+// ------------------------------------------------------------------------------------------------------
+
+// ODD = false: kernel_even_color (Kernel_multiphase.F90:371-725): read swapped own slots, write own slots.
+// ODD = true : kernel_odd_color  (Kernel_multiphase.F90:6-362): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
+template <bool ODD>
+__global__ void __launch_bounds__(BLOCK) k_collide(Dev d, CollideParams P) {
+    const Layout L = d.L;
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long nxy = (long)L.nx * L.ny;
+    if (t >= nxy * L.nz) return;
+    const int k = (int)(t / nxy) + 1;
+    const int r = (int)(t - (long)(k - 1) * nxy);
+    const int j = r / L.nx + 1;
+    const int i = r - (j - 1) * L.nx + 1;
+    const int n = L.at(i, j, k);
+    if (d.walls[n] != 0) return;
+    double a[19], b[19];
+    a[0] = F(d, 0)[n];
+    b[0] = Gp(d, 0)[n];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        if (ODD) {
+            const int s = n - (lat::EX[q] + lat::EY[q] * L.px + lat::EZ[q] * L.pxy);
+            a[q] = F(d, q)[s];
+            b[q] = Gp(d, q)[s];
+        } else {
+            a[q] = F(d, lat::OPP[q])[n];
+            b[q] = Gp(d, lat::OPP[q])[n];
+        }
+    }
+    const double phi = collide_node(a, b, d.cnx[n], d.cny[n], d.cnz[n], d.curv[n], d.cnorm[n], P);
+    d.phi[n] = phi;
+    F(d, 0)[n] = a[0];
+    Gp(d, 0)[n] = b[0];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        if (ODD) {
+            const int s = n + (lat::EX[q] + lat::EY[q] * L.px + lat::EZ[q] * L.pxy);
+            F(d, lat::OPP[q])[s] = a[q];
+            Gp(d, lat::OPP[q])[s] = b[q];
+        } else {
+            F(d, q)[n] = a[q];
+            Gp(d, q)[n] = b[q];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// periodic wrap of populations and phi (one rank of the reference's Cartesian MPI exchange, Mpi.F90)
+// ------------------------------------------------------------------------------------------------------
+
+// directions with e_z = +1 / -1 and e_y = +1 / -1 (the five populations per colour a face exchanges)
+__constant__ int ZP[5] = {5, 11, 12, 15, 16};
+__constant__ int ZM[5] = {6, 14, 13, 18, 17};
+__constant__ int YP[5] = {3, 7, 8, 15, 17};
+__constant__ int YM[5] = {4, 10, 9, 16, 18};
+
+__device__ __forceinline__ void copy_both(const Dev& d, int q, int dst, int src) {
+    F(d, q)[dst] = F(d, q)[src];
+    Gp(d, q)[dst] = Gp(d, q)[src];
+}
+
+// after an even step: outgoing boundary-layer populations into the opposite halo
+// (mpi_pdf_pull_async_pack / _sync_update, Mpi.F90:101-346)
+__global__ void k_wrap_pull_z(Dev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        copy_both(d, ZP[m], L.at(i, j, 0), L.at(i, j, L.nz));
+        copy_both(d, ZM[m], L.at(i, j, L.nz + 1), L.at(i, j, 1));
+    }
+}
+__global__ void k_wrap_pull_y(Dev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.nz) return;
+    const int i = t % L.nx + 1, k = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        copy_both(d, YP[m], L.at(i, 0, k), L.at(i, L.ny, k));
+        copy_both(d, YM[m], L.at(i, L.ny + 1, k), L.at(i, 1, k));
+    }
+}
+__global__ void k_wrap_pull_edge(Dev d) {
+    const Layout L = d.L;
+    const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
+    if (i > L.nx) return;
+    copy_both(d, 18, L.at(i, L.ny + 1, L.nz + 1), L.at(i, 1, 1));
+    copy_both(d, 16, L.at(i, L.ny + 1, 0), L.at(i, 1, L.nz));
+    copy_both(d, 17, L.at(i, 0, L.nz + 1), L.at(i, L.ny, 1));
+    copy_both(d, 15, L.at(i, 0, 0), L.at(i, L.ny, L.nz));
+}
+// after an odd step: populations pushed into the halo go back to the periodic image
+// (mpi_pdf_push_async_pack / _sync_update, Mpi.F90:354-598); faces first, x-parallel edges last
+__global__ void k_wrap_push_z(Dev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        copy_both(d, ZM[m], L.at(i, j, 1), L.at(i, j, L.nz + 1));
+        copy_both(d, ZP[m], L.at(i, j, L.nz), L.at(i, j, 0));
+    }
+}
+__global__ void k_wrap_push_y(Dev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.nz) return;
+    const int i = t % L.nx + 1, k = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        copy_both(d, YM[m], L.at(i, 1, k), L.at(i, L.ny + 1, k));
+        copy_both(d, YP[m], L.at(i, L.ny, k), L.at(i, 0, k));
+    }
+}
+__global__ void k_wrap_push_edge(Dev d) {
+    const Layout L = d.L;
+    const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
+    if (i > L.nx) return;
+    copy_both(d, 15, L.at(i, L.ny, L.nz), L.at(i, 0, 0));
+    copy_both(d, 17, L.at(i, L.ny, 1), L.at(i, 0, L.nz + 1));
+    copy_both(d, 16, L.at(i, 1, L.nz), L.at(i, L.ny + 1, 0));
+    copy_both(d, 18, L.at(i, 1, 1), L.at(i, L.ny + 1, L.nz + 1));
+}
+// phi halo of 4 layers in the periodic directions (MPI_phi_async_pack / _sync_update, Mpi.F90:608-867):
+// each halo node of the y-z plane (x interior) takes the value of its periodic image
+__global__ void k_wrap_phi(Dev d, int wrap_y, int wrap_z) {
+    const Layout L = d.L;
+    const int ny8 = L.ny + 8, nz8 = L.nz + 8;
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= (long)L.nx * ny8 * nz8) return;
+    const int i = (int)(t % L.nx) + 1;
+    const int j = (int)((t / L.nx) % ny8) - 3;
+    const int k = (int)(t / ((long)L.nx * ny8)) - 3;
+    const bool gy = j < 1 || j > L.ny, gz = k < 1 || k > L.nz;
+    if (!gy && !gz) return;
+    if (gy && !wrap_y) return;
+    if (gz && !wrap_z) return;
+    const int js = gy ? (j < 1 ? j + L.ny : j - L.ny) : j;
+    const int ks = gz ? (k < 1 ? k + L.nz : k - L.nz) : k;
+    d.phi[L.at(i, j, k)] = d.phi[L.at(i, js, ks)];
+}
+
+// ------------------------------------------------------------------------------------------------------
+// inlet / outlet / porous plate (Boundary_multiphase_inlet.F90, _outlet.F90, _other.F90), one thread per
+// (i, j) of the face; `x * (1 - w) + y * w` blends are kept literally (a solid face node keeps its value)
+// ------------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void set_inlet_phi(const Dev& d, int i, int j, double phi_inlet, int wi) {
+    const Layout L = d.L;
+    double* ph = d.phi;
+    ph[L.at(i, j, 0)] = phi_inlet * (double)(1 - wi) + ph[L.at(i, j, 0)] * (double)wi;
+    const double v = ph[L.at(i, j, 0)];
+    ph[L.at(i, j, -1)] = v;
+    ph[L.at(i, j, -2)] = v;
+    ph[L.at(i, j, -3)] = v;
+}
+
+// inlet_bounce_back_velocity_BC_before_odd (Boundary_multiphase_inlet.F90:6-53)
+__global__ void k_inlet_velocity_before(Dev d, double phi_inlet, double sa_inject, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    set_inlet_phi(d, i, j, phi_inlet, wi);
+    double tmp2 = d.w_in[plane_at(d, i, j)] * relaxation;
+    const double tmp1 = tmp2 * sa_inject;
+    tmp2 = tmp2 - tmp1;
+    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
+    for (int c = 0; c < 2; ++c) {
+        double* A[19];
+        for (int q = 0; q < 19; ++q) A[q] = c == 0 ? F(d, q) : Gp(d, q);
+        const double s = c == 0 ? tmp1 : tmp2;
+        int o;
+        o = L.at(i, j, 0);         A[5][o] = (A[6][L.at(i, j, 1)] + c1 * s) * om + A[5][o] * wd;
+        o = L.at(i - 1, j, 0);     A[11][o] = (A[14][L.at(i, j, 1)] + c2 * s) * om + A[11][o] * wd;
+        o = L.at(i + 1, j, 0);     A[12][o] = (A[13][L.at(i, j, 1)] + c2 * s) * om + A[12][o] * wd;
+        o = L.at(i, j - 1, 0);     A[15][o] = (A[18][L.at(i, j, 1)] + c2 * s) * om + A[15][o] * wd;
+        o = L.at(i, j + 1, 0);     A[16][o] = (A[17][L.at(i, j, 1)] + c2 * s) * om + A[16][o] * wd;
+    }
+}
+// inlet_bounce_back_velocity_BC_after_odd (Boundary_multiphase_inlet.F90:56-102)
+__global__ void k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    set_inlet_phi(d, i, j, phi_inlet, wi);
+    double tmp2 = d.w_in[plane_at(d, i, j)] * relaxation;
+    const double tmp1 = tmp2 * sa_inject;
+    tmp2 = tmp2 - tmp1;
+    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
+    const int o = L.at(i, j, 1);
+    for (int c = 0; c < 2; ++c) {
+        double* A[19];
+        for (int q = 0; q < 19; ++q) A[q] = c == 0 ? F(d, q) : Gp(d, q);
+        const double s = c == 0 ? tmp1 : tmp2;
+        A[6][o] = (A[5][L.at(i, j, 0)] + c1 * s) * om + A[6][o] * wd;
+        A[13][o] = (A[12][L.at(i + 1, j, 0)] + c2 * s) * om + A[13][o] * wd;
+        A[14][o] = (A[11][L.at(i - 1, j, 0)] + c2 * s) * om + A[14][o] * wd;
+        A[17][o] = (A[16][L.at(i, j + 1, 0)] + c2 * s) * om + A[17][o] * wd;
+        A[18][o] = (A[15][L.at(i, j - 1, 0)] + c2 * s) * om + A[18][o] * wd;
+    }
+}
+// inlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_inlet.F90:115-209)
+__global__ void k_inlet_pressure_before(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    set_inlet_phi(d, i, j, phi_inlet, wi);
+    double tmpRho2 = rho_in;
+    const double tmpRho1 = rho_in * sa_inject;
+    tmpRho2 = tmpRho2 - tmpRho1;
+    for (int c = 0; c < 2; ++c) {
+        double* A[19];
+        for (int q = 0; q < 19; ++q) A[q] = c == 0 ? F(d, q) : Gp(d, q);
+        const double rr = c == 0 ? tmpRho1 : tmpRho2;
+        const double s = (rr - (A[0][L.at(i, j, 1)] + A[1][L.at(i - 1, j, 1)] + A[2][L.at(i + 1, j, 1)] +
+                                A[3][L.at(i, j - 1, 1)] + A[4][L.at(i, j + 1, 1)] + A[7][L.at(i - 1, j - 1, 1)] +
+                                A[8][L.at(i + 1, j - 1, 1)] + A[9][L.at(i - 1, j + 1, 1)] + A[10][L.at(i + 1, j + 1, 1)] +
+                                2.0 * (A[6][L.at(i, j, 2)] + A[14][L.at(i + 1, j, 2)] + A[13][L.at(i - 1, j, 2)] +
+                                       A[18][L.at(i, j + 1, 2)] + A[17][L.at(i, j - 1, 2)]))) * relaxation;
+        const double tnx = 0.5 * (A[1][L.at(i - 1, j, 1)] + A[7][L.at(i - 1, j - 1, 1)] + A[9][L.at(i - 1, j + 1, 1)] -
+                                  (A[2][L.at(i + 1, j, 1)] + A[8][L.at(i + 1, j - 1, 1)] + A[10][L.at(i + 1, j + 1, 1)]));
+        const double tny = 0.5 * (A[3][L.at(i, j - 1, 1)] + A[7][L.at(i - 1, j - 1, 1)] + A[8][L.at(i + 1, j - 1, 1)] -
+                                  (A[4][L.at(i, j + 1, 1)] + A[10][L.at(i + 1, j + 1, 1)] + A[9][L.at(i - 1, j + 1, 1)]));
+        int o;
+        o = L.at(i, j, 0);     A[5][o] = (A[6][L.at(i, j, 2)] + lat::ONE3 * s) * om + A[5][o] * wd;
+        o = L.at(i - 1, j, 0); A[11][o] = (A[14][L.at(i + 1, j, 2)] + lat::ONE6 * s - tnx) * om + A[11][o] * wd;
+        o = L.at(i + 1, j, 0); A[12][o] = (A[13][L.at(i - 1, j, 2)] + lat::ONE6 * s + tnx) * om + A[12][o] * wd;
+        o = L.at(i, j - 1, 0); A[15][o] = (A[18][L.at(i, j + 1, 2)] + lat::ONE6 * s - tny) * om + A[15][o] * wd;
+        o = L.at(i, j + 1, 0); A[16][o] = (A[17][L.at(i, j - 1, 2)] + lat::ONE6 * s + tny) * om + A[16][o] * wd;
+    }
+}
+// inlet_Zou_He_pressure_BC_after_odd (Boundary_multiphase_inlet.F90:212-294)
+__global__ void k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    set_inlet_phi(d, i, j, phi_inlet, wi);
+    double tmpRho2 = rho_in;
+    const double tmpRho1 = rho_in * sa_inject;
+    tmpRho2 = tmpRho2 - tmpRho1;
+    const int o = L.at(i, j, 1);
+    for (int c = 0; c < 2; ++c) {
+        double* A[19];
+        for (int q = 0; q < 19; ++q) A[q] = c == 0 ? F(d, q) : Gp(d, q);
+        const double rr = c == 0 ? tmpRho1 : tmpRho2;
+        const double s = (rr - (A[0][o] + A[2][o] + A[1][o] + A[4][o] + A[3][o] + A[8][o] + A[7][o] + A[10][o] + A[9][o] +
+                                2.0 * (A[5][o] + A[11][o] + A[12][o] + A[15][o] + A[16][o]))) * relaxation;
+        const double tnx = 0.5 * (A[2][o] + A[8][o] + A[10][o] - (A[1][o] + A[7][o] + A[9][o]));
+        const double tny = 0.5 * (A[4][o] + A[9][o] + A[10][o] - (A[3][o] + A[8][o] + A[7][o]));
+        A[6][o] = (A[5][o] + lat::ONE3 * s) * om + A[6][o] * wd;
+        A[13][o] = (A[12][o] + lat::ONE6 * s + tnx) * om + A[13][o] * wd;
+        A[14][o] = (A[11][o] + lat::ONE6 * s - tnx) * om + A[14][o] * wd;
+        A[17][o] = (A[16][o] + lat::ONE6 * s + tny) * om + A[17][o] * wd;
+        A[18][o] = (A[15][o] + lat::ONE6 * s - tny) * om + A[18][o] * wd;
+    }
+}
+
+// outlet_convective_BC_before_odd (Boundary_multiphase_outlet.F90:7-65)
+__global__ void k_outlet_convective_before(Dev d, double u_convec) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const double temp = 1.0 / (1.0 + u_convec);
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const int pc = plane_at(d, i, j);
+    const long pl = (long)(L.nx + 2) * (L.ny + 2);
+    double* ph = d.phi;
+    ph[L.at(i, j, nz + 1)] = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    d.cphi[pc] = ph[L.at(i, j, nz + 1)];
+    const double v = ph[L.at(i, j, nz + 1)];
+    ph[L.at(i, j, nz + 2)] = v;
+    ph[L.at(i, j, nz + 3)] = v;
+    ph[L.at(i, j, nz + 4)] = v;
+    // (slot, x offset, y offset) of the five unknown populations at the outlet halo
+    const int QS[5] = {6, 13, 14, 17, 18};
+    const int DI[5] = {0, -1, 1, 0, 0};
+    const int DJ[5] = {0, 0, 0, -1, 1};
+    for (int c = 0; c < 2; ++c) {
+        double* hist = c == 0 ? d.cf : d.cg;
+        for (int m = 0; m < 5; ++m) {
+            double* A = c == 0 ? F(d, QS[m]) : Gp(d, QS[m]);
+            const int o = L.at(i + DI[m], j + DJ[m], nz + 1);
+            const int in = L.at(i + DI[m], j + DJ[m], nz);
+            A[o] = ((hist[pc + QS[m] * pl] + u_convec * A[in]) * temp) * om + A[o] * wd;
+        }
+        for (int m = 0; m < 5; ++m) {
+            double* A = c == 0 ? F(d, QS[m]) : Gp(d, QS[m]);
+            hist[pc + QS[m] * pl] = A[L.at(i + DI[m], j + DJ[m], nz + 1)];
+        }
+    }
+}
+// outlet_convective_BC_after_odd (Boundary_multiphase_outlet.F90:68-127)
+__global__ void k_outlet_convective_after(Dev d, double u_convec) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const double temp = 1.0 / (1.0 + u_convec);
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const int pc = plane_at(d, i, j);
+    const long pl = (long)(L.nx + 2) * (L.ny + 2);
+    double* ph = d.phi;
+    ph[L.at(i, j, nz + 1)] = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    d.cphi[pc] = ph[L.at(i, j, nz + 1)];
+    const double v = ph[L.at(i, j, nz + 1)];
+    ph[L.at(i, j, nz + 2)] = v;
+    ph[L.at(i, j, nz + 3)] = v;
+    ph[L.at(i, j, nz + 4)] = v;
+    // slot written at z = nz and the history slot it relaxes toward
+    const int QW[5] = {5, 11, 12, 15, 16};
+    const int QH[5] = {6, 14, 13, 18, 17};
+    const int o = L.at(i, j, nz), om1 = L.at(i, j, nz - 1);
+    for (int c = 0; c < 2; ++c) {
+        double* hist = c == 0 ? d.cf : d.cg;
+        for (int m = 0; m < 5; ++m) {
+            double* A = c == 0 ? F(d, QW[m]) : Gp(d, QW[m]);
+            A[o] = ((hist[pc + QH[m] * pl] + u_convec * A[om1]) * temp) * om + A[o] * wd;
+        }
+        for (int m = 0; m < 5; ++m) {
+            double* A = c == 0 ? F(d, QW[m]) : Gp(d, QW[m]);
+            hist[pc + QH[m] * pl] = A[o];
+        }
+    }
+}
+// outlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_outlet.F90:139-228)
+__global__ void k_outlet_pressure_before(Dev d, double rho_out) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = 1.0 - (double)wi, wd = (double)wi;
+    double* ph = d.phi;
+    const double pz = ph[L.at(i, j, nz)];
+    ph[L.at(i, j, nz + 1)] = pz;
+    ph[L.at(i, j, nz + 2)] = pz;
+    ph[L.at(i, j, nz + 3)] = pz;
+    ph[L.at(i, j, nz + 4)] = pz;
+    double* f[19];
+    double* g[19];
+    for (int q = 0; q < 19; ++q) {
+        f[q] = F(d, q);
+        g[q] = Gp(d, q);
+    }
+    const int a0 = L.at(i, j, nz), am = L.at(i - 1, j, nz), ap = L.at(i + 1, j, nz), bm = L.at(i, j - 1, nz),
+              bp = L.at(i, j + 1, nz), mm = L.at(i - 1, j - 1, nz), pm = L.at(i + 1, j - 1, nz),
+              mp = L.at(i - 1, j + 1, nz), pp = L.at(i + 1, j + 1, nz);
+    const int c1 = L.at(i, j, nz - 1), c2 = L.at(i - 1, j, nz - 1), c3 = L.at(i + 1, j, nz - 1), c4 = L.at(i, j - 1, nz - 1),
+              c5 = L.at(i, j + 1, nz - 1);
+    double tmp1 = (f[0][a0] + f[1][am] + f[2][ap] + f[3][bm] + f[4][bp] + f[7][mm] + f[8][pm] + f[9][mp] + f[10][pp] +
+                   2.0 * (f[5][c1] + f[11][c2] + f[12][c3] + f[15][c4] + f[16][c5]) + g[0][a0] + g[1][am] + g[2][ap] +
+                   g[3][bm] + g[4][bp] + g[7][mm] + g[8][pm] + g[9][mp] + g[10][pp] +
+                   2.0 * (g[5][c1] + g[11][c2] + g[12][c3] + g[15][c4] + g[16][c5])) - rho_out;
+    const double tmp2 = tmp1 * 0.5 * (1.0 - pz);
+    tmp1 = tmp1 - tmp2;
+    for (int c = 0; c < 2; ++c) {
+        double** A = c == 0 ? f : g;
+        const double s = c == 0 ? tmp1 : tmp2;
+        const double tnx = 0.5 * (A[1][am] + A[7][mm] + A[9][mp] - (A[2][ap] + A[8][pm] + A[10][pp]));
+        const double tny = 0.5 * (A[3][bm] + A[7][mm] + A[8][pm] - (A[4][bp] + A[10][pp] + A[9][mp]));
+        int o;
+        o = L.at(i, j, nz + 1);     A[6][o] = (A[5][c1] - lat::ONE3 * s) * om + A[6][o] * wd;
+        o = L.at(i - 1, j, nz + 1); A[13][o] = (A[12][c3] - lat::ONE6 * s - tnx) * om + A[13][o] * wd;
+        o = L.at(i + 1, j, nz + 1); A[14][o] = (A[11][c2] - lat::ONE6 * s + tnx) * om + A[14][o] * wd;
+        o = L.at(i, j - 1, nz + 1); A[17][o] = (A[16][c5] - lat::ONE6 * s - tny) * om + A[17][o] * wd;
+        o = L.at(i, j + 1, nz + 1); A[18][o] = (A[15][c4] - lat::ONE6 * s + tny) * om + A[18][o] * wd;
+    }
+}
+// outlet_Zou_He_pressure_BC_after_odd (Boundary_multiphase_outlet.F90:231-311)
+__global__ void k_outlet_pressure_after(Dev d, double rho_out) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = 1.0 - (double)wi, wd = (double)wi;
+    double* ph = d.phi;
+    const double pz = ph[L.at(i, j, nz)];
+    ph[L.at(i, j, nz + 1)] = pz;
+    ph[L.at(i, j, nz + 2)] = pz;
+    ph[L.at(i, j, nz + 3)] = pz;
+    ph[L.at(i, j, nz + 4)] = pz;
+    double* f[19];
+    double* g[19];
+    for (int q = 0; q < 19; ++q) {
+        f[q] = F(d, q);
+        g[q] = Gp(d, q);
+    }
+    const int o = L.at(i, j, nz);
+    double tmp1 = (f[0][o] + f[2][o] + f[1][o] + f[4][o] + f[3][o] + f[8][o] + f[7][o] + f[10][o] + f[9][o] +
+                   2.0 * (f[6][o] + f[14][o] + f[13][o] + f[18][o] + f[17][o]) + g[0][o] + g[2][o] + g[1][o] + g[4][o] +
+                   g[3][o] + g[8][o] + g[7][o] + g[10][o] + g[9][o] +
+                   2.0 * (g[6][o] + g[14][o] + g[13][o] + g[18][o] + g[17][o])) - rho_out;
+    const double tmp2 = tmp1 * 0.5 * (1.0 - pz);
+    tmp1 = tmp1 - tmp2;
+    for (int c = 0; c < 2; ++c) {
+        double** A = c == 0 ? f : g;
+        const double s = c == 0 ? tmp1 : tmp2;
+        const double tnx = 0.5 * (A[2][o] + A[8][o] + A[10][o] - (A[1][o] + A[7][o] + A[9][o]));
+        const double tny = 0.5 * (A[4][o] + A[10][o] + A[9][o] - (A[3][o] + A[7][o] + A[8][o]));
+        A[5][o] = (A[6][o] - lat::ONE3 * s) * om + A[5][o] * wd;
+        A[11][o] = (A[14][o] - lat::ONE6 * s + tnx) * om + A[11][o] * wd;
+        A[12][o] = (A[13][o] - lat::ONE6 * s - tnx) * om + A[12][o] * wd;
+        A[15][o] = (A[18][o] - lat::ONE6 * s + tny) * om + A[15][o] * wd;
+        A[16][o] = (A[17][o] - lat::ONE6 * s - tny) * om + A[16][o] * wd;
+    }
+}
+
+// porous_plate_BC_before_odd / _after_odd (Boundary_multiphase_other.F90:8-184); `blocked` is the colour
+// reflected by the plate (0 = fluid 1, 1 = fluid 2), the other colour passes straight through
+__global__ void k_porous_plate(Dev d, int zp, int blocked, int after) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    double* B[19];
+    double* P[19];
+    for (int q = 0; q < 19; ++q) {
+        B[q] = blocked == 0 ? F(d, q) : Gp(d, q);
+        P[q] = blocked == 0 ? Gp(d, q) : F(d, q);
+    }
+    if (!after) {
+        B[6][L.at(i, j, zp)] = B[5][L.at(i, j, zp - 1)];
+        B[13][L.at(i - 1, j, zp)] = B[12][L.at(i, j, zp - 1)];
+        B[14][L.at(i + 1, j, zp)] = B[11][L.at(i, j, zp - 1)];
+        B[17][L.at(i, j - 1, zp)] = B[16][L.at(i, j, zp - 1)];
+        B[18][L.at(i, j + 1, zp)] = B[15][L.at(i, j, zp - 1)];
+        B[5][L.at(i, j, zp)] = B[6][L.at(i, j, zp + 1)];
+        B[12][L.at(i + 1, j, zp)] = B[13][L.at(i, j, zp + 1)];
+        B[11][L.at(i - 1, j, zp)] = B[14][L.at(i, j, zp + 1)];
+        B[16][L.at(i, j + 1, zp)] = B[17][L.at(i, j, zp + 1)];
+        B[15][L.at(i, j - 1, zp)] = B[18][L.at(i, j, zp + 1)];
+        const int QU[5] = {6, 13, 14, 17, 18}, QD[5] = {5, 12, 11, 16, 15};
+        for (int m = 0; m < 5; ++m) P[QU[m]][L.at(i, j, zp)] = P[QU[m]][L.at(i, j, zp + 1)];
+        for (int m = 0; m < 5; ++m) P[QD[m]][L.at(i, j, zp)] = P[QD[m]][L.at(i, j, zp - 1)];
+    } else {
+        B[5][L.at(i, j, zp - 1)] = B[6][L.at(i, j, zp)];
+        B[11][L.at(i, j, zp - 1)] = B[14][L.at(i + 1, j, zp)];
+        B[12][L.at(i, j, zp - 1)] = B[13][L.at(i - 1, j, zp)];
+        B[15][L.at(i, j, zp - 1)] = B[18][L.at(i, j + 1, zp)];
+        B[16][L.at(i, j, zp - 1)] = B[17][L.at(i, j - 1, zp)];
+        B[6][L.at(i, j, zp + 1)] = B[5][L.at(i, j, zp)];
+        B[14][L.at(i, j, zp + 1)] = B[11][L.at(i - 1, j, zp)];
+        B[13][L.at(i, j, zp + 1)] = B[12][L.at(i + 1, j, zp)];
+        B[18][L.at(i, j, zp + 1)] = B[15][L.at(i, j - 1, zp)];
+        B[17][L.at(i, j, zp + 1)] = B[16][L.at(i, j + 1, zp)];
+        const int QD[5] = {5, 11, 12, 15, 16}, QU[5] = {6, 14, 13, 18, 17};
+        for (int m = 0; m < 5; ++m) P[QD[m]][L.at(i, j, zp - 1)] = P[QD[m]][L.at(i, j, zp)];
+        for (int m = 0; m < 5; ++m) P[QU[m]][L.at(i, j, zp + 1)] = P[QU[m]][L.at(i, j, zp)];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// colour-gradient chain (color_gradient, Phase_gradient.F90:5-204)
+// ------------------------------------------------------------------------------------------------------
+
+struct SolidList {
+    int n;
+    const int* ijk;      // 3 per node
+    const int* nfl;
+    const int* nbr;      // 18 per node
+    const double* w;
+};
+struct FluidList {
+    int n;
+    const int* ijk;
+    const double* nw;    // 3 per node
+    const double* cos_t;
+    const double* sin_t;
+};
+
+// ISO4 first derivatives with the reference's term order (Phase_gradient.F90:40-62)
+__device__ __forceinline__ double dx4(const double* A, int n, int px, int pxy) {
+    return lat::ISO4_1 * (A[n + 1] - A[n - 1]) +
+           lat::ISO4_2 * (A[n + 1 + px] - A[n - 1 - px] + A[n + 1 - px] - A[n - 1 + px] + A[n + 1 + pxy] - A[n - 1 - pxy] +
+                          A[n + 1 - pxy] - A[n - 1 + pxy]);
+}
+__device__ __forceinline__ double dy4(const double* A, int n, int px, int pxy) {
+    return lat::ISO4_1 * (A[n + px] - A[n - px]) +
+           lat::ISO4_2 * (A[n + 1 + px] - A[n - 1 - px] + A[n - 1 + px] - A[n + 1 - px] + A[n + px + pxy] - A[n - px - pxy] +
+                          A[n + px - pxy] - A[n - px + pxy]);
+}
+__device__ __forceinline__ double dz4(const double* A, int n, int px, int pxy) {
+    return lat::ISO4_1 * (A[n + pxy] - A[n - pxy]) +
+           lat::ISO4_2 * (A[n + 1 + pxy] - A[n - 1 - pxy] + A[n - 1 + pxy] - A[n + 1 - pxy] + A[n + px + pxy] - A[n - px - pxy] +
+                          A[n - px + pxy] - A[n + px - pxy]);
+}
+
+// phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30)
+__global__ void k_phi_extrapolate(Dev d, SolidList s) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= s.n) return;
+    const Layout L = d.L;
+    const int i = s.ijk[3 * t], j = s.ijk[3 * t + 1], k = s.ijk[3 * t + 2];
+    const int n = L.at(i, j, k);
+    double acc = 0.0;
+    const int nf = s.nfl[t];
+    for (int m = 0; m < nf; ++m) {
+        const int q = s.nbr[18 * t + m];
+        acc = acc + d.phi[n + lat::EX[q] + lat::EY[q] * L.px + lat::EZ[q] * L.pxy] * lat::W[q];
+    }
+    d.phi[n] = acc / s.w[t];
+}
+// interface normal and |grad phi| over [-1, n+2]^3 (Phase_gradient.F90:33-79)
+__global__ void k_normals(Dev d) {
+    const Layout L = d.L;
+    const int ex = L.nx + 4, ey = L.ny + 4;
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= (long)ex * ey * (L.nz + 4)) return;
+    const int i = (int)(t % ex) - 1;
+    const int j = (int)((t / ex) % ey) - 1;
+    const int k = (int)(t / ((long)ex * ey)) - 1;
+    const int n = L.at(i, j, k);
+    double cx = dx4(d.phi, n, L.px, L.pxy);
+    double cy = dy4(d.phi, n, L.px, L.pxy);
+    double cz = dz4(d.phi, n, L.px, L.pxy);
+    double cn = sqrt(cx * cx + cy * cy + cz * cz);
+    if (cn < 1e-6 || d.walls[n] == 1) {
+        cx = 0.0;
+        cy = 0.0;
+        cz = 0.0;
+        cn = 0.0;
+    } else {
+        cx = cx / cn;
+        cy = cy / cn;
+        cz = cz / cn;
+    }
+    d.cnx[n] = cx;
+    d.cny[n] = cy;
+    d.cnz[n] = cz;
+    d.cnorm[n] = cn;
+}
+// wetting: turn the normal at fluid nodes next to the wall to the contact angle (Akai et al. 2018;
+// alter_color_gradient_solid_surface, Phase_gradient.F90:210-265).  cos/sin of the angle come from the host.
+__global__ void k_wetting(Dev d, FluidList fl) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= fl.n) return;
+    const Layout L = d.L;
+    const int n = L.at(fl.ijk[3 * t], fl.ijk[3 * t + 1], fl.ijk[3 * t + 2]);
+    if (!(d.cnorm[n] > 1e-6)) return;
+    const double nwx = fl.nw[3 * t], nwy = fl.nw[3 * t + 1], nwz = fl.nw[3 * t + 2];
+    const double cnx0 = d.cnx[n], cny0 = d.cny[n], cnz0 = d.cnz[n];
+    const double tmpCos = fl.cos_t[t], sinT = fl.sin_t[t];
+    const double tmp1 = nwx * cnx0 + nwy * cny0 + nwz * cnz0;
+    const double tmp2 = 1.0 / sqrt(1.0 - tmp1 * tmp1);
+    const double coe1 = sinT * tmp1 * tmp2;
+    const double coe2 = sinT * tmp2;
+    const double cnxp = (tmpCos - coe1) * nwx + coe2 * cnx0;
+    const double cnyp = (tmpCos - coe1) * nwy + coe2 * cny0;
+    const double cnzp = (tmpCos - coe1) * nwz + coe2 * cnz0;
+    const double cnxm = (tmpCos + coe1) * nwx - coe2 * cnx0;
+    const double cnym = (tmpCos + coe1) * nwy - coe2 * cny0;
+    const double cnzm = (tmpCos + coe1) * nwz - coe2 * cnz0;
+    const double distP = (cnxp - cnx0) * (cnxp - cnx0) + (cnyp - cny0) * (cnyp - cny0) + (cnzp - cnz0) * (cnzp - cnz0);
+    const double distM = (cnxm - cnx0) * (cnxm - cnx0) + (cnym - cny0) * (cnym - cny0) + (cnzm - cnz0) * (cnzm - cnz0);
+    if (distP <= distM) {
+        d.cnx[n] = cnxp;
+        d.cny[n] = cnyp;
+        d.cnz[n] = cnzp;
+    } else {
+        d.cnx[n] = cnxm;
+        d.cny[n] = cnym;
+        d.cnz[n] = cnzm;
+    }
+}
+// normal extrapolated into solid boundary nodes within one halo layer (Phase_gradient.F90:84-110)
+__global__ void k_normal_extrapolate(Dev d, SolidList s) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= s.n) return;
+    const Layout L = d.L;
+    const int i = s.ijk[3 * t], j = s.ijk[3 * t + 1], k = s.ijk[3 * t + 2];
+    if (!(i >= 0 && i <= L.nx + 1 && j >= 0 && j <= L.ny + 1 && k >= 0 && k <= L.nz + 1)) return;
+    const int n = L.at(i, j, k);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int nf = s.nfl[t];
+    for (int m = 0; m < nf; ++m) {
+        const int q = s.nbr[18 * t + m];
+        const int o = n + lat::EX[q] + lat::EY[q] * L.px + lat::EZ[q] * L.pxy;
+        ax = ax + d.cnx[o] * lat::W[q];
+        ay = ay + d.cny[o] * lat::W[q];
+        az = az + d.cnz[o] * lat::W[q];
+    }
+    d.cnx[n] = ax / s.w[t];
+    d.cny[n] = ay / s.w[t];
+    d.cnz[n] = az / s.w[t];
+}
+// interface curvature over the interior (Phase_gradient.F90:112-201)
+__global__ void __launch_bounds__(BLOCK) k_curvature(Dev d) {
+    const Layout L = d.L;
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long nxy = (long)L.nx * L.ny;
+    if (t >= nxy * L.nz) return;
+    const int k = (int)(t / nxy) + 1;
+    const int r = (int)(t - (long)(k - 1) * nxy);
+    const int j = r / L.nx + 1;
+    const int i = r - (j - 1) * L.nx + 1;
+    const int n = L.at(i, j, k);
+    const int px = L.px, pxy = L.pxy;
+    const double kxx = dx4(d.cnx, n, px, pxy);
+    const double kyy = dy4(d.cny, n, px, pxy);
+    const double kzz = dz4(d.cnz, n, px, pxy);
+    const double kxy = dy4(d.cnx, n, px, pxy);
+    const double kxz = dz4(d.cnx, n, px, pxy);
+    const double kyx = dx4(d.cny, n, px, pxy);
+    const double kyz = dz4(d.cny, n, px, pxy);
+    const double kzx = dx4(d.cnz, n, px, pxy);
+    const double kzy = dy4(d.cnz, n, px, pxy);
+    const double cx = d.cnx[n], cy = d.cny[n], cz = d.cnz[n];
+    d.curv[n] = (cx * cx - 1.0) * kxx + (cy * cy - 1.0) * kyy + (cz * cz - 1.0) * kzz + cx * cy * (kxy + kyx) +
+                cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// macroscopic fields and monitor reductions (compute_macro_vars Misc.F90:372-430, Monitor.F90)
+// ------------------------------------------------------------------------------------------------------
+
+__global__ void k_macro(Dev d, double gamma, double force_z, double* u, double* v, double* w, double* rho) {
+    const Layout L = d.L;
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long nxy = (long)L.nx * L.ny;
+    if (t >= nxy * L.nz) return;
+    const int k = (int)(t / nxy) + 1;
+    const int r = (int)(t - (long)(k - 1) * nxy);
+    const int j = r / L.nx + 1;
+    const int i = r - (j - 1) * L.nx + 1;
+    const int n = L.at(i, j, k);
+    const int wi = d.walls[n];
+    const double om = (double)(1 - wi);
+    double ft[19];
+    for (int q = 0; q < 19; ++q) ft[q] = F(d, q)[n] + Gp(d, q)[n];
+    rho[n] = (ft[0] + ft[1] + ft[2] + ft[3] + ft[4] + ft[5] + ft[6] + ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] +
+              ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18]) * om;
+    const double tmp = 0.5 * gamma * d.curv[n] * d.cnorm[n];
+    const double fx = tmp * d.cnx[n];
+    const double fy = tmp * d.cny[n];
+    const double fz = tmp * d.cnz[n] + force_z;
+    u[n] = (ft[1] - ft[2] + ft[7] - ft[8] + ft[9] - ft[10] + ft[11] - ft[12] + ft[13] - ft[14] - 0.5 * fx) * om;
+    v[n] = (ft[3] - ft[4] + ft[7] + ft[8] - ft[9] - ft[10] + ft[15] - ft[16] + ft[17] - ft[18] - 0.5 * fy) * om;
+    w[n] = (ft[5] - ft[6] + ft[11] + ft[12] - ft[13] - ft[14] + ft[15] + ft[16] - ft[17] - ft[18] - 0.5 * fz) * om;
+    d.phi[n] = 0.0 * (double)wi + d.phi[n] * om;
+}
+
+// one block per z layer: the seven layer sums of Monitor.F90:62-84 and per-layer partials of the global
+// scalars, reduced in a fixed tree (deterministic run to run)
+constexpr int NPART = 12;
+__global__ void __launch_bounds__(BLOCK) k_monitor_layers(Dev d, const double* u, const double* v, const double* w,
+                                                           const double* rho, const double* phi_old, double* layers,
+                                                           double* part) {
+    const Layout L = d.L;
+    const int k = blockIdx.x + 1;
+    double s[NPART];
+    for (int m = 0; m < NPART; ++m) s[m] = 0.0;
+    double umax = 0.0, dphi = 0.0;
+    const int nxy = L.nx * L.ny;
+    for (int r = threadIdx.x; r < nxy; r += BLOCK) {
+        const int j = r / L.nx + 1, i = r % L.nx + 1;
+        const int n = L.at(i, j, k);
+        const int wi = d.walls[n];
+        const double om = (double)(1 - wi);
+        const double ph = d.phi[n];
+        const double temp = (u[n] * u[n] + v[n] * v[n] + w[n] * w[n]) * om;
+        umax = fmax(umax, temp);
+        if (ph > 0.999) s[7] += temp;
+        else if (ph < -0.999) s[8] += temp;
+        s[2] += 0.5 * (1.0 + ph) * om;
+        s[3] += 0.5 * (1.0 - ph) * om;
+        s[4] += rho[n] * 0.5 * (1.0 + ph) * om;
+        s[5] += rho[n] * 0.5 * (1.0 - ph) * om;
+        s[0] += w[n] * 0.5 * (1.0 + ph) * om;
+        s[1] += w[n] * 0.5 * (1.0 - ph) * om;
+        s[6] += rho[n] * om;
+        if (wi == 0) {
+            if (ph < -0.99) { s[9] += rho[n]; s[10] += 1.0; }
+            if (ph > 0.99) { s[11] += rho[n]; }
+            if (phi_old) dphi = fmax(dphi, fabs(ph - phi_old[n]) * om);
+        }
+    }
+    __shared__ double sh[BLOCK];
+    double out[NPART + 3];
+    for (int m = 0; m < NPART + 3; ++m) {
+        double x = m < NPART ? s[m] : (m == NPART ? umax : (m == NPART + 1 ? dphi : 0.0));
+        if (m == NPART + 2) {   // count of fluid-1 nodes (phi > 0.99) for the capillary-pressure monitor
+            double c = 0.0;
+            for (int r = threadIdx.x; r < nxy; r += BLOCK) {
+                const int n = L.at(r % L.nx + 1, r / L.nx + 1, k);
+                if (d.walls[n] == 0 && d.phi[n] > 0.99) c += 1.0;
+            }
+            x = c;
+        }
+        sh[threadIdx.x] = x;
+        __syncthreads();
+        for (int off = BLOCK / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) {
+                const double o = sh[threadIdx.x + off];
+                sh[threadIdx.x] = (m == NPART || m == NPART + 1) ? fmax(sh[threadIdx.x], o) : sh[threadIdx.x] + o;
+            }
+            __syncthreads();
+        }
+        out[m] = sh[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int m = 0; m < 7; ++m) layers[m * L.nz + (k - 1)] = out[m];
+        for (int m = 7; m < NPART + 3; ++m) part[(m - 7) * L.nz + (k - 1)] = out[m];
+    }
+}
+__global__ void k_copy_phi_old(Dev d, double* phi_old) {
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= d.L.total) return;
+    phi_old[t] = d.phi[t];
+}
+
+// ------------------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------------------
+
+inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct mflbm_ctx {
+    mflbm_params p{};
+    Layout L{};
+    Dev d{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    double* pdf = nullptr;
+    double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
+    int8_t* walls = nullptr;
+    double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
+    double* u = nullptr;
+    double* v = nullptr;
+    double* w = nullptr;
+    double* rho = nullptr;
+    double* phi_old = nullptr;
+    double* red = nullptr;      // monitor layer sums and partials
+    int* s_i = nullptr;
+    double* s_d = nullptr;
+    int* f_i = nullptr;
+    double* f_d = nullptr;
+    SolidList sl{};
+    FluidList fl{};
+    bool have_geom = false, have_state = false, phi_old_valid = false;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;   // pool of 4 events per timed step: start, after collide, before chain, end
+    size_t ev_used = 0;
+    double t_sum[3] = {0, 0, 0};
+    long t_cnt[3] = {0, 0, 0};
+    std::string err;
+};
+
+namespace {
+
+int fail(mflbm_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_create_error = buf;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                                      \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH(c, kern, grid, ...)                                                                           \
+    do {                                                                                                     \
+        if ((grid) > 0) {                                                                                    \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, (c)->stream, __VA_ARGS__);                  \
+            hipError_t e_ = hipGetLastError();                                                               \
+            if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(e_)); \
+        }                                                                                                    \
+    } while (0)
+
+CollideParams collide_params(const mflbm_params& p) {
+    return CollideParams{p.gamma, p.beta, p.force_z, p.la_nui1, p.la_nui2};
+}
+
+bool open_z(const mflbm_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
+
+// copy between the reference's halo-h layout (host) and the padded device layout (one array)
+template <typename T>
+void pack_host(const Layout& L, int h, const T* src, T* dst_padded) {
+    const int sx = L.nx + 2 * h, sy = L.ny + 2 * h;
+    for (int k = 1 - h; k <= L.nz + h; ++k)
+        for (int j = 1 - h; j <= L.ny + h; ++j) {
+            const T* s = src + (long)(j + h - 1) * sx + (long)(k + h - 1) * sx * sy;
+            std::memcpy(dst_padded + L.at(1 - h, j, k), s, sizeof(T) * sx);
+        }
+}
+template <typename T>
+void unpack_host(const Layout& L, int h, const T* src_padded, T* dst) {
+    const int sx = L.nx + 2 * h, sy = L.ny + 2 * h;
+    for (int k = 1 - h; k <= L.nz + h; ++k)
+        for (int j = 1 - h; j <= L.ny + h; ++j) {
+            T* d = dst + (long)(j + h - 1) * sx + (long)(k + h - 1) * sx * sy;
+            std::memcpy(d, src_padded + L.at(1 - h, j, k), sizeof(T) * sx);
+        }
+}
+
+int launch_gradient(mflbm_ctx* c) {
+    const Layout& L = c->L;
+    LAUNCH(c, k_phi_extrapolate, nblocks(c->sl.n), c->d, c->sl);
+    LAUNCH(c, k_normals, nblocks((long)(L.nx + 4) * (L.ny + 4) * (L.nz + 4)), c->d);
+    LAUNCH(c, k_wetting, nblocks(c->fl.n), c->d, c->fl);
+    LAUNCH(c, k_normal_extrapolate, nblocks(c->sl.n), c->d, c->sl);
+    LAUNCH(c, k_curvature, nblocks((long)L.nx * L.ny * L.nz), c->d);
+    return MFLBM_OK;
+}
+
+int launch_step(mflbm_ctx* c, int ntime) {
+    const mflbm_params& p = c->p;
+    const Layout& L = c->L;
+    const CollideParams P = collide_params(p);
+    const long nint = (long)L.nx * L.ny * L.nz;
+    const bool odd = (ntime % 2) != 0;
+    const bool t = c->timing;
+    hipEvent_t* ev = nullptr;
+    if (t) {
+        if (c->ev_used + 4 > c->ev.size()) {
+            for (int m = 0; m < 4; ++m) {
+                hipEvent_t e;
+                HIPCHK(c, hipEventCreate(&e));
+                c->ev.push_back(e);
+            }
+        }
+        ev = &c->ev[c->ev_used];
+        c->ev_used += 4;
+        HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    }
+    if (odd) LAUNCH(c, k_collide<true>, nblocks(nint), c->d, P);
+    else LAUNCH(c, k_collide<false>, nblocks(nint), c->d, P);
+    if (t) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    // periodic wrap (the reference's MPI exchange of one rank with itself)
+    if (p.kper == 1 || p.jper == 1) {
+        if (!odd) {
+            if (p.kper == 1) LAUNCH(c, k_wrap_pull_z, nblocks((long)L.nx * L.ny), c->d);
+            if (p.jper == 1) LAUNCH(c, k_wrap_pull_y, nblocks((long)L.nx * L.nz), c->d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_pull_edge, nblocks(L.nx), c->d);
+        } else {
+            if (p.kper == 1) LAUNCH(c, k_wrap_push_z, nblocks((long)L.nx * L.ny), c->d);
+            if (p.jper == 1) LAUNCH(c, k_wrap_push_y, nblocks((long)L.nx * L.nz), c->d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_push_edge, nblocks(L.nx), c->d);
+        }
+        LAUNCH(c, k_wrap_phi, nblocks((long)L.nx * (L.ny + 8) * (L.nz + 8)), c->d, p.jper, p.kper);
+    }
+    const unsigned gface = nblocks((long)L.nx * L.ny);
+    if (open_z(p)) {
+        if (!odd) {
+            if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_before, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_before, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_outlet_convective_before, gface, c->d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_outlet_pressure_before, gface, c->d, p.rho_out);
+        } else {
+            if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_after, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_after, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_outlet_convective_after, gface, c->d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_outlet_pressure_after, gface, c->d, p.rho_out);
+        }
+    }
+    if (p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2) {
+        const int zp = p.z_porous_plate;
+        if (zp >= 1 && zp <= L.nz) LAUNCH(c, k_porous_plate, gface, c->d, zp, p.porous_plate_cmd - 1, odd ? 1 : 0);
+    }
+    if (t) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    int rc = launch_gradient(c);
+    if (rc) return rc;
+    if (t) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    return MFLBM_OK;
+}
+
+}  // namespace
+
+extern "C" const char* mflbm_create_error(void) { return g_create_error.c_str(); }
+extern "C" const char* mflbm_last_error(const mflbm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
+    if (!out) return fail(nullptr, MFLBM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!prm) return fail(nullptr, MFLBM_ERR_ARG, "params is NULL");
+    if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return fail(nullptr, MFLBM_ERR_ARG, "lattice must be at least 3^3");
+    if (prm->wall_x_min != 1 || prm->wall_x_max != 1)
+        return fail(nullptr, MFLBM_ERR_ARG, "x needs solid walls at both ends (the reference rejects anything else)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MFLBM_ERR_NODEV, "no HIP device visible");
+    mflbm_ctx* c = new (std::nothrow) mflbm_ctx();
+    if (!c) return fail(nullptr, MFLBM_ERR_NOMEM, "context allocation failed");
+    c->p = *prm;
+    Layout& L = c->L;
+    L.nx = prm->nx;
+    L.ny = prm->ny;
+    L.nz = prm->nz;
+    L.px = ((prm->nx + 20 + 15) / 16) * 16;
+    L.pxy = L.px * (prm->ny + 8);
+    L.total = (long)L.pxy * (prm->nz + 8);
+    if (L.total > (1L << 31) - 1) {
+        delete c;
+        return fail(nullptr, MFLBM_ERR_ARG, "lattice too large for 32-bit node offsets");
+    }
+    const long ps = ((L.total + 31) / 32) * 32;   // 256-byte aligned array stride
+    auto bail = [&](int code) {
+        std::string m = c->err;
+        mflbm_destroy(c);
+        return fail(nullptr, code, "%s", m.c_str());
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(c, MFLBM_ERR_HIP, "hipGetDevice failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(c, MFLBM_ERR_HIP, "hipStreamCreate failed"));
+    const long plane = (long)(L.nx + 2) * (L.ny + 2);
+    if (hipMalloc(&c->pdf, sizeof(double) * ps * 38) != hipSuccess ||
+        hipMalloc(&c->fields, sizeof(double) * ps * 11) != hipSuccess ||
+        hipMalloc(&c->walls, (size_t)ps) != hipSuccess ||
+        hipMalloc(&c->planes, sizeof(double) * plane * 40) != hipSuccess ||
+        hipMalloc(&c->red, sizeof(double) * (size_t)(L.nz + 1) * 32) != hipSuccess)
+        return bail(fail(c, MFLBM_ERR_NOMEM, "device allocation of %.2f GB failed",
+                         (38.0 + 11.0) * 8.0 * ps / 1e9));
+    if (hipMemset(c->pdf, 0, sizeof(double) * ps * 38) != hipSuccess ||
+        hipMemset(c->fields, 0, sizeof(double) * ps * 11) != hipSuccess ||
+        hipMemset(c->walls, 1, (size_t)ps) != hipSuccess || hipMemset(c->planes, 0, sizeof(double) * plane * 40) != hipSuccess)
+        return bail(fail(c, MFLBM_ERR_HIP, "hipMemset failed"));
+    Dev& d = c->d;
+    d.L = L;
+    d.f = c->pdf;
+    d.g = c->pdf + 19 * ps;
+    d.ps = ps;
+    d.phi = c->fields;
+    d.cnx = c->fields + ps;
+    d.cny = c->fields + 2 * ps;
+    d.cnz = c->fields + 3 * ps;
+    d.cnorm = c->fields + 4 * ps;
+    d.curv = c->fields + 5 * ps;
+    c->u = c->fields + 6 * ps;
+    c->v = c->fields + 7 * ps;
+    c->w = c->fields + 8 * ps;
+    c->rho = c->fields + 9 * ps;
+    c->phi_old = c->fields + 10 * ps;
+    d.walls = c->walls;
+    d.w_in = c->planes;
+    d.cphi = c->planes + plane;
+    d.cf = c->planes + 2 * plane;
+    d.cg = c->planes + 21 * plane;
+    *out = c;
+    return MFLBM_OK;
+}
+
+extern "C" void mflbm_destroy(mflbm_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    void* bufs[] = {c->pdf, c->fields, c->walls, c->planes, c->red, c->s_i, c->s_d, c->f_i, c->f_d};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int mflbm_set_scalars(mflbm_ctx* c, double force_z, double rho_in, double uin_avg, double relaxation) {
+    if (!c) return MFLBM_ERR_ARG;
+    c->p.force_z = force_z;
+    c->p.rho_in = rho_in;
+    c->p.uin_avg = uin_avg;
+    c->p.relaxation = relaxation;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32_t ns, const int32_t* s_ijk,
+                                     const int32_t* s_nfl, const int32_t* s_nbr, const double* s_w, int32_t nf,
+                                     const int32_t* f_ijk, const double* f_nw, const double* f_theta, const double* w_in) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!walls_h2 || ns < 0 || nf < 0 || (ns > 0 && (!s_ijk || !s_nfl || !s_nbr || !s_w)) ||
+        (nf > 0 && (!f_ijk || !f_nw || !f_theta)))
+        return fail(c, MFLBM_ERR_ARG, "missing geometry array");
+    const Layout& L = c->L;
+    // bounds of the list coordinates against the ranges the kernels index (phi halo 4)
+    for (int t = 0; t < ns; ++t) {
+        const int i = s_ijk[3 * t], j = s_ijk[3 * t + 1], k = s_ijk[3 * t + 2];
+        if (i < -2 || i > L.nx + 3 || j < -2 || j > L.ny + 3 || k < -2 || k > L.nz + 3 || s_nfl[t] < 0 || s_nfl[t] > 18)
+            return fail(c, MFLBM_ERR_ARG, "solid boundary node %d out of range", t);
+        for (int m = 0; m < s_nfl[t]; ++m)
+            if (s_nbr[18 * t + m] < 1 || s_nbr[18 * t + m] > 18)
+                return fail(c, MFLBM_ERR_ARG, "solid boundary node %d has bad neighbour direction", t);
+    }
+    for (int t = 0; t < nf; ++t) {
+        const int i = f_ijk[3 * t], j = f_ijk[3 * t + 1], k = f_ijk[3 * t + 2];
+        if (i < -1 || i > L.nx + 2 || j < -1 || j > L.ny + 2 || k < -1 || k > L.nz + 2)
+            return fail(c, MFLBM_ERR_ARG, "fluid boundary node %d out of range", t);
+    }
+    std::vector<int8_t> wp(L.total, 1);
+    pack_host(L, 2, walls_h2, wp.data());
+    HIPCHK(c, hipMemcpy(c->walls, wp.data(), (size_t)L.total, hipMemcpyHostToDevice));
+    if (c->s_i) { (void)hipFree(c->s_i); c->s_i = nullptr; }
+    if (c->s_d) { (void)hipFree(c->s_d); c->s_d = nullptr; }
+    if (c->f_i) { (void)hipFree(c->f_i); c->f_i = nullptr; }
+    if (c->f_d) { (void)hipFree(c->f_d); c->f_d = nullptr; }
+    const size_t nsi = (size_t)ns * 22 + 1, nfi = (size_t)nf * 3 + 1;
+    std::vector<int> si(nsi, 0), fi(nfi, 0);
+    std::vector<double> sd((size_t)ns + 1, 1.0), fd((size_t)nf * 5 + 1, 0.0);
+    for (int t = 0; t < ns; ++t) {
+        for (int m = 0; m < 3; ++m) si[3 * t + m] = s_ijk[3 * t + m];
+        si[3 * (size_t)ns + t] = s_nfl[t];
+        for (int m = 0; m < 18; ++m) si[4 * (size_t)ns + 18 * t + m] = m < s_nfl[t] ? s_nbr[18 * t + m] : 0;
+        sd[t] = s_w[t];
+    }
+    for (int t = 0; t < nf; ++t) {
+        for (int m = 0; m < 3; ++m) {
+            fi[3 * t + m] = f_ijk[3 * t + m];
+            fd[3 * t + m] = f_nw[3 * t + m];
+        }
+        // the reference evaluates dcos/dsin of the node's angle on the host CPU; doing it here keeps the
+        // device free of a second math library's rounding
+        fd[3 * (size_t)nf + t] = std::cos(f_theta[t]);
+        fd[4 * (size_t)nf + t] = std::sin(f_theta[t]);
+    }
+    HIPCHK(c, hipMalloc(&c->s_i, sizeof(int) * nsi));
+    HIPCHK(c, hipMalloc(&c->s_d, sizeof(double) * sd.size()));
+    HIPCHK(c, hipMalloc(&c->f_i, sizeof(int) * nfi));
+    HIPCHK(c, hipMalloc(&c->f_d, sizeof(double) * fd.size()));
+    HIPCHK(c, hipMemcpy(c->s_i, si.data(), sizeof(int) * nsi, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->s_d, sd.data(), sizeof(double) * sd.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->f_i, fi.data(), sizeof(int) * nfi, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->f_d, fd.data(), sizeof(double) * fd.size(), hipMemcpyHostToDevice));
+    c->sl = SolidList{ns, c->s_i, c->s_i + 3 * (size_t)ns, c->s_i + 4 * (size_t)ns, c->s_d};
+    c->fl = FluidList{nf, c->f_i, c->f_d, c->f_d + 3 * (size_t)nf, c->f_d + 4 * (size_t)nf};
+    if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
+    c->have_geom = true;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_upload_state(mflbm_ctx* c, const double* const* f, const double* const* g, const double* phi,
+                                  const double* f_convec, const double* g_convec, const double* phi_convec) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!f || !g || !phi) return fail(c, MFLBM_ERR_ARG, "missing state array");
+    for (int q = 0; q < 19; ++q)
+        if (!f[q] || !g[q]) return fail(c, MFLBM_ERR_ARG, "population pointer %d is NULL", q);
+    const Layout& L = c->L;
+    std::vector<double> buf(L.total, 0.0);
+    for (int q = 0; q < 38; ++q) {
+        pack_host(L, 1, q < 19 ? f[q] : g[q - 19], buf.data());
+        HIPCHK(c, hipMemcpy(c->pdf + q * c->d.ps, buf.data(), sizeof(double) * L.total, hipMemcpyHostToDevice));
+    }
+    std::fill(buf.begin(), buf.end(), 0.0);
+    pack_host(L, 4, phi, buf.data());
+    HIPCHK(c, hipMemcpy(c->d.phi, buf.data(), sizeof(double) * L.total, hipMemcpyHostToDevice));
+    const size_t plane = (size_t)(L.nx + 2) * (L.ny + 2);
+    if (f_convec) HIPCHK(c, hipMemcpy(c->d.cf, f_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
+    if (g_convec) HIPCHK(c, hipMemcpy(c->d.cg, g_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
+    if (phi_convec) HIPCHK(c, hipMemcpy(c->d.cphi, phi_convec, sizeof(double) * plane, hipMemcpyHostToDevice));
+    c->have_state = true;
+    c->phi_old_valid = false;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* const* g, double* phi, double* f_convec,
+                                    double* g_convec, double* phi_convec) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    const Layout& L = c->L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> buf(L.total);
+    for (int q = 0; q < 38; ++q) {
+        double* dst = q < 19 ? (f ? f[q] : nullptr) : (g ? g[q - 19] : nullptr);
+        if (!dst) continue;
+        HIPCHK(c, hipMemcpy(buf.data(), c->pdf + q * c->d.ps, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, 1, buf.data(), dst);
+    }
+    if (phi) {
+        HIPCHK(c, hipMemcpy(buf.data(), c->d.phi, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, 4, buf.data(), phi);
+    }
+    const size_t plane = (size_t)(L.nx + 2) * (L.ny + 2);
+    if (f_convec) HIPCHK(c, hipMemcpy(f_convec, c->d.cf, sizeof(double) * plane * 19, hipMemcpyDeviceToHost));
+    if (g_convec) HIPCHK(c, hipMemcpy(g_convec, c->d.cg, sizeof(double) * plane * 19, hipMemcpyDeviceToHost));
+    if (phi_convec) HIPCHK(c, hipMemcpy(phi_convec, c->d.cphi, sizeof(double) * plane, hipMemcpyDeviceToHost));
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_color_gradient(mflbm_ctx* c) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom || !c->have_state) return fail(c, MFLBM_ERR_STATE, "upload geometry and state first");
+    return launch_gradient(c);
+}
+
+extern "C" int mflbm_step(mflbm_ctx* c, int32_t ntime) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom || !c->have_state) return fail(c, MFLBM_ERR_STATE, "upload geometry and state first");
+    return launch_step(c, ntime);
+}
+
+extern "C" int mflbm_run(mflbm_ctx* c, int32_t ntime0, int32_t nsteps) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom || !c->have_state) return fail(c, MFLBM_ERR_STATE, "upload geometry and state first");
+    if (nsteps < 0) return fail(c, MFLBM_ERR_ARG, "nsteps < 0");
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = launch_step(c, ntime0 + s);
+        if (rc) return rc;
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    LAUNCH(c, k_macro, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d, c->p.gamma, c->p.force_z, c->u, c->v, c->w, c->rho);
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_monitor(mflbm_ctx* c, double* layers, mflbm_monitor_sums* sums, int32_t update_phi_old) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    int rc = mflbm_compute_macro(c);
+    if (rc) return rc;
+    const Layout& L = c->L;
+    double* lay = c->red;
+    double* part = c->red + 7 * (size_t)L.nz;
+    LAUNCH(c, k_monitor_layers, (unsigned)L.nz, c->d, c->u, c->v, c->w, c->rho, c->phi_old_valid ? c->phi_old : nullptr,
+           lay, part);
+    std::vector<double> h((size_t)L.nz * (7 + NPART + 3 - 7));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (layers) std::memcpy(layers, h.data(), sizeof(double) * 7 * L.nz);
+    if (sums) {
+        const double* P = h.data() + 7 * (size_t)L.nz;
+        auto row = [&](int m) { return P + (size_t)m * L.nz; };
+        mflbm_monitor_sums s{};
+        // layer partials combined in layer order on the host (deterministic)
+        for (int k = 0; k < L.nz; ++k) {
+            s.usq1 += row(0)[k];
+            s.usq2 += row(1)[k];
+            s.pre_w += row(2)[k];
+            s.n_w += (int64_t)row(3)[k];
+            s.pre_nw += row(4)[k];
+            s.umax2 = k == 0 ? row(5)[k] : (row(5)[k] > s.umax2 ? row(5)[k] : s.umax2);
+            s.dphi_max = k == 0 ? row(6)[k] : (row(6)[k] > s.dphi_max ? row(6)[k] : s.dphi_max);
+            s.n_nw += (int64_t)row(7)[k];
+            s.vol1_full += h[2 * (size_t)L.nz + k];
+            s.vol2_full += h[3 * (size_t)L.nz + k];
+        }
+        // breakthrough count at z = nz-1 (Monitor.F90:472-507)
+        std::vector<double> phil((size_t)L.nx * L.ny);
+        std::vector<int8_t> wl((size_t)L.nx * L.ny);
+        for (int j = 1; j <= L.ny; ++j) {
+            HIPCHK(c, hipMemcpy(phil.data() + (size_t)(j - 1) * L.nx, c->d.phi + L.at(1, j, L.nz - 1), sizeof(double) * L.nx,
+                                hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(wl.data() + (size_t)(j - 1) * L.nx, c->walls + L.at(1, j, L.nz - 1), (size_t)L.nx,
+                                hipMemcpyDeviceToHost));
+        }
+        for (size_t m = 0; m < phil.size(); ++m)
+            if (wl[m] == 0 && phil[m] > 0.0) s.outlet_phase1++;
+        *sums = s;
+    }
+    if (update_phi_old) {
+        LAUNCH(c, k_copy_phi_old, nblocks(L.total), c->d, c->phi_old);
+        c->phi_old_valid = true;
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_download_macro(mflbm_ctx* c, double* u, double* v, double* w, double* rho, double* phi) {
+    if (!c) return MFLBM_ERR_ARG;
+    const Layout& L = c->L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> buf(L.total);
+    double* src[5] = {c->u, c->v, c->w, c->rho, c->d.phi};
+    double* dst[5] = {u, v, w, rho, phi};
+    for (int m = 0; m < 5; ++m) {
+        if (!dst[m]) continue;
+        HIPCHK(c, hipMemcpy(buf.data(), src[m], sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, m == 4 ? 4 : 1, buf.data(), dst[m]);
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_download_gradient(mflbm_ctx* c, double* cn_x, double* cn_y, double* cn_z, double* c_norm, double* curv) {
+    if (!c) return MFLBM_ERR_ARG;
+    const Layout& L = c->L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> buf(L.total);
+    double* src[5] = {c->d.cnx, c->d.cny, c->d.cnz, c->d.cnorm, c->d.curv};
+    double* dst[5] = {cn_x, cn_y, cn_z, c_norm, curv};
+    for (int m = 0; m < 5; ++m) {
+        if (!dst[m]) continue;
+        HIPCHK(c, hipMemcpy(buf.data(), src[m], sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, m == 4 ? 1 : 2, buf.data(), dst[m]);
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_synchronize(mflbm_ctx* c) {
+    if (!c) return MFLBM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MFLBM_OK;
+}
+
+namespace {
+int fold_events(mflbm_ctx* c) {
+    if (c->ev_used == 0) return MFLBM_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t b = 0; b + 4 <= c->ev_used; b += 4) {
+        float a = 0, g = 0, s = 0;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[b], c->ev[b + 1]));
+        HIPCHK(c, hipEventElapsedTime(&g, c->ev[b + 2], c->ev[b + 3]));
+        HIPCHK(c, hipEventElapsedTime(&s, c->ev[b], c->ev[b + 3]));
+        c->t_sum[0] += a;
+        c->t_sum[1] += g;
+        c->t_sum[2] += s;
+        c->t_cnt[0]++;
+        c->t_cnt[1]++;
+        c->t_cnt[2]++;
+    }
+    c->ev_used = 0;
+    return MFLBM_OK;
+}
+}  // namespace
+
+extern "C" int mflbm_timing_enable(mflbm_ctx* c, int32_t enable) {
+    if (!c) return MFLBM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing = enable != 0;
+    c->ev_used = 0;
+    for (int m = 0; m < 3; ++m) {
+        c->t_sum[m] = 0;
+        c->t_cnt[m] = 0;
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_timing(mflbm_ctx* c, int32_t which, double* mean_ms, int64_t* count) {
+    if (!c || which < 0 || which > 2) return MFLBM_ERR_ARG;
+    int rc = fold_events(c);
+    if (rc) return rc;
+    if (mean_ms) *mean_ms = c->t_cnt[which] ? c->t_sum[which] / c->t_cnt[which] : 0.0;
+    if (count) *count = c->t_cnt[which];
+    return MFLBM_OK;
+}

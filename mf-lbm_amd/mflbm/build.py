@@ -1,0 +1,68 @@
+"""Build the native libraries in-tree.
+
+* ``mflbm/lib/libmflbm.so``: the gfx950 device code and the C ABI (hipcc, FP contraction off so the FP64
+  arithmetic rounds exactly as written; packed-FP32 instructions disabled).
+* ``oracle/liboracle.so``: the CPU checker (g++, OpenMP), built only when its sources are present.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)                 # mf-lbm_amd/
+REPO = os.path.dirname(ROOT)
+CSRC = os.path.join(ROOT, "csrc")
+INCLUDE = os.path.join(REPO, "include")
+LIB = os.path.join(PKG, "lib", "libmflbm.so")
+ORACLE_DIR = os.path.join(REPO, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MFLBM_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def build_device(verbose: bool = False) -> str:
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    srcs = [os.path.join(CSRC, "mflbm_device.hip"), os.path.join(CSRC, "host_setup.cpp")]
+    tmp = LIB + ".tmp"
+    out = _run([HIPCC, *HIP_FLAGS, f"-I{CSRC}", f"-I{INCLUDE}", *srcs, "-o", tmp])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(out)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> str | None:
+    src = os.path.join(ORACLE_DIR, "oracle.cpp")
+    if not os.path.exists(src):
+        return None
+    cxx = shutil.which("g++") or "g++"
+    tmp = ORACLE_LIB + ".tmp"
+    _run([cxx, "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+          src, "-o", tmp])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build_all(verbose: bool = False) -> None:
+    build_device(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)

@@ -1,0 +1,127 @@
+"""ctypes view of the C ABI declared in include/mflbm.h.
+
+The library is the in-tree build ``mflbm/lib/libmflbm.so`` (see mflbm.build).  There is no fallback: if the
+library is missing or fails to load, importing the solver raises, so a run can never silently use a
+different implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import LIB
+
+c_int = C.c_int32
+c_dbl = C.c_double
+P_dbl = C.POINTER(C.c_double)
+P_int = C.POINTER(C.c_int32)
+P_i8 = C.POINTER(C.c_int8)
+
+
+class Params(C.Structure):
+    """mflbm_params"""
+    _fields_ = [(n, c_int) for n in (
+        "nx", "ny", "nz", "jper", "kper", "wall_x_min", "wall_x_max", "wall_y_min", "wall_y_max",
+        "wall_z_min", "wall_z_max", "inlet_bc", "outlet_bc", "porous_plate_cmd", "z_porous_plate",
+        "n_exclude_inlet", "n_exclude_outlet")] + [(n, c_dbl) for n in (
+        "la_nui1", "la_nui2", "gamma", "beta", "force_z", "sa_inject", "phi_inlet", "rho_in", "rho_out",
+        "uin_avg", "relaxation")]
+
+
+class MonitorSums(C.Structure):
+    """mflbm_monitor_sums"""
+    _fields_ = [(n, c_dbl) for n in ("umax2", "usq1", "usq2", "vol1_full", "vol2_full", "dphi_max",
+                                     "pre_w", "pre_nw")] + [(n, C.c_int64) for n in ("n_w", "n_nw", "outlet_phase1")]
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(P_dbl)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(P_int)
+
+
+def i8ptr(a: np.ndarray):
+    assert a.dtype == np.int8 and a.flags.c_contiguous
+    return a.ctypes.data_as(P_i8)
+
+
+def ptr_array(arrs):
+    """19 population pointers for the (const) double* const* arguments."""
+    return (P_dbl * len(arrs))(*[dptr(a) for a in arrs])
+
+
+_ENTRY_POINTS = {
+    # name: (restype, argtypes)
+    "create": (c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "destroy": (None, [C.c_void_p]),
+    "last_error": (C.c_char_p, [C.c_void_p]),
+    "set_scalars": (c_int, [C.c_void_p, c_dbl, c_dbl, c_dbl, c_dbl]),
+    "upload_geometry": (c_int, [C.c_void_p, P_i8, c_int, P_int, P_int, P_int, P_dbl, c_int, P_int, P_dbl, P_dbl, P_dbl]),
+    "upload_state": (c_int, [C.c_void_p, C.POINTER(P_dbl), C.POINTER(P_dbl), P_dbl, P_dbl, P_dbl, P_dbl]),
+    "download_state": (c_int, [C.c_void_p, C.POINTER(P_dbl), C.POINTER(P_dbl), P_dbl, P_dbl, P_dbl, P_dbl]),
+    "step": (c_int, [C.c_void_p, c_int]),
+    "run": (c_int, [C.c_void_p, c_int, c_int]),
+    "color_gradient": (c_int, [C.c_void_p]),
+    "compute_macro": (c_int, [C.c_void_p]),
+    "monitor": (c_int, [C.c_void_p, P_dbl, C.POINTER(MonitorSums), c_int]),
+    "download_macro": (c_int, [C.c_void_p, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]),
+    "download_gradient": (c_int, [C.c_void_p, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]),
+    "synchronize": (c_int, [C.c_void_p]),
+    "timing": (c_int, [C.c_void_p, c_int, P_dbl, C.POINTER(C.c_int64)]),
+    "timing_enable": (c_int, [C.c_void_p, c_int]),
+}
+
+_HOST_ENTRY_POINTS = {
+    "mflbm_create_error": (C.c_char_p, []),
+    "mflbm_setup_walls": (c_int, [C.POINTER(Params), P_i8, P_i8]),
+    "mflbm_geometry_new": (c_int, [C.POINTER(Params), P_i8, c_dbl, C.POINTER(C.c_void_p)]),
+    "mflbm_geometry_counts": (c_int, [C.c_void_p, P_int, P_int]),
+    "mflbm_geometry_export": (c_int, [C.c_void_p, P_int, P_int, P_int, P_dbl, P_int, P_dbl, P_dbl]),
+    "mflbm_geometry_free": (None, [C.c_void_p]),
+    "mflbm_init_fields": (c_int, [C.POINTER(Params), c_int, c_dbl, P_dbl, P_dbl, P_dbl, C.POINTER(P_dbl),
+                                  C.POINTER(P_dbl), P_dbl, P_dbl, P_dbl, P_dbl]),
+    "mflbm_inlet_profile": (c_int, [C.POINTER(Params), c_dbl, c_dbl, c_dbl, P_dbl]),
+}
+
+# every symbol include/mflbm.h declares
+EXPORTED = ["mflbm_" + n for n in _ENTRY_POINTS] + list(_HOST_ENTRY_POINTS)
+
+
+def bind(lib: C.CDLL, prefix: str, names=None):
+    """Attach restype/argtypes to ``prefix + name`` for the context entry points; returns a dict."""
+    out = {}
+    for name, (res, args) in _ENTRY_POINTS.items():
+        if names is not None and name not in names:
+            continue
+        if not hasattr(lib, prefix + name):
+            continue
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+        out[name] = fn
+    return out
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree HIP library; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                               "or `python -m mflbm.build`")
+        lib = C.CDLL(LIB)
+        for name, (res, args) in _HOST_ENTRY_POINTS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
