@@ -1,0 +1,138 @@
+#!/usr/bin/env python3
+"""Benchmark: MLUPS of the FP64 two-phase time step on a 256^3 synthetic porous medium, one MI355X.
+
+Workload (the headline configuration): 256^3 lattice, randomly placed overlapping solid spheres of radius
+uniform in [6, 12] (SplitMix64 seed 2024) added until the porosity of the 254x254x240 sample reaches 0.22,
+8 fluid buffer layers at each z end, solid walls at x and y ends, D3Q19 MRT colour-gradient model with two
+population sets, velocity inlet (mean 1e-4, rectangular-duct profile) and convective outlet, contact angle
+150 degrees, viscosity ratio 1 (nu = 0.05), surface tension 0.05 (Ca = 1e-4), drainage start.
+
+A "step" is one full time step: collision/streaming of both colours, boundary conditions and the
+colour-gradient chain, exactly as the reference's main_iteration_kernel.  MLUPS counts fluid nodes
+(pore_sum * steps / t / 1e6, the reference's formula).  Inputs are resident in HBM before timing starts.
+
+roofline: the dominant kernel (collision + streaming) timed with HIP events on its own stream over the timed
+steps; achieved = 625 B per fluid node (the algorithmic bytes of one two-phase FP64 step) * fluid nodes /
+mean kernel duration.  cpu_baseline: the CPU oracle (C++/OpenMP restatement of the same step) on the host
+cores for a bounded number of steps of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "mf-lbm_amd"))
+
+B_ALG = 625.0          # bytes per fluid node per step, two-phase FP64
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def workload(n: int = 256):
+    from mflbm import Case, geometry
+    walls = geometry.sphere_pack(n, n, n, 0.22, 6.0, 12.0, 2024, buffer=8)
+    ctl = {"lattice_dimensions": (n, n, n), "periodic_indicator": (0, 0, 0), "domain_wall_status_z": (0, 0),
+           "MPI_async_layers_num": (0, 0, 0), "inlet_BC": 1, "outlet_BC": 1, "fluid1_viscosity": 0.05,
+           "fluid2_viscosity": 0.05, "surface_tension": 0.05, "theta": 150.0, "capillary_number": 1e-4,
+           "initial_fluid_distribution_option": 1, "initial_interface_position": 8.0,
+           "saturation_injection": 1.0, "excluded_layers": (8, 8)}
+    return Case(ctl, walls)
+
+
+def cpu_baseline(case, st, steps: int, threads: int):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle_backend import OracleSimulation, set_threads
+    used = set_threads(threads)
+    sim = OracleSimulation(case, state=st)
+    sim.run(1, 2)   # first touch
+    t0 = time.perf_counter()
+    sim.run(3, steps)
+    dt = time.perf_counter() - t0
+    sim.close()
+    return case.n_fluid * steps / dt / 1e6, used
+
+
+def traffic_from_profiles(name: str):
+    p = os.path.join(REPO, "profiles", f"traffic_{name}.json")
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
+              "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from mflbm import Simulation
+
+    t0 = time.time()
+    case = workload(args.size)
+    st = case.initial_state()
+    print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
+    sim = Simulation(case, state=st)
+    ntime = 1
+    sim.run(ntime, args.warmup)
+    ntime += args.warmup
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.run(ntime, args.steps)
+    sim.synchronize()
+    dt = time.perf_counter() - t1
+    ntime += args.steps
+    # kernel timing on the context's stream over a second pass of the same length
+    sim._call("timing_enable", 1)
+    sim.run(ntime, args.steps)
+    import ctypes as C
+    means = []
+    for which in range(3):
+        m, cnt = C.c_double(), C.c_int64()
+        sim._call("timing", which, C.byref(m), C.byref(cnt))
+        means.append(m.value)
+    sim._call("timing_enable", 0)
+    mon = sim.monitor()
+    sim.close()
+    mlups = case.n_fluid * args.steps / dt / 1e6
+    kernel_ms = means[0]
+    achieved = B_ALG * case.n_fluid / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        v, used = cpu_baseline(case, st, args.cpu_steps, threads)
+        cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": used, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {args.size}^3 workload after 2 untimed steps, "
+                         f"C++/OpenMP oracle, {used} threads"}
+    name = f"sphere_pack_{args.size}"
+    out = {
+        "metric": "MLUPS (two-phase D3Q19 colour model, FP64) at 256^3 porous",
+        "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"two-phase drainage, {args.size}^3 seeded sphere pack porosity 0.22, 8-layer "
+                               "buffers, velocity inlet 1e-4 + convective outlet, 150 deg, FP64",
+                   "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": traffic_from_profiles(name), "kernel": "k_collide (even/odd mean)",
+                     "kernel_ms": round(kernel_ms, 4), "gradient_chain_ms": round(means[1], 4),
+                     "step_ms_events": round(means[2], 4)},
+        "cpu_baseline": cpu,
+        "check": {"saturation": mon.saturation, "umax": mon.umax, "failed": mon.failed},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

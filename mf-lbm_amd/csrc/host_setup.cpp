@@ -313,6 +313,10 @@ extern "C" int mflbm_init_fields(const mflbm_params* p, int32_t option, double z
                 phi[P.at(i, j, k)] = v;
             }
     const double W0 = lat::W[0], W1 = lat::W[1], W2 = lat::W[7];
+    // per colour and direction over contiguous arrays; the expression of each population is the reference's
+    // with the node's velocity (zero for a run started at rest)
+    const long npdf = F.size();
+    std::vector<double> rho1(npdf), rho2(npdf), usq(npdf);
     for (int k = 0; k <= nz + 1; ++k)
         for (int j = 0; j <= ny + 1; ++j)
             for (int i = 0; i <= nx + 1; ++i) {
@@ -320,34 +324,46 @@ extern "C" int mflbm_init_fields(const mflbm_params* p, int32_t option, double z
                 const double u = u0 ? u0[n] : 0.0, v = v0 ? v0[n] : 0.0, w = w0 ? w0[n] : 0.0;
                 const double rho = 1.0;
                 const double ph = phi[P.at(i, j, k)];
-                const double usq = u * u + v * v + w * w;
-                const double r[2] = {rho * (1.0 + ph) * 0.5, rho * (1.0 - ph) * 0.5};
-                for (int c = 0; c < 2; ++c) {
-                    double* const* d = c == 0 ? f : g;
-                    const double rr = r[c];
-                    auto eq1 = [&](double eu, double uu) { return rr * W1 + rr * W1 * (3.0 * eu + 4.5 * uu * uu - 1.5 * usq); };
-                    auto eq2 = [&](double eu) { return rr * W2 + rr * W2 * (3.0 * eu + 4.5 * eu * eu - 1.5 * usq); };
-                    d[0][n] = rr * W0 + rr * W0 * (-1.5 * usq);
-                    d[1][n] = eq1(u, u);
-                    d[2][n] = eq1(-u, u);
-                    d[3][n] = eq1(v, v);
-                    d[4][n] = eq1(-v, v);
-                    d[5][n] = eq1(w, w);
-                    d[6][n] = eq1(-w, w);
-                    d[7][n] = eq2(u + v);
-                    d[8][n] = eq2(-u + v);
-                    d[9][n] = eq2(u - v);
-                    d[10][n] = eq2(-u - v);
-                    d[11][n] = eq2(u + w);
-                    d[12][n] = eq2(-u + w);
-                    d[13][n] = eq2(u - w);
-                    d[14][n] = eq2(-u - w);
-                    d[15][n] = eq2(v + w);
-                    d[16][n] = eq2(-v + w);
-                    d[17][n] = eq2(v - w);
-                    d[18][n] = eq2(-v - w);
-                }
+                usq[n] = u * u + v * v + w * w;
+                rho1[n] = rho * (1.0 + ph) * 0.5;
+                rho2[n] = rho * (1.0 - ph) * 0.5;
             }
+    for (int c = 0; c < 2; ++c) {
+        double* const* d = c == 0 ? f : g;
+        const double* rr = c == 0 ? rho1.data() : rho2.data();
+        for (int q = 0; q < 19; ++q) {
+            double* out = d[q];
+            const double wq = q == 0 ? W0 : (q <= 6 ? W1 : W2);
+            for (long n = 0; n < npdf; ++n) {
+                const double u = u0 ? u0[n] : 0.0, v = v0 ? v0[n] : 0.0, w = w0 ? w0[n] : 0.0;
+                double eu, uu;   // e_q . u, and the term squared by the reference (u_x for q = 1, 2, ...)
+                switch (q) {
+                    case 0: eu = 0.0; uu = 0.0; break;
+                    case 1: eu = u; uu = u; break;
+                    case 2: eu = -u; uu = u; break;
+                    case 3: eu = v; uu = v; break;
+                    case 4: eu = -v; uu = v; break;
+                    case 5: eu = w; uu = w; break;
+                    case 6: eu = -w; uu = w; break;
+                    case 7: eu = u + v; uu = eu; break;
+                    case 8: eu = -u + v; uu = eu; break;
+                    case 9: eu = u - v; uu = eu; break;
+                    case 10: eu = -u - v; uu = eu; break;
+                    case 11: eu = u + w; uu = eu; break;
+                    case 12: eu = -u + w; uu = eu; break;
+                    case 13: eu = u - w; uu = eu; break;
+                    case 14: eu = -u - w; uu = eu; break;
+                    case 15: eu = v + w; uu = eu; break;
+                    case 16: eu = -v + w; uu = eu; break;
+                    case 17: eu = v - w; uu = eu; break;
+                    default: eu = -v - w; uu = eu; break;
+                }
+                const double r = rr[n];
+                if (q == 0) out[n] = r * wq + r * wq * (-1.5 * usq[n]);
+                else out[n] = r * wq + r * wq * (3.0 * eu + 4.5 * uu * uu - 1.5 * usq[n]);
+            }
+        }
+    }
     if (p->outlet_bc == 1 && f_convec && g_convec && phi_convec) {
         const long plane = (long)(nx + 2) * (ny + 2);
         for (int j = 0; j <= ny + 1; ++j)

@@ -1024,6 +1024,7 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     d.cphi = c->planes + plane;
     d.cf = c->planes + 2 * plane;
     d.cg = c->planes + 21 * plane;
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(c, MFLBM_ERR_HIP, "hipDeviceSynchronize failed"));
     *out = c;
     return MFLBM_OK;
 }
@@ -1108,6 +1109,7 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     c->sl = SolidList{ns, c->s_i, c->s_i + 3 * (size_t)ns, c->s_i + 4 * (size_t)ns, c->s_d};
     c->fl = FluidList{nf, c->f_i, c->f_d, c->f_d + 3 * (size_t)nf, c->f_d + 4 * (size_t)nf};
     if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipDeviceSynchronize());
     c->have_geom = true;
     return MFLBM_OK;
 }
@@ -1131,6 +1133,7 @@ extern "C" int mflbm_upload_state(mflbm_ctx* c, const double* const* f, const do
     if (f_convec) HIPCHK(c, hipMemcpy(c->d.cf, f_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
     if (g_convec) HIPCHK(c, hipMemcpy(c->d.cg, g_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
     if (phi_convec) HIPCHK(c, hipMemcpy(c->d.cphi, phi_convec, sizeof(double) * plane, hipMemcpyHostToDevice));
+    HIPCHK(c, hipDeviceSynchronize());
     c->have_state = true;
     c->phi_old_valid = false;
     return MFLBM_OK;
