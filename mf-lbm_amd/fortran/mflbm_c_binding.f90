@@ -1,0 +1,110 @@
+! ISO_C_BINDING interface to the C ABI of include/mflbm.h, for the reference's Fortran driver.
+!
+! The reference's per-step subroutines (kernel_even_color / kernel_odd_color, the boundary routines and
+! color_gradient, called from main_iteration_kernel in multiphase_3D/0.src/Main_multiphase.F90:341-486)
+! become one call to mflbm_step; the module arrays keep their shapes and are handed over by address.
+module mflbm_c_binding
+    use, intrinsic :: iso_c_binding
+    implicit none
+
+    type, bind(c) :: mflbm_params
+        integer(c_int32_t) :: nx, ny, nz
+        integer(c_int32_t) :: jper, kper
+        integer(c_int32_t) :: wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max
+        integer(c_int32_t) :: inlet_bc, outlet_bc, porous_plate_cmd, z_porous_plate
+        integer(c_int32_t) :: n_exclude_inlet, n_exclude_outlet
+        real(c_double) :: la_nui1, la_nui2, gamma, beta, force_z, sa_inject, phi_inlet
+        real(c_double) :: rho_in, rho_out, uin_avg, relaxation
+    end type mflbm_params
+
+    type, bind(c) :: mflbm_monitor_sums
+        real(c_double) :: umax2, usq1, usq2, vol1_full, vol2_full, dphi_max, pre_w, pre_nw
+        integer(c_int64_t) :: n_w, n_nw, outlet_phase1
+    end type mflbm_monitor_sums
+
+    interface
+        integer(c_int) function mflbm_create(params, ctx) bind(c, name="mflbm_create")
+            import :: c_int, c_ptr, mflbm_params
+            type(mflbm_params), intent(in) :: params
+            type(c_ptr), intent(out) :: ctx
+        end function
+        subroutine mflbm_destroy(ctx) bind(c, name="mflbm_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end subroutine
+        type(c_ptr) function mflbm_last_error(ctx) bind(c, name="mflbm_last_error")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mflbm_set_scalars(ctx, force_z, rho_in, uin_avg, relaxation) &
+                bind(c, name="mflbm_set_scalars")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: force_z, rho_in, uin_avg, relaxation
+        end function
+        integer(c_int) function mflbm_upload_geometry(ctx, walls_h2, n_solid, s_ijk, s_nfluid, s_nbr, s_weight, &
+                n_fluid, f_ijk, f_nw, f_theta, w_in) bind(c, name="mflbm_upload_geometry")
+            import :: c_int, c_ptr, c_int8_t, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int8_t), intent(in) :: walls_h2(*)
+            integer(c_int32_t), value :: n_solid, n_fluid
+            integer(c_int32_t), intent(in) :: s_ijk(*), s_nfluid(*), s_nbr(*), f_ijk(*)
+            real(c_double), intent(in) :: s_weight(*), f_nw(*), f_theta(*), w_in(*)
+        end function
+        ! f, g: arrays of 19 addresses (c_loc(f0) ... c_loc(f18))
+        integer(c_int) function mflbm_upload_state(ctx, f, g, phi, f_convec, g_convec, phi_convec) &
+                bind(c, name="mflbm_upload_state")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            type(c_ptr), intent(in) :: f(19), g(19)
+            real(c_double), intent(in) :: phi(*), f_convec(*), g_convec(*), phi_convec(*)
+        end function
+        integer(c_int) function mflbm_download_state(ctx, f, g, phi, f_convec, g_convec, phi_convec) &
+                bind(c, name="mflbm_download_state")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            type(c_ptr), intent(in) :: f(19), g(19)
+            real(c_double), intent(inout) :: phi(*), f_convec(*), g_convec(*), phi_convec(*)
+        end function
+        integer(c_int) function mflbm_step(ctx, ntime) bind(c, name="mflbm_step")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: ntime
+        end function
+        integer(c_int) function mflbm_run(ctx, ntime0, nsteps) bind(c, name="mflbm_run")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: ntime0, nsteps
+        end function
+        integer(c_int) function mflbm_color_gradient(ctx) bind(c, name="mflbm_color_gradient")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mflbm_compute_macro(ctx) bind(c, name="mflbm_compute_macro")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mflbm_monitor(ctx, layers, sums, update_phi_old) bind(c, name="mflbm_monitor")
+            import :: c_int, c_ptr, c_double, c_int32_t, mflbm_monitor_sums
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: layers(*)
+            type(mflbm_monitor_sums), intent(out) :: sums
+            integer(c_int32_t), value :: update_phi_old
+        end function
+        integer(c_int) function mflbm_download_macro(ctx, u, v, w, rho, phi) bind(c, name="mflbm_download_macro")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: u(*), v(*), w(*), rho(*), phi(*)
+        end function
+        integer(c_int) function mflbm_synchronize(ctx) bind(c, name="mflbm_synchronize")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mflbm_setup_walls(params, walls_global, walls_h2) bind(c, name="mflbm_setup_walls")
+            import :: c_int, c_int8_t, mflbm_params
+            type(mflbm_params), intent(in) :: params
+            integer(c_int8_t), intent(in) :: walls_global(*)
+            integer(c_int8_t), intent(inout) :: walls_h2(*)
+        end function
+    end interface
+end module mflbm_c_binding
