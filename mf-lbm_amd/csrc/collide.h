@@ -10,10 +10,11 @@ struct CollideParams {
 };
 
 // in:  a[q] = incoming fluid-1 populations, b[q] = incoming fluid-2 populations (q = 0..18),
-//      cnx/cny/cnz unit interface normal, curv curvature, cnorm |grad phi|.
+//      cnx/cny/cnz unit interface normal, tf = 0.5*gamma*curv*c_norm (the CSF magnitude, evaluated by the
+//      colour-gradient chain in the reference's order, Kernel_multiphase.F90:116).
 // out: a[q], b[q] overwritten with the outgoing recoloured populations; returns phi.
-MFLBM_HD inline double collide_node(double* __restrict__ a, double* __restrict__ b, double cnx, double cny,
-                                    double cnz, double curv, double cnorm, const CollideParams& P) {
+MFLBM_HD inline double collide_node_tf(double* __restrict__ a, double* __restrict__ b, double cnx, double cny,
+                                       double cnz, double tf, const CollideParams& P) {
     using namespace lat;
     double ft0 = a[0] + b[0], ft1 = a[1] + b[1], ft2 = a[2] + b[2], ft3 = a[3] + b[3], ft4 = a[4] + b[4];
     double ft5 = a[5] + b[5], ft6 = a[6] + b[6], ft7 = a[7] + b[7], ft8 = a[8] + b[8], ft9 = a[9] + b[9];
@@ -26,7 +27,7 @@ MFLBM_HD inline double collide_node(double* __restrict__ a, double* __restrict__
                         b[13] + b[14] + b[15] + b[16] + b[17] + b[18];
     const double phi = (rho1 - rho2) / (rho1 + rho2);
 
-    double tmp = 0.5 * P.gamma * curv * cnorm;
+    double tmp = tf;
     const double fx = tmp * cnx;
     const double fy = tmp * cny;
     const double fz = tmp * cnz + P.force_z;

@@ -76,6 +76,17 @@ CASES = {
                  "initial_interface_position": 6.0, "saturation_injection": 0.0,
                  "excluded_layers": (2, 2)},
         steps=[0, 1, 2, 3, 40], monitor=10),
+    # porous plate blocking fluid 1 near the outlet of a velocity-driven drainage (plate at z = 20)
+    "porous_plate": dict(
+        geom=lambda: geometry.tube_sphere(12, 12, 24, 4.8, 2.4, buffer=3),
+        control={"lattice_dimensions": (12, 12, 24), "periodic_indicator": (0, 0, 0),
+                 "domain_wall_status_z": (0, 0), "MPI_async_layers_num": (0, 0, 0),
+                 "inlet_BC": 1, "outlet_BC": 1, "fluid1_viscosity": 0.1,
+                 "fluid2_viscosity": 0.1, "surface_tension": 0.01, "theta": 120.0,
+                 "capillary_number": 0.05, "initial_fluid_distribution_option": 1,
+                 "initial_interface_position": 14.0, "saturation_injection": 1.0,
+                 "porous_plate_cmd": 1, "Z_porous_plate": 20, "excluded_layers": (3, 3)},
+        steps=[0, 1, 2, 3, 40], monitor=10),
     # y- and z-periodic wall drop (contact-angle case), exercises the y faces and x-parallel edges
     "periodic_yz_drop": dict(
         geom=lambda: np.zeros((16, 14, 12), dtype=np.int8),

@@ -10,7 +10,7 @@ import numpy as np
 from mflbm import Case
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-SMALL_CASES = ["periodic_tube", "open_velocity", "open_pressure", "periodic_yz_drop"]
+SMALL_CASES = ["periodic_tube", "open_velocity", "open_pressure", "porous_plate", "periodic_yz_drop"]
 C1_CASE = "c1_tube_sphere_48x48x128"
 
 
