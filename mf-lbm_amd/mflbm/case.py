@@ -98,6 +98,10 @@ class Case:
         wg = np.ascontiguousarray(self.walls_global)
         _check(lib.mflbm_setup_walls(C.byref(self.params), i8ptr(wg), i8ptr(self.walls_h2)), "mflbm_setup_walls")
         geo = C.c_void_p()
+        if (rp.kper == 1 and nz < 10) or (rp.jper == 1 and ny < 10):
+            # the geometry pass wraps 10 ghost layers around periodic directions
+            # (Geometry_preprocessing.F90:54-83), which needs at least 10 nodes along them
+            raise ValueError("a periodic direction needs at least 10 lattice nodes")
         _check(lib.mflbm_geometry_new(C.byref(self.params), i8ptr(wg), rp.theta, C.byref(geo)), "mflbm_geometry_new")
         try:
             ns, nf = C.c_int32(), C.c_int32()
