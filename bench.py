@@ -15,6 +15,10 @@ roofline: the dominant kernel (collision + streaming) timed with HIP events on i
 steps; achieved = 625 B per fluid node (the algorithmic bytes of one two-phase FP64 step) * fluid nodes /
 mean kernel duration.  cpu_baseline: the CPU oracle (C++/OpenMP restatement of the same step) on the host
 cores for a bounded number of steps of the same workload.
+
+``--workload singlephase`` measures the single-phase program's step instead (D3Q19 MRT, one population set,
+bounce-back-optimised rates, nu = 0.1) on the same 256^3 pack, z-periodic and driven by a body force of 1e-6
+(a permeability run); 305 B per fluid node.
 """
 from __future__ import annotations
 
@@ -28,6 +32,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "mf-lbm_amd"))
 
 B_ALG = 625.0          # bytes per fluid node per step, two-phase FP64
+B_ALG_SP = 305.0       # bytes per fluid node per step, single-phase FP64
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
@@ -42,11 +47,21 @@ def workload(n: int = 256):
     return Case(ctl, walls)
 
 
-def cpu_baseline(case, st, steps: int, threads: int):
+def workload_sp(n: int = 256):
+    from mflbm import geometry
+    from mflbm.singlephase import SPCase
+    walls = geometry.sphere_pack(n, n, n, 0.22, 6.0, 12.0, 2024, buffer=8)
+    ctl = {"lattice_dimensions": (n, n, n), "periodic_indicator": (0, 0, 1), "domain_wall_status_z": (0, 0),
+           "MPI_async_layers_num": (0, 0, 4), "fluid_viscosity": 0.1, "body_force_0": 1e-6,
+           "MRT_collision_parameter_preset": 1}
+    return SPCase(ctl, walls)
+
+
+def cpu_baseline(case, st, steps: int, threads: int, single_phase: bool = False):
     sys.path.insert(0, os.path.join(REPO, "tests"))
-    from oracle_backend import OracleSimulation, set_threads
+    from oracle_backend import OracleSimulation, SPOracleSimulation, set_threads
     used = set_threads(threads)
-    sim = OracleSimulation(case, state=st)
+    sim = (SPOracleSimulation if single_phase else OracleSimulation)(case, state=st)
     sim.run(1, 2)   # first touch
     t0 = time.perf_counter()
     sim.run(3, steps)
@@ -63,26 +78,9 @@ def traffic_from_profiles(name: str):
     return None
 
 
-def main() -> None:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
-              "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    from mflbm import Simulation
-
-    t0 = time.time()
-    case = workload(args.size)
-    st = case.initial_state()
-    print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
-    sim = Simulation(case, state=st)
+def timed_run(sim, args, nwhich):
+    """Warm-up, timed steps, then a second pass of the same length with HIP-event kernel timing."""
+    import ctypes as C
     ntime = 1
     sim.run(ntime, args.warmup)
     ntime += args.warmup
@@ -92,30 +90,48 @@ def main() -> None:
     sim.synchronize()
     dt = time.perf_counter() - t1
     ntime += args.steps
-    # kernel timing on the context's stream over a second pass of the same length
     sim._call("timing_enable", 1)
     sim.run(ntime, args.steps)
-    import ctypes as C
     means = []
-    for which in range(3):
+    for which in nwhich:
         m, cnt = C.c_double(), C.c_int64()
         sim._call("timing", which, C.byref(m), C.byref(cnt))
         means.append(m.value)
     sim._call("timing_enable", 0)
+    return dt, means
+
+
+def cpu_leg(args, case, st, single_phase=False):
+    if args.no_cpu:
+        return None
+    threads = min(os.cpu_count() or 1, 16)
+    v, used = cpu_baseline(case, st, args.cpu_steps, threads, single_phase)
+    return {"value": round(v, 3), "unit": "MLUPS", "cores": used, "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same {args.size}^3 workload after 2 untimed steps, "
+                      f"C++/OpenMP oracle, {used} threads"}
+
+
+def roofline(b_alg, n_fluid, kernel_ms, traffic, kernel, **extra):
+    achieved = b_alg * n_fluid / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    r = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic, "kernel": kernel,
+         "kernel_ms": round(kernel_ms, 4)}
+    r.update(extra)
+    return r
+
+
+def run_twophase(args) -> dict:
+    from mflbm import Simulation
+    t0 = time.time()
+    case = workload(args.size)
+    st = case.initial_state()
+    print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
+    sim = Simulation(case, state=st)
+    dt, means = timed_run(sim, args, (0, 1, 2))
     mon = sim.monitor()
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
-    kernel_ms = means[0]
-    achieved = B_ALG * case.n_fluid / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
-    cpu = None
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        v, used = cpu_baseline(case, st, args.cpu_steps, threads)
-        cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": used, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same {args.size}^3 workload after 2 untimed steps, "
-                         f"C++/OpenMP oracle, {used} threads"}
-    name = f"sphere_pack_{args.size}"
-    out = {
+    return {
         "metric": "MLUPS (two-phase D3Q19 colour model, FP64) at 256^3 porous",
         "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
@@ -123,14 +139,56 @@ def main() -> None:
         "config": {"workload": f"two-phase drainage, {args.size}^3 seeded sphere pack porosity 0.22, 8-layer "
                                "buffers, velocity inlet 1e-4 + convective outlet, 150 deg, FP64",
                    "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": traffic_from_profiles(name), "kernel": "k_collide (even/odd mean)",
-                     "kernel_ms": round(kernel_ms, 4), "gradient_chain_ms": round(means[1], 4),
-                     "step_ms_events": round(means[2], 4)},
-        "cpu_baseline": cpu,
+        "roofline": roofline(B_ALG, case.n_fluid, means[0], traffic_from_profiles(f"sphere_pack_{args.size}"),
+                             "k_collide (even/odd mean)", gradient_chain_ms=round(means[1], 4),
+                             step_ms_events=round(means[2], 4)),
+        "cpu_baseline": cpu_leg(args, case, st),
         "check": {"saturation": mon.saturation, "umax": mon.umax, "failed": mon.failed},
     }
+
+
+def run_singlephase(args) -> dict:
+    from mflbm.singlephase import SPSimulation, permeability
+    t0 = time.time()
+    case = workload_sp(args.size)
+    st = case.initial_state()
+    print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
+    sim = SPSimulation(case, state=st)
+    dt, means = timed_run(sim, args, (0, 2))
+    mon = sim.monitor()
+    sim.close()
+    mlups = case.n_fluid * args.steps / dt / 1e6
+    return {
+        "metric": "MLUPS (single-phase D3Q19 MRT, FP64) at 256^3 porous",
+        "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"single-phase permeability run, {args.size}^3 seeded sphere pack porosity 0.22, "
+                               "z-periodic, body force 1e-6, nu 0.1, FP64",
+                   "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3},
+        "roofline": roofline(B_ALG_SP, case.n_fluid, means[0], traffic_from_profiles(f"sp_sphere_pack_{args.size}"),
+                             "k_sp_collide (even/odd mean)", step_ms_events=round(means[1], 4)),
+        "cpu_baseline": cpu_leg(args, case, st, single_phase=True),
+        "check": {"flowrate": mon.flowrate, "umax": mon.umax, "permeability": permeability(case, mon.flowrate),
+                  "failed": mon.failed},
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("twophase", "singlephase"), default="twophase")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
+              "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    out = run_singlephase(args) if args.workload == "singlephase" else run_twophase(args)
     print(json.dumps(out))
 
 

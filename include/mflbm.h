@@ -164,6 +164,65 @@ int mflbm_init_fields(const mflbm_params* params, int32_t option, double interfa
  * inlet_vel_profile_rectangular, Misc.F90:625-665). */
 int mflbm_inlet_profile(const mflbm_params* params, double uin_avg, double la_x, double la_y, double* w_in);
 
+/* ================================================================ single-phase solver =================
+ * The reference's single-phase program (singlephase_3D/0.src) runs the same D3Q19 lattice, AA streaming,
+ * periodic self-exchange and open boundaries with one population set and no phase field.  Host arrays
+ * keep its layout: f0..f18, u, v, w, rho over (0:nx+1, 0:ny+1, 0:nz+1); walls with halo 2; w_in over
+ * (0:nx+1, 0:ny+1); f_convec_bc over (0:nx+1, 0:ny+1, 0:18) (Initialization.F90:390-402).  Walls and the
+ * inlet profile come from mflbm_setup_walls / mflbm_inlet_profile (the routines are the same in both
+ * programs; only the mflbm_params fields nx..wall_z_max are read by them). */
+
+/* Per-run constants after initialization_basic (singlephase_3D/0.src/Initialization.F90:4-152). */
+typedef struct mflbm_sp_params {
+    int32_t nx, ny, nz;
+    int32_t jper, kper;
+    int32_t wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max;
+    int32_t inlet_bc;          /* 1 velocity bounce-back, 2 Zou-He pressure (Boundary.F90) */
+    int32_t outlet_bc;         /* 1 convective, 2 Zou-He pressure */
+    int32_t n_exclude_inlet, n_exclude_outlet;
+    int32_t mrt_preset;        /* MRT_collision_parameter_preset: 1 bounce-back optimised, else SRT */
+    double la_nu;              /* fluid_viscosity */
+    double force_z;            /* body force along z (force_z0) */
+    double rho_in, rho_out;    /* Zou-He densities (rho_out = 1) */
+    double uin_avg;            /* convective outlet velocity (= mean inlet velocity) */
+    double relaxation;         /* inlet ramp factor, 1 in the reference */
+} mflbm_sp_params;
+
+typedef struct mflbm_sp_ctx mflbm_sp_ctx;
+
+/* Context on the current HIP device (MemAllocate, Initialization.F90:329-460, and the `!$acc data`
+ * region, Main.F90:104-110).  A failed create leaves its message in mflbm_create_error(). */
+int mflbm_sp_create(const mflbm_sp_params* params, mflbm_sp_ctx** out);
+void mflbm_sp_destroy(mflbm_sp_ctx* ctx);
+const char* mflbm_sp_last_error(const mflbm_sp_ctx* ctx);
+int mflbm_sp_set_scalars(mflbm_sp_ctx* ctx, double force_z, double rho_in, double uin_avg, double relaxation);
+/* walls with halo 2 (mflbm_setup_walls) and the inlet profile (may be NULL unless inlet_bc == 1) */
+int mflbm_sp_upload_geometry(mflbm_sp_ctx* ctx, const int8_t* walls_h2, const double* w_in);
+/* checkpoint content (save_checkpoint, IO.F90:490-547): 19 population pointers and the convective
+ * outlet history (may be NULL when outlet_bc != 1) */
+int mflbm_sp_upload_state(mflbm_sp_ctx* ctx, const double* const* f, const double* f_convec);
+int mflbm_sp_download_state(mflbm_sp_ctx* ctx, double* const* f, double* f_convec);
+/* One time step (main_iteration_kernel, Main.F90:291-422): kernel_even / kernel_odd (Kernel.F90:5-400),
+ * periodic wrap (Mpi.F90), inlet/outlet boundaries (Boundary.F90). */
+int mflbm_sp_step(mflbm_sp_ctx* ctx, int32_t ntime);
+int mflbm_sp_run(mflbm_sp_ctx* ctx, int32_t ntime0, int32_t nsteps);
+/* u, v, w, rho (compute_macro_vars, Misc.F90:368-423; meaningful after even steps), on the device */
+int mflbm_sp_compute_macro(mflbm_sp_ctx* ctx);
+/* Monitor reductions (monitor, Monitor.F90:4-172); calls compute_macro first.  layers: 2*nz doubles,
+ * fl(k) = sum of w over layer k at [k], pre(k) = sum of rho at [nz + k]; umax2 = max of u^2+v^2+w^2. */
+int mflbm_sp_monitor(mflbm_sp_ctx* ctx, double* layers, double* umax2);
+int mflbm_sp_download_macro(mflbm_sp_ctx* ctx, double* u, double* v, double* w, double* rho);
+int mflbm_sp_synchronize(mflbm_sp_ctx* ctx);
+/* Mean duration (ms) per step since the last reset: which = 0 lattice-update kernel, 2 whole step. */
+int mflbm_sp_timing(mflbm_sp_ctx* ctx, int32_t which, double* mean_ms, int64_t* count);
+int mflbm_sp_timing_enable(mflbm_sp_ctx* ctx, int32_t enable);
+
+/* Initial populations of a new run (initialization_new / _pdf, Initialization.F90:207-307): equilibrium
+ * at (rho, u, v, w) over the whole halo-1 array (NULL = rho 1 at rest, the reference's start), and the
+ * convective-outlet history f_convec_bc(:, :, q) = f_q(:, :, nz) (may be NULL). */
+int mflbm_sp_init_fields(const mflbm_sp_params* params, const double* rho0, const double* u0, const double* v0,
+                         const double* w0, double* const* f, double* f_convec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,178 @@
+// Pieces shared by the two-phase and the single-phase gfx950 paths: the padded dense layout, the compact
+// node set the populations are stored on, the XCD-aware block order, host-side halo packing and the error
+// plumbing of the C ABI.
+//
+// Dense layout (every dense field: phase field, normals, walls, macroscopic fields):
+//
+//   offset(i, j, k) = (i + XO) + (j + YO) * px + (k + ZO) * px * py      (1-based interior indices)
+//
+// with XO = 15 so that i = 1 sits on a 128-byte boundary, px = round_up(nx + 20, 16), py = ny + 8,
+// pz = nz + 8 (at least the 4-layer halo of the phase field in every direction).
+//
+// Compact population storage: one slot per interior fluid node (compact indices [0, nf), x-fastest order)
+// followed by the "support" nodes: every non-fluid node a fluid node reaches in one lattice link (solid
+// neighbours, whose slots carry the wall reflection of the AA update, and halo nodes) and every node of the
+// planes the boundary conditions and the periodic exchange address.  A dense int32 map gives the compact
+// index of a padded offset (-1 where nothing is stored).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lattice.h"
+#include "mflbm.h"
+
+namespace mflbm_dev {
+
+constexpr int XO = 15, YO = 3, ZO = 3;
+constexpr int BLOCK = 256;
+constexpr int NXCD = 8;   // gfx950: 8 accelerator complex dies, work-groups dispatched round-robin
+
+struct Layout {
+    int nx, ny, nz;
+    int px, pxy;   // row and plane pitch (elements)
+    long total;    // elements per dense array
+    __host__ __device__ int at(int i, int j, int k) const { return (i + XO) + (j + YO) * px + (k + ZO) * pxy; }
+    __host__ __device__ int off(int q) const { return lat::EX[q] + lat::EY[q] * px + lat::EZ[q] * pxy; }
+};
+
+inline Layout make_layout(int nx, int ny, int nz) {
+    Layout L{};
+    L.nx = nx;
+    L.ny = ny;
+    L.nz = nz;
+    L.px = ((nx + 20 + 15) / 16) * 16;
+    L.pxy = L.px * (ny + 8);
+    L.total = (long)L.pxy * (nz + 8);
+    return L;
+}
+
+// Logical block index with each XCD taking one contiguous range of blocks: node lists are in x-fastest
+// order, so a die then works on a z-slab and stencil neighbours stay in its own L2.  The grid is launched
+// as NXCD * per blocks.
+__device__ __forceinline__ int xcd_block(int per) { return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD); }
+
+inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+// grid of a list kernel that uses xcd_block(): per = blocks per die
+inline int xcd_per(long n) { return (int)((nblocks(n) + NXCD - 1) / NXCD); }
+
+// copy between the reference's halo-h layout (host) and the padded device layout (one array)
+template <typename T>
+void pack_host(const Layout& L, int h, const T* src, T* dst_padded) {
+    const int sx = L.nx + 2 * h, sy = L.ny + 2 * h;
+    for (int k = 1 - h; k <= L.nz + h; ++k)
+        for (int j = 1 - h; j <= L.ny + h; ++j) {
+            const T* s = src + (long)(j + h - 1) * sx + (long)(k + h - 1) * sx * sy;
+            std::memcpy(dst_padded + L.at(1 - h, j, k), s, sizeof(T) * sx);
+        }
+}
+template <typename T>
+void unpack_host(const Layout& L, int h, const T* src_padded, T* dst) {
+    const int sx = L.nx + 2 * h, sy = L.ny + 2 * h;
+    for (int k = 1 - h; k <= L.nz + h; ++k)
+        for (int j = 1 - h; j <= L.ny + h; ++j) {
+            T* d = dst + (long)(j + h - 1) * sx + (long)(k + h - 1) * sx * sy;
+            std::memcpy(d, src_padded + L.at(1 - h, j, k), sizeof(T) * sx);
+        }
+}
+
+// Compact node set of a wall array (padded layout, walls != 0 is not fluid).
+//   open_z: the z faces carry inlet/outlet conditions (planes k = 0, 1, 2, nz-1, nz, nz+1 are addressed)
+//   kper / jper: periodic exchange along z / y (planes 0, 1, n, n+1)
+//   extra_z: further z planes addressed by a boundary condition (e.g. a porous plate and its neighbours)
+struct CompactSet {
+    std::vector<int> cidx;   // padded offset -> compact index or -1
+    std::vector<int> cpos;   // compact index -> padded offset
+    int nfl = 0;             // interior fluid nodes (the first nfl entries)
+};
+
+inline void build_compact(const Layout& L, const std::vector<int8_t>& wp, bool open_z, bool kper, bool jper,
+                          const std::vector<int>& extra_z, CompactSet& cs) {
+    cs.cidx.assign(L.total, -1);
+    cs.cpos.clear();
+    cs.cpos.reserve((size_t)L.nx * L.ny * L.nz);
+    for (int k = 1; k <= L.nz; ++k)
+        for (int j = 1; j <= L.ny; ++j)
+            for (int i = 1; i <= L.nx; ++i)
+                if (wp[L.at(i, j, k)] == 0) {
+                    cs.cidx[L.at(i, j, k)] = (int)cs.cpos.size();
+                    cs.cpos.push_back(L.at(i, j, k));
+                }
+    cs.nfl = (int)cs.cpos.size();
+    std::vector<uint8_t> need(L.total, 0);
+    for (int m = 0; m < cs.nfl; ++m)
+        for (int q = 1; q < 19; ++q) need[cs.cpos[m] + L.off(q)] = 1;
+    auto plane_z = [&](int k) {
+        for (int j = 0; j <= L.ny + 1; ++j)
+            for (int i = 0; i <= L.nx + 1; ++i) need[L.at(i, j, k)] = 1;
+    };
+    auto plane_y = [&](int j) {
+        for (int k = 0; k <= L.nz + 1; ++k)
+            for (int i = 0; i <= L.nx + 1; ++i) need[L.at(i, j, k)] = 1;
+    };
+    if (open_z)
+        for (int k : {0, 1, 2, L.nz - 1, L.nz, L.nz + 1}) plane_z(k);
+    if (kper)
+        for (int k : {0, 1, L.nz, L.nz + 1}) plane_z(k);
+    if (jper)
+        for (int j : {0, 1, L.ny, L.ny + 1}) plane_y(j);
+    for (int k : extra_z)
+        if (k >= 0 && k <= L.nz + 1) plane_z(k);
+    for (int k = 0; k <= L.nz + 1; ++k)
+        for (int j = 0; j <= L.ny + 1; ++j)
+            for (int i = 0; i <= L.nx + 1; ++i) {
+                const int o = L.at(i, j, k);
+                if (need[o] && cs.cidx[o] < 0) {
+                    cs.cidx[o] = (int)cs.cpos.size();
+                    cs.cpos.push_back(o);
+                }
+            }
+}
+
+// message of the last failed create call (contexts keep their own)
+inline thread_local std::string g_create_error;
+
+// record an error on a context (or, for a NULL context, as the create error) and return its code
+inline int vfail(std::string* dst, int code, const char* fmt, va_list ap) {
+    char buf[512];
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    *(dst ? dst : &g_create_error) = buf;
+    return code;
+}
+template <class Ctx>
+int fail(Ctx* c, int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    const int r = vfail(c ? &c->err : nullptr, code, fmt, ap);
+    va_end(ap);
+    return r;
+}
+inline int fail(std::nullptr_t, int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    const int r = vfail(nullptr, code, fmt, ap);
+    va_end(ap);
+    return r;
+}
+
+}  // namespace mflbm_dev
+
+#define HIPCHK(c, call)                                                                                      \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH(c, kern, grid, ...)                                                                           \
+    do {                                                                                                     \
+        if ((grid) > 0) {                                                                                    \
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, (c)->stream, __VA_ARGS__);                  \
+            hipError_t e_ = hipGetLastError();                                                               \
+            if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(e_)); \
+        }                                                                                                    \
+    } while (0)

@@ -421,3 +421,51 @@ extern "C" int mflbm_inlet_profile(const mflbm_params* p, double uin_avg, double
         }
     return MFLBM_OK;
 }
+
+extern "C" int mflbm_sp_init_fields(const mflbm_sp_params* p, const double* rho0, const double* u0, const double* v0,
+                                    const double* w0, double* const* f, double* f_convec) {
+    // initialization_new / initialization_new_pdf (singlephase_3D/0.src/Initialization.F90:207-307): the
+    // second-order equilibrium at (rho, u, v, w) over the whole halo-1 array, each population written as
+    // the reference writes it (e.g. f7 = rho w2 + rho w2 (3 (u+v) + 4.5 (u+v)(u+v) - 1.5 |u|^2)).
+    if (!p || p->nx < 3 || p->ny < 3 || p->nz < 3 || !f) return MFLBM_ERR_ARG;
+    for (int q = 0; q < 19; ++q)
+        if (!f[q]) return MFLBM_ERR_ARG;
+    const Grid G(p->nx, p->ny, p->nz, 1);
+    const long n = G.size();
+    const double w0e = lat::W0, w1 = lat::W1, w2 = lat::W2;
+    for (long t = 0; t < n; ++t) {
+        const double u = u0 ? u0[t] : 0.0, v = v0 ? v0[t] : 0.0, w = w0 ? w0[t] : 0.0;
+        const double rho1 = rho0 ? rho0[t] : 1.0;
+        const double usqrt = u * u + v * v + w * w;
+        // axis directions: (component, sign) of e_q . u
+        auto axis = [&](double c, double cu) { return rho1 * w1 + rho1 * w1 * (3.0 * cu + 4.5 * c * c - 1.5 * usqrt); };
+        auto diag = [&](double cu) { return rho1 * w2 + rho1 * w2 * (3.0 * cu + 4.5 * cu * cu - 1.5 * usqrt); };
+        f[0][t] = rho1 * w0e + rho1 * w0e * (-1.5 * usqrt);
+        f[1][t] = axis(u, u);
+        f[2][t] = axis(u, -u);
+        f[3][t] = axis(v, v);
+        f[4][t] = axis(v, -v);
+        f[5][t] = axis(w, w);
+        f[6][t] = axis(w, -w);
+        f[7][t] = diag(u + v);
+        f[8][t] = diag(-u + v);
+        f[9][t] = diag(u - v);
+        f[10][t] = diag(-u - v);
+        f[11][t] = diag(u + w);
+        f[12][t] = diag(-u + w);
+        f[13][t] = diag(u - w);
+        f[14][t] = diag(-u - w);
+        f[15][t] = diag(v + w);
+        f[16][t] = diag(-v + w);
+        f[17][t] = diag(v - w);
+        f[18][t] = diag(-v - w);
+    }
+    if (f_convec) {
+        // f_convec_bc(i, j, q) = f_q(i, j, nz) over the halo-1 plane (Initialization.F90:281-304)
+        const long pl = (long)(p->nx + 2) * (p->ny + 2);
+        for (int q = 0; q < 19; ++q)
+            for (int j = 0; j <= p->ny + 1; ++j)
+                for (int i = 0; i <= p->nx + 1; ++i) f_convec[i + (long)j * (p->nx + 2) + q * pl] = f[q][G.at(i, j, p->nz)];
+    }
+    return MFLBM_OK;
+}

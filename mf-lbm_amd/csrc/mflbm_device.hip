@@ -1,22 +1,13 @@
 // gfx950 implementation of MF-LBM's per-time-step lattice update (two-colour D3Q19 MRT, FP64) and its
 // C ABI.
 //
-// Storage.  Dense fields (phi, interface normal, curvature, walls, macroscopic fields) live on one padded
-// grid shared by all of them,
-//
-//   offset(i, j, k) = (i + XO) + (j + YO) * px + (k + ZO) * px * py      (1-based interior indices)
-//
-// with XO = 15 so that i = 1 sits on a 128-byte boundary, px = round_up(nx + 20, 16), py = ny + 8,
-// pz = nz + 8 (every array carries at least the 4-layer halo of the phase field).
-//
-// The 38 population arrays (f0..f18 fluid 1, g0..g18 fluid 2) are stored compactly: one slot per interior
-// fluid node (compact indices [0, nf), in x-fastest order) followed by "support" nodes: every non-fluid node
-// that the AA update reaches from a fluid node (solid neighbours, whose slots carry the wall reflection, and
-// halo nodes) and every node of the boundary-condition and periodic-exchange planes.  A dense int32 map gives
-// the compact index of a padded offset.  A slot is therefore stored exactly where the reference keeps a value
-// that is ever read, so the update is the reference's, slot for slot, while HBM traffic follows the fluid
-// nodes instead of whole cache lines of a mostly solid grid.  A dense copy of all 38 arrays is kept only
-// as the staging area of upload/download.
+// Storage (device_common.h): dense fields (phi, interface normal, curvature, walls, macroscopic fields) on
+// one padded grid; the 38 population arrays (f0..f18 fluid 1, g0..g18 fluid 2) compact over the interior
+// fluid nodes plus the support nodes the update, the boundary conditions and the periodic exchange
+// address.  A slot is therefore stored exactly where the reference keeps a value that is ever read, so the
+// update is the reference's, slot for slot, while HBM traffic follows the fluid nodes instead of whole
+// cache lines of a mostly solid grid.  A dense copy of all 38 arrays is kept only as the staging area of
+// upload/download.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,22 +20,13 @@
 #include <vector>
 
 #include "collide.h"
+#include "device_common.h"
 #include "lattice.h"
 #include "mflbm.h"
 
+using namespace mflbm_dev;
+
 namespace {
-
-constexpr int XO = 15, YO = 3, ZO = 3;
-constexpr int BLOCK = 256;
-constexpr int NXCD = 8;   // gfx950: 8 accelerator complex dies, work-groups dispatched round-robin
-
-struct Layout {
-    int nx, ny, nz;
-    int px, pxy;   // row and plane pitch (elements)
-    long total;    // elements per dense array
-    __host__ __device__ int at(int i, int j, int k) const { return (i + XO) + (j + YO) * px + (k + ZO) * pxy; }
-    __host__ __device__ int off(int q) const { return lat::EX[q] + lat::EY[q] * px + lat::EZ[q] * pxy; }
-};
 
 // device view of the state passed to kernels by value
 struct Dev {
@@ -81,19 +63,39 @@ __device__ __forceinline__ Pop fpop(const Dev& d) { return Pop{d.fc, d.cs, d.map
 __device__ __forceinline__ Pop gpop(const Dev& d) { return Pop{d.fc + 19 * d.cs, d.cs, d.map}; }
 __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
-// Logical block index with each XCD taking one contiguous range of blocks: node lists are in x-fastest
-// order, so a die then works on a z-slab and stencil neighbours stay in its own L2.  The grid is launched
-// as NXCD * per blocks.
-__device__ __forceinline__ int xcd_block(int per) { return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD); }
 
 // ------------------------------------------------------------------------------------------------------
 // lattice update: collision + recolouring + AA-pattern streaming of both colours, one thread per fluid node
 // ------------------------------------------------------------------------------------------------------
 
+// Tuning knobs of the lattice-update kernel (build-time): the minimum number of waves per SIMD the register
+// allocation must allow, and whether population traffic uses non-temporal (streaming) loads and stores.
+#ifndef MFLBM_COLLIDE_WAVES
+#define MFLBM_COLLIDE_WAVES 2
+#endif
+#ifndef MFLBM_COLLIDE_NT
+#define MFLBM_COLLIDE_NT 0
+#endif
+__device__ __forceinline__ double ld_pop(const double* p) {
+#if MFLBM_COLLIDE_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void st_pop(double* p, double v) {
+#if MFLBM_COLLIDE_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // ODD = false: kernel_even_color (Kernel_multiphase.F90:371-725): read swapped own slots, write own slots.
 // ODD = true : kernel_odd_color  (Kernel_multiphase.F90:6-362): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
 template <bool ODD>
-__global__ void __launch_bounds__(BLOCK) k_collide(Dev d, CollideParams P, int per) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MFLBM_COLLIDE_WAVES, 8)))
+k_collide(Dev d, CollideParams P, int per) {
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const long cs = d.cs;
@@ -101,8 +103,8 @@ __global__ void __launch_bounds__(BLOCK) k_collide(Dev d, CollideParams P, int p
     double* __restrict__ ga = d.fc + 19 * cs;
     double a[19], b[19];
     int nb[19];
-    a[0] = fa[m];
-    b[0] = ga[m];
+    a[0] = ld_pop(fa + m);
+    b[0] = ld_pop(ga + m);
     if (ODD) {
         const int n = d.cpos[m];
 #pragma unroll
@@ -110,27 +112,27 @@ __global__ void __launch_bounds__(BLOCK) k_collide(Dev d, CollideParams P, int p
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
             const long s = q * cs + nb[lat::OPP[q]];   // slot q of the upwind node x - e_q
-            a[q] = fa[s];
-            b[q] = ga[s];
+            a[q] = ld_pop(fa + s);
+            b[q] = ld_pop(ga + s);
         }
     } else {
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
             const long s = lat::OPP[q] * cs + m;
-            a[q] = fa[s];
-            b[q] = ga[s];
+            a[q] = ld_pop(fa + s);
+            b[q] = ld_pop(ga + s);
         }
     }
     const double cnx = d.force[m], cny = d.force[d.fs + m], cnz = d.force[2 * d.fs + m], tf = d.force[3 * d.fs + m];
     const double phi = collide_node_tf(a, b, cnx, cny, cnz, tf, P);
     d.phi[d.cpos[m]] = phi;
-    fa[m] = a[0];
-    ga[m] = b[0];
+    st_pop(fa + m, a[0]);
+    st_pop(ga + m, b[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const long s = ODD ? lat::OPP[q] * cs + nb[q] : q * cs + m;   // odd: slot opp(q) of x + e_q
-        fa[s] = a[q];
-        ga[s] = b[q];
+        st_pop(fa + s, a[q]);
+        st_pop(ga + s, b[q]);
     }
 }
 
@@ -605,9 +607,39 @@ __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, i
         if (mk & (1u << q)) acc = acc + d.phi[n + d.L.off(q)] * lat::W[q];
     d.phi[n] = acc / s.w[t];
 }
-// interface normal and |grad phi| at the non-solid nodes of [-1, n+2]^3 (Phase_gradient.F90:33-79); solid
-// nodes keep 0 or their extrapolated value, as after the reference's zero-then-extrapolate sequence
-__global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, int per) {
+// wetting: turn the normal at a fluid node next to the wall to the contact angle (Akai et al. 2018;
+// alter_color_gradient_solid_surface, Phase_gradient.F90:210-265).  cos/sin of the angle come from the host.
+__device__ __forceinline__ void wet(double& cx, double& cy, double& cz, double nwx, double nwy, double nwz, double tmpCos,
+                                    double sinT) {
+    const double cnx0 = cx, cny0 = cy, cnz0 = cz;
+    const double tmp1 = nwx * cnx0 + nwy * cny0 + nwz * cnz0;
+    const double tmp2 = 1.0 / sqrt(1.0 - tmp1 * tmp1);
+    const double coe1 = sinT * tmp1 * tmp2;
+    const double coe2 = sinT * tmp2;
+    const double cnxp = (tmpCos - coe1) * nwx + coe2 * cnx0;
+    const double cnyp = (tmpCos - coe1) * nwy + coe2 * cny0;
+    const double cnzp = (tmpCos - coe1) * nwz + coe2 * cnz0;
+    const double cnxm = (tmpCos + coe1) * nwx - coe2 * cnx0;
+    const double cnym = (tmpCos + coe1) * nwy - coe2 * cny0;
+    const double cnzm = (tmpCos + coe1) * nwz - coe2 * cnz0;
+    const double distP = (cnxp - cnx0) * (cnxp - cnx0) + (cnyp - cny0) * (cnyp - cny0) + (cnzp - cnz0) * (cnzp - cnz0);
+    const double distM = (cnxm - cnx0) * (cnxm - cnx0) + (cnym - cny0) * (cnym - cny0) + (cnzm - cnz0) * (cnzm - cnz0);
+    if (distP <= distM) {
+        cx = cnxp;
+        cy = cnyp;
+        cz = cnzp;
+    } else {
+        cx = cnxm;
+        cy = cnym;
+        cz = cnzm;
+    }
+}
+// interface normal and |grad phi| at the non-solid nodes of [-1, n+2]^3 (Phase_gradient.F90:33-79), with the
+// wetting rotation applied at once where the node is a fluid boundary node (the rotation reads and writes only
+// the node's own normal, so doing it here equals the reference's separate pass).  wb[t] indexes the fluid
+// boundary list or is -1.  Solid nodes keep 0 or their extrapolated value, as after the reference's
+// zero-then-extrapolate sequence.
+__global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl, int per) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= nl.n) return;
     const Layout L = d.L;
@@ -625,43 +657,13 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, int per) 
         cx = cx / cn;
         cy = cy / cn;
         cz = cz / cn;
+        const int b = wb[t];
+        if (b >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * b], fl.nw[3 * b + 1], fl.nw[3 * b + 2], fl.cos_t[b], fl.sin_t[b]);
     }
     d.cnx[n] = cx;
     d.cny[n] = cy;
     d.cnz[n] = cz;
     d.cnorm[n] = cn;
-}
-// wetting: turn the normal at fluid nodes next to the wall to the contact angle (Akai et al. 2018;
-// alter_color_gradient_solid_surface, Phase_gradient.F90:210-265).  cos/sin of the angle come from the host.
-__global__ void k_wetting(Dev d, FluidList fl) {
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= fl.n) return;
-    const int n = fl.pos[t];
-    if (!(d.cnorm[n] > 1e-6)) return;
-    const double nwx = fl.nw[3 * t], nwy = fl.nw[3 * t + 1], nwz = fl.nw[3 * t + 2];
-    const double cnx0 = d.cnx[n], cny0 = d.cny[n], cnz0 = d.cnz[n];
-    const double tmpCos = fl.cos_t[t], sinT = fl.sin_t[t];
-    const double tmp1 = nwx * cnx0 + nwy * cny0 + nwz * cnz0;
-    const double tmp2 = 1.0 / sqrt(1.0 - tmp1 * tmp1);
-    const double coe1 = sinT * tmp1 * tmp2;
-    const double coe2 = sinT * tmp2;
-    const double cnxp = (tmpCos - coe1) * nwx + coe2 * cnx0;
-    const double cnyp = (tmpCos - coe1) * nwy + coe2 * cny0;
-    const double cnzp = (tmpCos - coe1) * nwz + coe2 * cnz0;
-    const double cnxm = (tmpCos + coe1) * nwx - coe2 * cnx0;
-    const double cnym = (tmpCos + coe1) * nwy - coe2 * cny0;
-    const double cnzm = (tmpCos + coe1) * nwz - coe2 * cnz0;
-    const double distP = (cnxp - cnx0) * (cnxp - cnx0) + (cnyp - cny0) * (cnyp - cny0) + (cnzp - cnz0) * (cnzp - cnz0);
-    const double distM = (cnxm - cnx0) * (cnxm - cnx0) + (cnym - cny0) * (cnym - cny0) + (cnzm - cnz0) * (cnzm - cnz0);
-    if (distP <= distM) {
-        d.cnx[n] = cnxp;
-        d.cny[n] = cnyp;
-        d.cnz[n] = cnzp;
-    } else {
-        d.cnx[n] = cnxm;
-        d.cny[n] = cnym;
-        d.cnz[n] = cnzm;
-    }
 }
 // normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110)
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, int per) {
@@ -833,11 +835,7 @@ __global__ void k_copy_phi_old(Dev d, double* phi_old) {
 // host side
 // ------------------------------------------------------------------------------------------------------
 
-inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
-// grid of a list kernel that uses xcd_block(): per = blocks per die
-inline int xcd_per(long n) { return (int)((nblocks(n) + NXCD - 1) / NXCD); }
 
-thread_local std::string g_create_error;
 
 }  // namespace
 
@@ -863,6 +861,7 @@ struct mflbm_ctx {
     double* force = nullptr;    // 4 compact force arrays
     int* map = nullptr;
     int* ibuf = nullptr;        // cpos, normal list, solid lists, fluid list positions
+    int* wbuf = nullptr;        // per normal-list node: index into the fluid boundary list or -1
     unsigned* mbuf = nullptr;   // solid list masks
     double* dbuf = nullptr;     // solid weights, fluid list normals / cos / sin
     SolidList sl{}, sl1{};
@@ -879,62 +878,15 @@ struct mflbm_ctx {
 
 namespace {
 
-int fail(mflbm_ctx* c, int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    if (c) c->err = buf;
-    else g_create_error = buf;
-    return code;
-}
-
-#define HIPCHK(c, call)                                                                                      \
-    do {                                                                                                     \
-        hipError_t e_ = (call);                                                                              \
-        if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
-
-#define LAUNCH(c, kern, grid, ...)                                                                           \
-    do {                                                                                                     \
-        if ((grid) > 0) {                                                                                    \
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, (c)->stream, __VA_ARGS__);                  \
-            hipError_t e_ = hipGetLastError();                                                               \
-            if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(e_)); \
-        }                                                                                                    \
-    } while (0)
-
 CollideParams collide_params(const mflbm_params& p) {
     return CollideParams{p.gamma, p.beta, p.force_z, p.la_nui1, p.la_nui2};
 }
 
 bool open_z(const mflbm_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
 
-// copy between the reference's halo-h layout (host) and the padded device layout (one array)
-template <typename T>
-void pack_host(const Layout& L, int h, const T* src, T* dst_padded) {
-    const int sx = L.nx + 2 * h, sy = L.ny + 2 * h;
-    for (int k = 1 - h; k <= L.nz + h; ++k)
-        for (int j = 1 - h; j <= L.ny + h; ++j) {
-            const T* s = src + (long)(j + h - 1) * sx + (long)(k + h - 1) * sx * sy;
-            std::memcpy(dst_padded + L.at(1 - h, j, k), s, sizeof(T) * sx);
-        }
-}
-template <typename T>
-void unpack_host(const Layout& L, int h, const T* src_padded, T* dst) {
-    const int sx = L.nx + 2 * h, sy = L.ny + 2 * h;
-    for (int k = 1 - h; k <= L.nz + h; ++k)
-        for (int j = 1 - h; j <= L.ny + h; ++j) {
-            T* d = dst + (long)(j + h - 1) * sx + (long)(k + h - 1) * sx * sy;
-            std::memcpy(d, src_padded + L.at(1 - h, j, k), sizeof(T) * sx);
-        }
-}
-
 int launch_gradient(mflbm_ctx* c) {
     LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
-    LAUNCH(c, k_normals, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, xcd_per(c->nrm.n));
-    LAUNCH(c, k_wetting, nblocks(c->fl.n), c->d, c->fl);
+    LAUNCH(c, k_normals, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
     LAUNCH(c, k_normal_extrapolate, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
     LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     return MFLBM_OK;
@@ -1002,13 +954,14 @@ int launch_step(mflbm_ctx* c, int ntime) {
 }
 
 void free_geometry(mflbm_ctx* c) {
-    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->mbuf, c->dbuf};
+    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
     c->force = nullptr;
     c->map = nullptr;
     c->ibuf = nullptr;
+    c->wbuf = nullptr;
     c->mbuf = nullptr;
     c->dbuf = nullptr;
     c->have_geom = false;
@@ -1033,12 +986,7 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     if (!c) return fail(nullptr, MFLBM_ERR_NOMEM, "context allocation failed");
     c->p = *prm;
     Layout& L = c->L;
-    L.nx = prm->nx;
-    L.ny = prm->ny;
-    L.nz = prm->nz;
-    L.px = ((prm->nx + 20 + 15) / 16) * 16;
-    L.pxy = L.px * (prm->ny + 8);
-    L.total = (long)L.pxy * (prm->nz + 8);
+    L = make_layout(prm->nx, prm->ny, prm->nz);
     if (L.total > (1L << 31) - 1) {
         delete c;
         return fail(nullptr, MFLBM_ERR_ARG, "lattice too large for 32-bit node offsets");
@@ -1141,52 +1089,15 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     }
     std::vector<int8_t> wp(L.total, 1);
     pack_host(L, 2, walls_h2, wp.data());
-    // node classes: interior fluid nodes; support nodes = non-fluid nodes of the population halo region
-    // [0, n+1]^3 that a fluid node reaches in one lattice link, plus every node of the planes that the
-    // boundary conditions and the periodic exchange address
-    std::vector<int> cidx(L.total, -1);
-    std::vector<int> cpos;
-    cpos.reserve((size_t)L.nx * L.ny * L.nz);
-    for (int k = 1; k <= L.nz; ++k)
-        for (int j = 1; j <= L.ny; ++j)
-            for (int i = 1; i <= L.nx; ++i)
-                if (wp[L.at(i, j, k)] == 0) {
-                    cidx[L.at(i, j, k)] = (int)cpos.size();
-                    cpos.push_back(L.at(i, j, k));
-                }
-    const int nfl = (int)cpos.size();
-    std::vector<uint8_t> need(L.total, 0);
-    for (int m = 0; m < nfl; ++m)
-        for (int q = 1; q < 19; ++q) need[cpos[m] + L.off(q)] = 1;
-    auto plane_z = [&](int k) {
-        for (int j = 0; j <= L.ny + 1; ++j)
-            for (int i = 0; i <= L.nx + 1; ++i) need[L.at(i, j, k)] = 1;
-    };
-    auto plane_y = [&](int j) {
-        for (int k = 0; k <= L.nz + 1; ++k)
-            for (int i = 0; i <= L.nx + 1; ++i) need[L.at(i, j, k)] = 1;
-    };
-    if (open_z(p)) {
-        for (int k : {0, 1, 2, L.nz - 1, L.nz, L.nz + 1}) plane_z(k);
-    }
-    if (p.kper == 1) {
-        for (int k : {0, 1, L.nz, L.nz + 1}) plane_z(k);
-    }
-    if (p.jper == 1) {
-        for (int j : {0, 1, L.ny, L.ny + 1}) plane_y(j);
-    }
-    if ((p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2) && p.z_porous_plate >= 1 && p.z_porous_plate <= L.nz) {
-        for (int k : {p.z_porous_plate - 1, p.z_porous_plate, p.z_porous_plate + 1}) plane_z(k);
-    }
-    for (int k = 0; k <= L.nz + 1; ++k)
-        for (int j = 0; j <= L.ny + 1; ++j)
-            for (int i = 0; i <= L.nx + 1; ++i) {
-                const int o = L.at(i, j, k);
-                if (need[o] && cidx[o] < 0) {
-                    cidx[o] = (int)cpos.size();
-                    cpos.push_back(o);
-                }
-            }
+    // compact node set: interior fluid nodes, then the support nodes (see device_common.h)
+    std::vector<int> extra_z;
+    if ((p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2) && p.z_porous_plate >= 1 && p.z_porous_plate <= L.nz)
+        extra_z = {p.z_porous_plate - 1, p.z_porous_plate, p.z_porous_plate + 1};
+    CompactSet cset;
+    build_compact(L, wp, open_z(p), p.kper == 1, p.jper == 1, extra_z, cset);
+    const std::vector<int>& cidx = cset.cidx;
+    const std::vector<int>& cpos = cset.cpos;
+    const int nfl = cset.nfl;
     const int nc = (int)cpos.size();
     // nodes whose normal is computed: walls == 0 in [-1, n+2]^3
     std::vector<int> nrm;
@@ -1217,6 +1128,14 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
         fd[3 * (size_t)nf + t] = std::cos(f_theta[t]);
         fd[4 * (size_t)nf + t] = std::sin(f_theta[t]);
     }
+    // fluid boundary nodes are rotated inside the normal kernel: index of each normal-list node in the list
+    // (nodes of the list that are solid in the wall array get a zero normal in the reference, so no rotation)
+    std::vector<int> wb(nrm.size(), -1);
+    {
+        std::vector<int> at(L.total, -1);
+        for (int t = 0; t < nf; ++t) at[fpos[t]] = t;
+        for (size_t t = 0; t < nrm.size(); ++t) wb[t] = at[nrm[t]];
+    }
     // device buffers
     free_geometry(c);
     const long cs = (((long)nc + 63) / 64) * 64;
@@ -1227,6 +1146,8 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMalloc(&c->force, sizeof(double) * fs * 4));
     HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
     HIPCHK(c, hipMalloc(&c->ibuf, sizeof(int) * ni));
+    HIPCHK(c, hipMalloc(&c->wbuf, sizeof(int) * (wb.size() + 1)));
+    if (!wb.empty()) HIPCHK(c, hipMemcpy(c->wbuf, wb.data(), sizeof(int) * wb.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->mbuf, sizeof(unsigned) * ((size_t)ns + s1mask.size() + 1)));
     HIPCHK(c, hipMalloc(&c->dbuf, sizeof(double) * nd));
     HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * cs * 38));

@@ -1,0 +1,737 @@
+// gfx950 implementation of MF-LBM's single-phase time step (D3Q19 MRT, FP64, singlephase_3D/0.src) and its
+// C ABI (the mflbm_sp_* entry points of mflbm.h).
+//
+// Storage follows the two-phase path (device_common.h): the 19 population arrays are compact over the
+// interior fluid nodes plus the support nodes the update, the open boundaries and the periodic exchange
+// address; macroscopic fields and walls live on the padded dense grid.  One thread updates one fluid node;
+// the AA pattern makes every node read and write the same slot set, so the update has no races and the
+// result does not depend on the thread order (Kernel.F90:5-400).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "collide_sp.h"
+#include "device_common.h"
+#include "lattice.h"
+#include "mflbm.h"
+
+using namespace mflbm_dev;
+
+namespace {
+
+struct SDev {
+    Layout L;
+    double* fc;       // compact populations, direction q at fc + q*cs
+    long cs;
+    const int* map;   // padded offset -> compact index
+    const int* cpos;  // compact index -> padded offset
+    int nf;           // interior fluid nodes
+    int nc;           // all compact nodes
+    const int8_t* walls;
+    double* w_in;     // (nx+2)(ny+2), reference layout
+    double* cf;       // convective-outlet history: 19 planes (nx+2)(ny+2)
+};
+
+struct SPop {
+    double* base;
+    long cs;
+    const int* map;
+    __device__ __forceinline__ double& operator()(int q, int o) const { return base[q * cs + map[o]]; }
+};
+__device__ __forceinline__ SPop spop(const SDev& d) { return SPop{d.fc, d.cs, d.map}; }
+__device__ __forceinline__ int plane_at(const SDev& d, int i, int j) { return i + j * (d.L.nx + 2); }
+
+// ODD = false: kernel_even (Kernel.F90:206-400): read the swapped own slots, write the own slots.
+// ODD = true : kernel_odd  (Kernel.F90:5-200): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
+template <bool ODD>
+__global__ void __launch_bounds__(BLOCK) k_sp_collide(SDev d, SPCollideParams P, int per) {
+    const int m = xcd_block(per) * BLOCK + threadIdx.x;
+    if (m >= d.nf) return;
+    const long cs = d.cs;
+    double* __restrict__ fa = d.fc;
+    double a[19];
+    int nb[19];
+    a[0] = fa[m];
+    if (ODD) {
+        const int n = d.cpos[m];
+#pragma unroll
+        for (int r = 1; r < 19; ++r) nb[r] = d.map[n + d.L.off(r)];
+#pragma unroll
+        for (int q = 1; q < 19; ++q) a[q] = fa[q * cs + nb[lat::OPP[q]]];   // slot q of the upwind node x - e_q
+    } else {
+#pragma unroll
+        for (int q = 1; q < 19; ++q) a[q] = fa[lat::OPP[q] * cs + m];
+    }
+    collide_single(a, P);
+    fa[m] = a[0];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const long s = ODD ? lat::OPP[q] * cs + nb[q] : q * cs + m;   // odd: slot opp(q) of x + e_q
+        fa[s] = a[q];
+    }
+}
+
+// dense staging <-> compact storage
+__global__ void k_sp_gather(SDev d, const double* __restrict__ dense, long ps) {
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= d.nc) return;
+    const int n = d.cpos[t];
+    for (int q = 0; q < 19; ++q) d.fc[q * d.cs + t] = dense[q * ps + n];
+}
+__global__ void k_sp_scatter(SDev d, double* __restrict__ dense, long ps) {
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= d.nc) return;
+    const int n = d.cpos[t];
+    for (int q = 0; q < 19; ++q) dense[q * ps + n] = d.fc[q * d.cs + t];
+}
+
+// ------------------------------------------------------------------------------------------------------
+// periodic wrap: the reference's MPI exchange of one rank with itself (Mpi.F90:73-448)
+// ------------------------------------------------------------------------------------------------------
+__constant__ int SZP[5] = {5, 11, 12, 15, 16};
+__constant__ int SZM[5] = {6, 14, 13, 18, 17};
+__constant__ int SYP[5] = {3, 7, 8, 15, 17};
+__constant__ int SYM[5] = {4, 10, 9, 16, 18};
+
+__device__ __forceinline__ void cp(const SDev& d, int q, int dst, int src) {
+    const SPop f = spop(d);
+    f(q, dst) = f(q, src);
+}
+// after an even step (mpi_pdf_pull_async_pack / _sync_update, Mpi.F90:73-256)
+__global__ void k_sp_pull_z(SDev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        cp(d, SZP[m], L.at(i, j, 0), L.at(i, j, L.nz));
+        cp(d, SZM[m], L.at(i, j, L.nz + 1), L.at(i, j, 1));
+    }
+}
+__global__ void k_sp_pull_y(SDev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.nz) return;
+    const int i = t % L.nx + 1, k = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        cp(d, SYP[m], L.at(i, 0, k), L.at(i, L.ny, k));
+        cp(d, SYM[m], L.at(i, L.ny + 1, k), L.at(i, 1, k));
+    }
+}
+__global__ void k_sp_pull_edge(SDev d) {
+    const Layout L = d.L;
+    const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
+    if (i > L.nx) return;
+    cp(d, 18, L.at(i, L.ny + 1, L.nz + 1), L.at(i, 1, 1));
+    cp(d, 16, L.at(i, L.ny + 1, 0), L.at(i, 1, L.nz));
+    cp(d, 17, L.at(i, 0, L.nz + 1), L.at(i, L.ny, 1));
+    cp(d, 15, L.at(i, 0, 0), L.at(i, L.ny, L.nz));
+}
+// after an odd step (mpi_pdf_push_async_pack / _sync_update, Mpi.F90:264-448): faces, then x-edges
+__global__ void k_sp_push_z(SDev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        cp(d, SZM[m], L.at(i, j, 1), L.at(i, j, L.nz + 1));
+        cp(d, SZP[m], L.at(i, j, L.nz), L.at(i, j, 0));
+    }
+}
+__global__ void k_sp_push_y(SDev d) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.nz) return;
+    const int i = t % L.nx + 1, k = t / L.nx + 1;
+    for (int m = 0; m < 5; ++m) {
+        cp(d, SYM[m], L.at(i, 1, k), L.at(i, L.ny + 1, k));
+        cp(d, SYP[m], L.at(i, L.ny, k), L.at(i, 0, k));
+    }
+}
+__global__ void k_sp_push_edge(SDev d) {
+    const Layout L = d.L;
+    const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
+    if (i > L.nx) return;
+    cp(d, 15, L.at(i, L.ny, L.nz), L.at(i, 0, 0));
+    cp(d, 17, L.at(i, L.ny, 1), L.at(i, 0, L.nz + 1));
+    cp(d, 16, L.at(i, 1, L.nz), L.at(i, L.ny + 1, 0));
+    cp(d, 18, L.at(i, 1, 1), L.at(i, L.ny + 1, L.nz + 1));
+}
+
+// ------------------------------------------------------------------------------------------------------
+// open boundaries (Boundary.F90), one thread per (i, j) of the face; `x * (1 - w) + y * w` blends are kept
+// literally (a solid face node keeps its value)
+// ------------------------------------------------------------------------------------------------------
+
+// inlet_bounce_back_velocity_BC_before_odd (Boundary.F90:5-37)
+__global__ void k_sp_inlet_velocity_before(SDev d, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const double s = d.w_in[plane_at(d, i, j)] * relaxation;
+    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
+    const SPop A = spop(d);
+    int o;
+    o = L.at(i, j, 0);     A(5, o) = (A(6, L.at(i, j, 1)) + c1 * s) * om + A(5, o) * wd;
+    o = L.at(i - 1, j, 0); A(11, o) = (A(14, L.at(i, j, 1)) + c2 * s) * om + A(11, o) * wd;
+    o = L.at(i + 1, j, 0); A(12, o) = (A(13, L.at(i, j, 1)) + c2 * s) * om + A(12, o) * wd;
+    o = L.at(i, j - 1, 0); A(15, o) = (A(18, L.at(i, j, 1)) + c2 * s) * om + A(15, o) * wd;
+    o = L.at(i, j + 1, 0); A(16, o) = (A(17, L.at(i, j, 1)) + c2 * s) * om + A(16, o) * wd;
+}
+// inlet_bounce_back_velocity_BC_after_odd (Boundary.F90:40-70)
+__global__ void k_sp_inlet_velocity_after(SDev d, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const double s = d.w_in[plane_at(d, i, j)] * relaxation;
+    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
+    const SPop A = spop(d);
+    const int o = L.at(i, j, 1);
+    A(6, o) = (A(5, L.at(i, j, 0)) + c1 * s) * om + A(6, o) * wd;
+    A(13, o) = (A(12, L.at(i + 1, j, 0)) + c2 * s) * om + A(13, o) * wd;
+    A(14, o) = (A(11, L.at(i - 1, j, 0)) + c2 * s) * om + A(14, o) * wd;
+    A(17, o) = (A(16, L.at(i, j + 1, 0)) + c2 * s) * om + A(17, o) * wd;
+    A(18, o) = (A(15, L.at(i, j - 1, 0)) + c2 * s) * om + A(18, o) * wd;
+}
+// inlet_Zou_He_pressure_BC_before_odd (Boundary.F90:76-132)
+__global__ void k_sp_inlet_pressure_before(SDev d, double rho_in, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const SPop A = spop(d);
+    const double s = (rho_in - (A(0, L.at(i, j, 1)) + A(1, L.at(i - 1, j, 1)) + A(2, L.at(i + 1, j, 1)) +
+                                A(3, L.at(i, j - 1, 1)) + A(4, L.at(i, j + 1, 1)) + A(7, L.at(i - 1, j - 1, 1)) +
+                                A(8, L.at(i + 1, j - 1, 1)) + A(9, L.at(i - 1, j + 1, 1)) + A(10, L.at(i + 1, j + 1, 1)) +
+                                2.0 * (A(6, L.at(i, j, 2)) + A(14, L.at(i + 1, j, 2)) + A(13, L.at(i - 1, j, 2)) +
+                                       A(18, L.at(i, j + 1, 2)) + A(17, L.at(i, j - 1, 2))))) * relaxation;
+    const double tnx = 0.5 * (A(1, L.at(i - 1, j, 1)) + A(7, L.at(i - 1, j - 1, 1)) + A(9, L.at(i - 1, j + 1, 1)) -
+                              (A(2, L.at(i + 1, j, 1)) + A(8, L.at(i + 1, j - 1, 1)) + A(10, L.at(i + 1, j + 1, 1))));
+    const double tny = 0.5 * (A(3, L.at(i, j - 1, 1)) + A(7, L.at(i - 1, j - 1, 1)) + A(8, L.at(i + 1, j - 1, 1)) -
+                              (A(4, L.at(i, j + 1, 1)) + A(10, L.at(i + 1, j + 1, 1)) + A(9, L.at(i - 1, j + 1, 1))));
+    int o;
+    o = L.at(i, j, 0);     A(5, o) = (A(6, L.at(i, j, 2)) + lat::ONE3 * s) * om + A(5, o) * wd;
+    o = L.at(i - 1, j, 0); A(11, o) = (A(14, L.at(i + 1, j, 2)) + lat::ONE6 * s - tnx) * om + A(11, o) * wd;
+    o = L.at(i + 1, j, 0); A(12, o) = (A(13, L.at(i - 1, j, 2)) + lat::ONE6 * s + tnx) * om + A(12, o) * wd;
+    o = L.at(i, j - 1, 0); A(15, o) = (A(18, L.at(i, j + 1, 2)) + lat::ONE6 * s - tny) * om + A(15, o) * wd;
+    o = L.at(i, j + 1, 0); A(16, o) = (A(17, L.at(i, j - 1, 2)) + lat::ONE6 * s + tny) * om + A(16, o) * wd;
+}
+// inlet_Zou_He_pressure_BC_after_odd (Boundary.F90:135-183)
+__global__ void k_sp_inlet_pressure_after(SDev d, double rho_in, double relaxation) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1;
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const SPop A = spop(d);
+    const int o = L.at(i, j, 1);
+    const double s = (rho_in - (A(0, o) + A(2, o) + A(1, o) + A(4, o) + A(3, o) + A(8, o) + A(7, o) + A(10, o) + A(9, o) +
+                                2.0 * (A(5, o) + A(11, o) + A(12, o) + A(15, o) + A(16, o)))) * relaxation;
+    const double tnx = 0.5 * (A(2, o) + A(8, o) + A(10, o) - (A(1, o) + A(7, o) + A(9, o)));
+    const double tny = 0.5 * (A(4, o) + A(9, o) + A(10, o) - (A(3, o) + A(8, o) + A(7, o)));
+    A(6, o) = (A(5, o) + lat::ONE3 * s) * om + A(6, o) * wd;
+    A(13, o) = (A(12, o) + lat::ONE6 * s + tnx) * om + A(13, o) * wd;
+    A(14, o) = (A(11, o) + lat::ONE6 * s - tnx) * om + A(14, o) * wd;
+    A(17, o) = (A(16, o) + lat::ONE6 * s + tny) * om + A(17, o) * wd;
+    A(18, o) = (A(15, o) + lat::ONE6 * s - tny) * om + A(18, o) * wd;
+}
+// outlet_convective_BC_before_odd (Boundary.F90:192-230)
+__global__ void k_sp_outlet_convective_before(SDev d, double u_convec) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const double temp = 1.0 / (1.0 + u_convec);
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const int pc = plane_at(d, i, j);
+    const long pl = (long)(L.nx + 2) * (L.ny + 2);
+    // (slot, x offset, y offset) of the five unknown populations at the outlet halo
+    const int QS[5] = {6, 13, 14, 17, 18};
+    const int DI[5] = {0, -1, 1, 0, 0};
+    const int DJ[5] = {0, 0, 0, -1, 1};
+    const SPop A = spop(d);
+    for (int m = 0; m < 5; ++m) {
+        const int o = L.at(i + DI[m], j + DJ[m], nz + 1);
+        const int in = L.at(i + DI[m], j + DJ[m], nz);
+        A(QS[m], o) = ((d.cf[pc + QS[m] * pl] + u_convec * A(QS[m], in)) * temp) * om + A(QS[m], o) * wd;
+    }
+    for (int m = 0; m < 5; ++m) d.cf[pc + QS[m] * pl] = A(QS[m], L.at(i + DI[m], j + DJ[m], nz + 1));
+}
+// outlet_convective_BC_after_odd (Boundary.F90:233-267)
+__global__ void k_sp_outlet_convective_after(SDev d, double u_convec) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const double temp = 1.0 / (1.0 + u_convec);
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    const int pc = plane_at(d, i, j);
+    const long pl = (long)(L.nx + 2) * (L.ny + 2);
+    // slot written at z = nz and the history slot it relaxes toward
+    const int QW[5] = {5, 11, 12, 15, 16};
+    const int QH[5] = {6, 14, 13, 18, 17};
+    const int o = L.at(i, j, nz), om1 = L.at(i, j, nz - 1);
+    const SPop A = spop(d);
+    for (int m = 0; m < 5; ++m)
+        A(QW[m], o) = ((d.cf[pc + QH[m] * pl] + u_convec * A(QW[m], om1)) * temp) * om + A(QW[m], o) * wd;
+    for (int m = 0; m < 5; ++m) d.cf[pc + QH[m] * pl] = A(QW[m], o);
+}
+// outlet_Zou_He_pressure_BC_before_odd (Boundary.F90:276-330)
+__global__ void k_sp_outlet_pressure_before(SDev d, double rho_out) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = 1.0 - (double)wi, wd = (double)wi;
+    const SPop A = spop(d);
+    const int a0 = L.at(i, j, nz), am = L.at(i - 1, j, nz), ap = L.at(i + 1, j, nz), bm = L.at(i, j - 1, nz),
+              bp = L.at(i, j + 1, nz), mm = L.at(i - 1, j - 1, nz), pm = L.at(i + 1, j - 1, nz),
+              mp = L.at(i - 1, j + 1, nz), pp = L.at(i + 1, j + 1, nz);
+    const int c1 = L.at(i, j, nz - 1), c2 = L.at(i - 1, j, nz - 1), c3 = L.at(i + 1, j, nz - 1), c4 = L.at(i, j - 1, nz - 1),
+              c5 = L.at(i, j + 1, nz - 1);
+    const double s = (A(0, a0) + A(1, am) + A(2, ap) + A(3, bm) + A(4, bp) + A(7, mm) + A(8, pm) + A(9, mp) + A(10, pp) +
+                      2.0 * (A(5, c1) + A(11, c2) + A(12, c3) + A(15, c4) + A(16, c5))) - rho_out;
+    const double tnx = 0.5 * (A(1, am) + A(7, mm) + A(9, mp) - (A(2, ap) + A(8, pm) + A(10, pp)));
+    const double tny = 0.5 * (A(3, bm) + A(7, mm) + A(8, pm) - (A(4, bp) + A(10, pp) + A(9, mp)));
+    int o;
+    o = L.at(i, j, nz + 1);     A(6, o) = (A(5, c1) - lat::ONE3 * s) * om + A(6, o) * wd;
+    o = L.at(i - 1, j, nz + 1); A(13, o) = (A(12, c3) - lat::ONE6 * s - tnx) * om + A(13, o) * wd;
+    o = L.at(i + 1, j, nz + 1); A(14, o) = (A(11, c2) - lat::ONE6 * s + tnx) * om + A(14, o) * wd;
+    o = L.at(i, j - 1, nz + 1); A(17, o) = (A(16, c5) - lat::ONE6 * s - tny) * om + A(17, o) * wd;
+    o = L.at(i, j + 1, nz + 1); A(18, o) = (A(15, c4) - lat::ONE6 * s + tny) * om + A(18, o) * wd;
+}
+// outlet_Zou_He_pressure_BC_after_odd (Boundary.F90:333-389)
+__global__ void k_sp_outlet_pressure_after(SDev d, double rho_out) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = 1.0 - (double)wi, wd = (double)wi;
+    const SPop A = spop(d);
+    const int o = L.at(i, j, nz);
+    const double s = (A(0, o) + A(2, o) + A(1, o) + A(4, o) + A(3, o) + A(8, o) + A(7, o) + A(10, o) + A(9, o) +
+                      2.0 * (A(6, o) + A(14, o) + A(13, o) + A(18, o) + A(17, o))) - rho_out;
+    const double tnx = 0.5 * (A(2, o) + A(8, o) + A(10, o) - (A(1, o) + A(7, o) + A(9, o)));
+    const double tny = 0.5 * (A(4, o) + A(10, o) + A(9, o) - (A(3, o) + A(7, o) + A(8, o)));
+    A(5, o) = (A(6, o) - lat::ONE3 * s) * om + A(5, o) * wd;
+    A(11, o) = (A(14, o) - lat::ONE6 * s + tnx) * om + A(11, o) * wd;
+    A(12, o) = (A(13, o) - lat::ONE6 * s - tnx) * om + A(12, o) * wd;
+    A(15, o) = (A(18, o) - lat::ONE6 * s + tny) * om + A(15, o) * wd;
+    A(16, o) = (A(17, o) - lat::ONE6 * s - tny) * om + A(16, o) * wd;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// macroscopic fields and monitor reductions
+// ------------------------------------------------------------------------------------------------------
+
+// compute_macro_vars (Misc.F90:368-423): rho = sum f, u = sum e f - F/2 at fluid nodes.  Solid nodes hold 0
+// (the fields are cleared when the geometry is uploaded and nothing else writes them).
+__global__ void k_sp_macro(SDev d, double force_z, double* u, double* v, double* w, double* rho) {
+    const int m = blockIdx.x * BLOCK + threadIdx.x;
+    if (m >= d.nf) return;
+    const long cs = d.cs;
+    double f[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = d.fc[q * cs + m];
+    const double fx = 0.0, fy = 0.0, fz = force_z;
+    const int o = d.cpos[m];
+    rho[o] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] +
+             f[15] + f[16] + f[17] + f[18];
+    u[o] = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14] - 0.5 * fx;
+    v[o] = f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18] - 0.5 * fy;
+    w[o] = f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18] - 0.5 * fz;
+}
+
+// monitor (Monitor.F90:20-48), one block per z layer: fl(k) = sum of w, pre(k) = sum of rho over the layer,
+// and the layer's max of u^2+v^2+w^2.  The sums are tree reductions (the reference adds in x-fastest order).
+__global__ void __launch_bounds__(BLOCK) k_sp_monitor_layers(SDev d, const double* u, const double* v, const double* w,
+                                                             const double* rho, double* out) {
+    const Layout L = d.L;
+    const int k = blockIdx.x + 1;
+    __shared__ double s_fl[BLOCK], s_pre[BLOCK], s_um[BLOCK];
+    double fl = 0.0, pre = 0.0, um = 0.0;
+    for (int t = threadIdx.x; t < L.nx * L.ny; t += BLOCK) {
+        const int o = L.at(t % L.nx + 1, t / L.nx + 1, k);
+        fl += w[o];
+        pre += rho[o];
+        const double q = u[o] * u[o] + v[o] * v[o] + w[o] * w[o];
+        if (um < q) um = q;
+    }
+    s_fl[threadIdx.x] = fl;
+    s_pre[threadIdx.x] = pre;
+    s_um[threadIdx.x] = um;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s_fl[threadIdx.x] += s_fl[threadIdx.x + h];
+            s_pre[threadIdx.x] += s_pre[threadIdx.x + h];
+            if (s_um[threadIdx.x] < s_um[threadIdx.x + h]) s_um[threadIdx.x] = s_um[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[k - 1] = s_fl[0];
+        out[L.nz + k - 1] = s_pre[0];
+        out[2 * L.nz + k - 1] = s_um[0];
+    }
+}
+
+bool sp_open_z(const mflbm_sp_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
+
+}  // namespace
+
+struct mflbm_sp_ctx {
+    mflbm_sp_params p{};
+    SPCollideParams cp{};
+    Layout L{};
+    SDev d{};
+    hipStream_t stream = nullptr;
+    long ps = 0;                 // stride of the dense arrays
+    double* dense_pdf = nullptr; // 19 dense arrays: upload/download staging
+    double* fields = nullptr;    // u, v, w, rho
+    int8_t* walls = nullptr;
+    double* planes = nullptr;    // w_in, cf (19)
+    double* red = nullptr;       // monitor layer results (3*nz)
+    double* compact = nullptr;
+    int* map = nullptr;
+    int* cpos = nullptr;
+    bool have_geom = false, have_state = false;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;  // 3 events per timed step: start, after the lattice update, end
+    size_t ev_used = 0;
+    double t_sum[3] = {0, 0, 0};
+    long t_cnt[3] = {0, 0, 0};
+    std::string err;
+};
+
+namespace {
+
+int sp_launch_step(mflbm_sp_ctx* c, int ntime) {
+    const mflbm_sp_params& p = c->p;
+    const Layout& L = c->L;
+    const bool odd = (ntime % 2) != 0;
+    hipEvent_t* ev = nullptr;
+    if (c->timing) {
+        if (c->ev_used + 3 > c->ev.size()) {
+            for (int m = 0; m < 3; ++m) {
+                hipEvent_t e;
+                HIPCHK(c, hipEventCreate(&e));
+                c->ev.push_back(e);
+            }
+        }
+        ev = &c->ev[c->ev_used];
+        c->ev_used += 3;
+        HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    }
+    const int per = xcd_per(c->d.nf);
+    if (odd) LAUNCH(c, k_sp_collide<true>, NXCD * per, c->d, c->cp, per);
+    else LAUNCH(c, k_sp_collide<false>, NXCD * per, c->d, c->cp, per);
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    if (p.kper == 1 || p.jper == 1) {
+        if (!odd) {
+            if (p.kper == 1) LAUNCH(c, k_sp_pull_z, nblocks((long)L.nx * L.ny), c->d);
+            if (p.jper == 1) LAUNCH(c, k_sp_pull_y, nblocks((long)L.nx * L.nz), c->d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_pull_edge, nblocks(L.nx), c->d);
+        } else {
+            if (p.kper == 1) LAUNCH(c, k_sp_push_z, nblocks((long)L.nx * L.ny), c->d);
+            if (p.jper == 1) LAUNCH(c, k_sp_push_y, nblocks((long)L.nx * L.nz), c->d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_push_edge, nblocks(L.nx), c->d);
+        }
+    }
+    if (sp_open_z(p)) {
+        const unsigned gface = nblocks((long)L.nx * L.ny);
+        if (!odd) {
+            if (p.inlet_bc == 1) LAUNCH(c, k_sp_inlet_velocity_before, gface, c->d, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_sp_inlet_pressure_before, gface, c->d, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_sp_outlet_convective_before, gface, c->d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_sp_outlet_pressure_before, gface, c->d, p.rho_out);
+        } else {
+            if (p.inlet_bc == 1) LAUNCH(c, k_sp_inlet_velocity_after, gface, c->d, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_sp_inlet_pressure_after, gface, c->d, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_sp_outlet_convective_after, gface, c->d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_sp_outlet_pressure_after, gface, c->d, p.rho_out);
+        }
+    }
+    if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    return MFLBM_OK;
+}
+
+void sp_free_geometry(mflbm_sp_ctx* c) {
+    void* bufs[] = {c->compact, c->map, c->cpos};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    c->compact = nullptr;
+    c->map = nullptr;
+    c->cpos = nullptr;
+    c->have_geom = false;
+    c->have_state = false;
+}
+
+int sp_fold_events(mflbm_sp_ctx* c) {
+    if (c->ev_used == 0) return MFLBM_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t b = 0; b + 3 <= c->ev_used; b += 3) {
+        float a = 0, s = 0;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[b], c->ev[b + 1]));
+        HIPCHK(c, hipEventElapsedTime(&s, c->ev[b], c->ev[b + 2]));
+        c->t_sum[0] += a;
+        c->t_sum[2] += s;
+        c->t_cnt[0]++;
+        c->t_cnt[2]++;
+    }
+    c->ev_used = 0;
+    return MFLBM_OK;
+}
+
+}  // namespace
+
+extern "C" const char* mflbm_sp_last_error(const mflbm_sp_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+extern "C" int mflbm_sp_create(const mflbm_sp_params* prm, mflbm_sp_ctx** out) {
+    if (!out) return fail(nullptr, MFLBM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!prm) return fail(nullptr, MFLBM_ERR_ARG, "params is NULL");
+    if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return fail(nullptr, MFLBM_ERR_ARG, "lattice must be at least 3^3");
+    if (prm->wall_x_min != 1 || prm->wall_x_max != 1)
+        return fail(nullptr, MFLBM_ERR_ARG, "x needs solid walls at both ends (the reference rejects anything else)");
+    if (!(prm->la_nu > 0.0)) return fail(nullptr, MFLBM_ERR_ARG, "viscosity must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MFLBM_ERR_NODEV, "no HIP device visible");
+    mflbm_sp_ctx* c = new (std::nothrow) mflbm_sp_ctx();
+    if (!c) return fail(nullptr, MFLBM_ERR_NOMEM, "context allocation failed");
+    c->p = *prm;
+    c->cp = sp_collide_params(prm->la_nu, prm->mrt_preset, prm->force_z);
+    Layout& L = c->L;
+    L = make_layout(prm->nx, prm->ny, prm->nz);
+    if (L.total > (1L << 31) - 1) {
+        delete c;
+        return fail(nullptr, MFLBM_ERR_ARG, "lattice too large for 32-bit node offsets");
+    }
+    const long ps = ((L.total + 31) / 32) * 32;
+    c->ps = ps;
+    auto bail = [&](int code) {
+        std::string m = c->err;
+        mflbm_sp_destroy(c);
+        return fail(nullptr, code, "%s", m.c_str());
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(c, MFLBM_ERR_HIP, "hipStreamCreate failed"));
+    const long plane = (long)(L.nx + 2) * (L.ny + 2);
+    if (hipMalloc(&c->dense_pdf, sizeof(double) * ps * 19) != hipSuccess ||
+        hipMalloc(&c->fields, sizeof(double) * ps * 4) != hipSuccess || hipMalloc(&c->walls, (size_t)ps) != hipSuccess ||
+        hipMalloc(&c->planes, sizeof(double) * plane * 20) != hipSuccess ||
+        hipMalloc(&c->red, sizeof(double) * (size_t)L.nz * 3) != hipSuccess)
+        return bail(fail(c, MFLBM_ERR_NOMEM, "device allocation of %.2f GB failed", 23.0 * 8.0 * ps / 1e9));
+    if (hipMemset(c->dense_pdf, 0, sizeof(double) * ps * 19) != hipSuccess ||
+        hipMemset(c->fields, 0, sizeof(double) * ps * 4) != hipSuccess || hipMemset(c->walls, 1, (size_t)ps) != hipSuccess ||
+        hipMemset(c->planes, 0, sizeof(double) * plane * 20) != hipSuccess)
+        return bail(fail(c, MFLBM_ERR_HIP, "hipMemset failed"));
+    SDev& d = c->d;
+    d.L = L;
+    d.walls = c->walls;
+    d.w_in = c->planes;
+    d.cf = c->planes + plane;
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(c, MFLBM_ERR_HIP, "hipDeviceSynchronize failed"));
+    *out = c;
+    return MFLBM_OK;
+}
+
+extern "C" void mflbm_sp_destroy(mflbm_sp_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    sp_free_geometry(c);
+    void* bufs[] = {c->dense_pdf, c->fields, c->walls, c->planes, c->red};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int mflbm_sp_set_scalars(mflbm_sp_ctx* c, double force_z, double rho_in, double uin_avg, double relaxation) {
+    if (!c) return MFLBM_ERR_ARG;
+    c->p.force_z = force_z;
+    c->p.rho_in = rho_in;
+    c->p.uin_avg = uin_avg;
+    c->p.relaxation = relaxation;
+    c->cp.force_z = force_z;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_upload_geometry(mflbm_sp_ctx* c, const int8_t* walls_h2, const double* w_in) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!walls_h2) return fail(c, MFLBM_ERR_ARG, "walls is NULL");
+    const Layout& L = c->L;
+    const mflbm_sp_params& p = c->p;
+    if (sp_open_z(p) && p.inlet_bc == 1 && !w_in) return fail(c, MFLBM_ERR_ARG, "velocity inlet needs w_in");
+    std::vector<int8_t> wp(L.total, 1);
+    pack_host(L, 2, walls_h2, wp.data());
+    CompactSet cset;
+    build_compact(L, wp, sp_open_z(p), p.kper == 1, p.jper == 1, {}, cset);
+    const int nc = (int)cset.cpos.size();
+    sp_free_geometry(c);
+    const long cs = (((long)nc + 63) / 64) * 64;
+    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * cs * 19));
+    HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
+    HIPCHK(c, hipMalloc(&c->cpos, sizeof(int) * ((size_t)nc + 1)));
+    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * cs * 19));
+    HIPCHK(c, hipMemset(c->fields, 0, sizeof(double) * c->ps * 4));
+    HIPCHK(c, hipMemcpy(c->map, cset.cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
+    if (nc > 0) HIPCHK(c, hipMemcpy(c->cpos, cset.cpos.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->walls, wp.data(), (size_t)L.total, hipMemcpyHostToDevice));
+    if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
+    SDev& d = c->d;
+    d.fc = c->compact;
+    d.cs = cs;
+    d.map = c->map;
+    d.cpos = c->cpos;
+    d.nf = cset.nfl;
+    d.nc = nc;
+    HIPCHK(c, hipDeviceSynchronize());
+    c->have_geom = true;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_upload_state(mflbm_sp_ctx* c, const double* const* f, const double* f_convec) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom) return fail(c, MFLBM_ERR_STATE, "upload the geometry before the state");
+    if (!f) return fail(c, MFLBM_ERR_ARG, "missing population arrays");
+    for (int q = 0; q < 19; ++q)
+        if (!f[q]) return fail(c, MFLBM_ERR_ARG, "population pointer %d is NULL", q);
+    const Layout& L = c->L;
+    std::vector<double> buf(L.total, 0.0);
+    for (int q = 0; q < 19; ++q) {
+        pack_host(L, 1, f[q], buf.data());
+        HIPCHK(c, hipMemcpy(c->dense_pdf + q * c->ps, buf.data(), sizeof(double) * L.total, hipMemcpyHostToDevice));
+    }
+    const size_t plane = (size_t)(L.nx + 2) * (L.ny + 2);
+    if (f_convec) HIPCHK(c, hipMemcpy(c->d.cf, f_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
+    HIPCHK(c, hipDeviceSynchronize());
+    LAUNCH(c, k_sp_gather, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_download_state(mflbm_sp_ctx* c, double* const* f, double* f_convec) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    const Layout& L = c->L;
+    LAUNCH(c, k_sp_scatter, nblocks(c->d.nc), c->d, c->dense_pdf, c->ps);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> buf(L.total);
+    for (int q = 0; q < 19 && f; ++q) {
+        if (!f[q]) continue;
+        HIPCHK(c, hipMemcpy(buf.data(), c->dense_pdf + q * c->ps, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, 1, buf.data(), f[q]);
+    }
+    const size_t plane = (size_t)(L.nx + 2) * (L.ny + 2);
+    if (f_convec) HIPCHK(c, hipMemcpy(f_convec, c->d.cf, sizeof(double) * plane * 19, hipMemcpyDeviceToHost));
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_step(mflbm_sp_ctx* c, int32_t ntime) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom || !c->have_state) return fail(c, MFLBM_ERR_STATE, "upload geometry and state first");
+    return sp_launch_step(c, ntime);
+}
+
+extern "C" int mflbm_sp_run(mflbm_sp_ctx* c, int32_t ntime0, int32_t nsteps) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom || !c->have_state) return fail(c, MFLBM_ERR_STATE, "upload geometry and state first");
+    if (nsteps < 0) return fail(c, MFLBM_ERR_ARG, "nsteps < 0");
+    for (int s = 0; s < nsteps; ++s) {
+        const int rc = sp_launch_step(c, ntime0 + s);
+        if (rc) return rc;
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_compute_macro(mflbm_sp_ctx* c) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    const long ps = c->ps;
+    LAUNCH(c, k_sp_macro, nblocks(c->d.nf), c->d, c->p.force_z, c->fields, c->fields + ps, c->fields + 2 * ps,
+           c->fields + 3 * ps);
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_monitor(mflbm_sp_ctx* c, double* layers, double* umax2) {
+    if (!c) return MFLBM_ERR_ARG;
+    const int rc = mflbm_sp_compute_macro(c);
+    if (rc) return rc;
+    const Layout& L = c->L;
+    const long ps = c->ps;
+    LAUNCH(c, k_sp_monitor_layers, (unsigned)L.nz, c->d, c->fields, c->fields + ps, c->fields + 2 * ps, c->fields + 3 * ps,
+           c->red);
+    std::vector<double> h((size_t)L.nz * 3);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (layers) std::memcpy(layers, h.data(), sizeof(double) * 2 * L.nz);
+    if (umax2) {
+        double um = 0.0;
+        for (int k = 0; k < L.nz; ++k)
+            if (um < h[2 * (size_t)L.nz + k]) um = h[2 * (size_t)L.nz + k];
+        *umax2 = um;
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_download_macro(mflbm_sp_ctx* c, double* u, double* v, double* w, double* rho) {
+    if (!c) return MFLBM_ERR_ARG;
+    const Layout& L = c->L;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> buf(L.total);
+    double* dst[4] = {u, v, w, rho};
+    for (int m = 0; m < 4; ++m) {
+        if (!dst[m]) continue;
+        HIPCHK(c, hipMemcpy(buf.data(), c->fields + m * c->ps, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, 1, buf.data(), dst[m]);
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_synchronize(mflbm_sp_ctx* c) {
+    if (!c) return MFLBM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_timing_enable(mflbm_sp_ctx* c, int32_t enable) {
+    if (!c) return MFLBM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing = enable != 0;
+    c->ev_used = 0;
+    for (int m = 0; m < 3; ++m) {
+        c->t_sum[m] = 0;
+        c->t_cnt[m] = 0;
+    }
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_sp_timing(mflbm_sp_ctx* c, int32_t which, double* mean_ms, int64_t* count) {
+    if (!c || (which != 0 && which != 2)) return MFLBM_ERR_ARG;
+    const int rc = sp_fold_events(c);
+    if (rc) return rc;
+    if (mean_ms) *mean_ms = c->t_cnt[which] ? c->t_sum[which] / c->t_cnt[which] : 0.0;
+    if (count) *count = c->t_cnt[which];
+    return MFLBM_OK;
+}

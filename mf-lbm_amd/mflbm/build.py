@@ -1,6 +1,7 @@
 """Build the native libraries in-tree.
 
-* ``mflbm/lib/libmflbm.so``: the gfx950 device code and the C ABI (hipcc, FP contraction off so the FP64
+* ``mflbm/lib/libmflbm.so``: the gfx950 device code of the two-phase and single-phase paths and the C ABI
+  (hipcc, FP contraction off so the FP64
   arithmetic rounds exactly as written; packed-FP32 instructions disabled).
 * ``oracle/liboracle.so``: the CPU checker (g++, OpenMP), built only when its sources are present.
 """
@@ -38,7 +39,8 @@ def _run(cmd, cwd=None):
 
 def build_device(verbose: bool = False) -> str:
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    srcs = [os.path.join(CSRC, "mflbm_device.hip"), os.path.join(CSRC, "host_setup.cpp")]
+    srcs = [os.path.join(CSRC, "mflbm_device.hip"), os.path.join(CSRC, "sp_device.hip"),
+            os.path.join(CSRC, "host_setup.cpp")]
     tmp = LIB + ".tmp"
     out = _run([HIPCC, *HIP_FLAGS, f"-I{CSRC}", f"-I{INCLUDE}", *srcs, "-o", tmp])
     os.replace(tmp, LIB)
@@ -51,10 +53,11 @@ def build_oracle(verbose: bool = False) -> str | None:
     src = os.path.join(ORACLE_DIR, "oracle.cpp")
     if not os.path.exists(src):
         return None
+    srcs = [src, os.path.join(ORACLE_DIR, "oracle_sp.cpp")]
     cxx = shutil.which("g++") or "g++"
     tmp = ORACLE_LIB + ".tmp"
     _run([cxx, "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
-          src, "-o", tmp])
+          *srcs, "-o", tmp])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
 

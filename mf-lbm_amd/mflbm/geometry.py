@@ -5,6 +5,8 @@ produce the seeded synthetic geometries the benchmarks are defined on:
 
 * ``tube_sphere``: solid tube around the z axis plus a centred solid sphere, the shape the reference's
   built-in geometry uses (multiphase_3D/0.src/Misc.F90:213-244) with free radii.
+* ``duct_obstacle``: the single-phase program's built-in obstacle, a solid sphere off the duct axis
+  (singlephase_3D/0.src/Misc.F90:212-243).
 * ``sphere_pack``: randomly placed overlapping solid spheres (radius uniform in [rmin, rmax]) added until
   the porosity of the sample reaches the target, with fluid buffer layers at both z ends.
 """
@@ -27,6 +29,17 @@ def tube_sphere(nx: int, ny: int, nz: int, r_tube: float, r_sphere: float, buffe
     w[np.broadcast_to(sphere, w.shape)] = 1
     w[np.broadcast_to(tube, w.shape)] = 1
     return w
+
+
+def duct_obstacle(nx: int, ny: int, nz: int) -> np.ndarray:
+    """Solid sphere of radius 0.15*ny centred at (0.5(nx+1), 0.45(ny+1), 0.3(nz+1)) in 1-based lattice
+    coordinates, where (i-xc)^2+(j-yc)^2+(k-zc)^2 < r^2 (modify_geometry, singlephase_3D/0.src/Misc.F90:212-243)."""
+    xc, yc, zc = 0.5 * (nx + 1), 0.45 * (ny + 1), 0.3 * (nz + 1)
+    r1 = 0.15 * ny
+    k = np.arange(1, nz + 1, dtype=np.float64)[:, None, None]
+    j = np.arange(1, ny + 1, dtype=np.float64)[None, :, None]
+    i = np.arange(1, nx + 1, dtype=np.float64)[None, None, :]
+    return ((i - xc) ** 2 + (j - yc) ** 2 + (k - zc) ** 2 < r1 ** 2).astype(np.int8)
 
 
 class SplitMix64:

@@ -1,4 +1,4 @@
-"""On-disk formats of the MF-LBM two-phase solver: control file, wall file, checkpoint file.
+"""On-disk formats of the MF-LBM two-phase and single-phase solvers: control file, wall file, checkpoint file.
 
 Formats kept byte-compatible with the reference so existing run directories keep working:
 
@@ -318,3 +318,178 @@ def write_checkpoint(path: str, ck: Checkpoint) -> None:
             fh.write(np.ascontiguousarray(ck.f_convec, dtype="<f8").tobytes())
             fh.write(np.ascontiguousarray(ck.g_convec, dtype="<f8").tobytes())
             fh.write(np.ascontiguousarray(ck.phi_convec, dtype="<f8").tobytes())
+
+
+# =================================================================================================
+# single-phase program (singlephase_3D/0.src): control file, derived constants, checkpoint
+# =================================================================================================
+
+_SP_INT_KEYS = {
+    "steady_state_option": 1, "MRT_collision_parameter_preset": 1, "benchmark_cmd": 1,
+    "output_fieldData_precision_cmd": 1, "extreme_large_sim_cmd": 1, "modify_geometry_cmd": 1,
+    "external_geometry_read_cmd": 1, "lattice_dimensions": 3, "excluded_layers": 2,
+    "domain_wall_status_x": 2, "domain_wall_status_y": 2, "domain_wall_status_z": 2,
+    "periodic_indicator": 3, "MPI_async_layers_num": 3, "MPI_process_num": 3, "inlet_BC": 1,
+    "outlet_BC": 1, "max_time_step": 1, "max_time_step_benchmark": 1, "ntime_visual": 1,
+    "monitor_timer": 1, "monitor_profile_timer_ratio": 1, "computation_time_timer": 1,
+    "display_steps_timer": 1,
+}
+_SP_REAL_KEYS = {
+    "convergence_criteria", "char_length", "fluid_viscosity", "body_force_0", "body_force_increment",
+    "Reynolds_number", "rho_drop", "target_inject_pore_volume", "checkpoint_save_timer",
+    "checkpoint_2rd_save_timer", "simulation_duration_timer", "d_vol_detail", "d_vol_monitor",
+}
+
+SP_DEFAULT_CONTROL: Dict[str, object] = {
+    "benchmark_cmd": 0, "extreme_large_sim_cmd": 0, "steady_state_option": 0, "MRT_collision_parameter_preset": 1,
+    "convergence_criteria": 1e-6, "modify_geometry_cmd": 0, "external_geometry_read_cmd": 1,
+    "output_fieldData_precision_cmd": 1, "lattice_dimensions": (16, 16, 32), "excluded_layers": (0, 0),
+    "domain_wall_status_x": (1, 1), "domain_wall_status_y": (1, 1), "domain_wall_status_z": (0, 0),
+    "periodic_indicator": (0, 0, 1), "MPI_process_num": (1, 1, 1), "MPI_async_layers_num": (0, 0, 4),
+    "char_length": 20.0, "Reynolds_number": 10.0, "rho_drop": 0.01, "fluid_viscosity": 0.05, "inlet_BC": 0,
+    "outlet_BC": 0, "target_inject_pore_volume": -1.0, "body_force_0": 1e-6, "body_force_increment": 0.0,
+    "max_time_step": 100, "max_time_step_benchmark": 100, "ntime_visual": 100000000, "monitor_timer": 100000000,
+    "monitor_profile_timer_ratio": 1000, "computation_time_timer": 100000000, "display_steps_timer": 100000000,
+    "checkpoint_save_timer": 1000.0, "checkpoint_2rd_save_timer": 1000.0, "simulation_duration_timer": 1000.0,
+    "d_vol_detail": -1.0, "d_vol_monitor": -1.0,
+}
+
+
+def write_control_sp(path: str, values: Dict[str, object]) -> None:
+    """Write a single-phase control file with every key its parser knows (IO.F90:65-290)."""
+    merged = dict(SP_DEFAULT_CONTROL)
+    merged.update(values)
+    with open(path, "w") as fh:
+        fh.write("# control file written by mflbm.io.write_control_sp\n")
+        for k, v in merged.items():
+            fh.write(f"{k} {_fmt(v)}\n")
+
+
+def read_control_sp(path: str) -> Dict[str, object]:
+    """Parse a single-phase control file (IO.F90:65-290); same line grammar as the two-phase parser."""
+    out: Dict[str, object] = {}
+    with open(path) as fh:
+        for line in fh:
+            if not line.strip():
+                continue
+            parts = line.split(None, 1)
+            key = parts[0]
+            toks = [t for t in (parts[1] if len(parts) > 1 else "").replace(",", " ").split() if t]
+            if key in _SP_INT_KEYS:
+                n = _SP_INT_KEYS[key]
+                vals = [int(t) for t in toks[:n]]
+                out[key] = vals[0] if n == 1 else tuple(vals)
+            elif key in _SP_REAL_KEYS:
+                out[key] = _parse_real(toks[0])
+    return out
+
+
+@dataclasses.dataclass
+class SPRunParams:
+    nx: int
+    ny: int
+    nz: int
+    jper: int
+    kper: int
+    wall_x: tuple
+    wall_y: tuple
+    wall_z: tuple
+    inlet_bc: int
+    outlet_bc: int
+    la_nu: float
+    mrt_preset: int
+    force_z: float
+    rho_in: float
+    rho_out: float
+    uin_avg: float
+    flowrate0: float        # inlet flow rate of a velocity inlet (uin_avg * A_xy)
+    la_x: float
+    la_y: float
+    A_xy: float
+    n_exclude_inlet: int
+    n_exclude_outlet: int
+    steady_state_option: int
+    convergence_criteria: float
+
+    @property
+    def open_z(self) -> bool:
+        return self.kper == 0 and self.wall_z[0] == 0 and self.wall_z[1] == 0
+
+
+def derive_sp_params(ctl: Dict[str, object]) -> SPRunParams:
+    """Per-run constants of the single-phase program (read_parameter, IO.F90:318-484, and
+    initialization_basic / initialization_open_velocity_inlet_BC, Initialization.F90:4-205)."""
+    c = dict(SP_DEFAULT_CONTROL)
+    c.update(ctl)
+    nx, ny, nz = c["lattice_dimensions"]
+    iper, jper, kper = c["periodic_indicator"]
+    wx, wy, wz = tuple(c["domain_wall_status_x"]), tuple(c["domain_wall_status_y"]), tuple(c["domain_wall_status_z"])
+    if iper == 1 or wx[0] == 0 or wx[1] == 0:
+        raise ValueError("x must carry solid walls at both ends and cannot be periodic")
+    if jper == 0 and (wy[0] == 0 or wy[1] == 0):
+        raise ValueError("y without periodicity needs solid walls at both ends")
+    if jper == 1 and (wy[0] == 1 or wy[1] == 1):
+        raise ValueError("y periodic with solid y walls")
+    if kper == 1 and (wz[0] == 1 or wz[1] == 1):
+        raise ValueError("z periodic with solid z walls")
+    inlet_bc, outlet_bc = int(c["inlet_BC"]), int(c["outlet_BC"])
+    if outlet_bc == 1 and inlet_bc == 2:
+        raise ValueError("pressure inlet with convective outlet is not supported")
+    la_nu = float(c["fluid_viscosity"])
+    la_x = _f32(_f32(_f32(nx - 1) - 0.5) - 0.5)
+    la_y = _f32(_f32(_f32(ny - 1) - 0.5) - 0.5)
+    A_xy = la_x * la_y
+    rho_out, rho_in, uin_avg, flowrate0 = 1.0, 0.0, 0.0, 0.0
+    open_z = kper == 0 and wz[0] == 0 and wz[1] == 0
+    if open_z:
+        if inlet_bc == 1:
+            uin_avg = float(c["Reynolds_number"]) * la_nu / float(c["char_length"])
+            flowrate0 = uin_avg * A_xy
+        elif inlet_bc == 2:
+            rho_in = rho_out + float(c["rho_drop"])
+    return SPRunParams(
+        nx=nx, ny=ny, nz=nz, jper=jper, kper=kper, wall_x=wx, wall_y=wy, wall_z=wz, inlet_bc=inlet_bc,
+        outlet_bc=outlet_bc, la_nu=la_nu, mrt_preset=int(c["MRT_collision_parameter_preset"]),
+        force_z=float(c["body_force_0"]), rho_in=rho_in, rho_out=rho_out, uin_avg=uin_avg, flowrate0=flowrate0,
+        la_x=la_x, la_y=la_y, A_xy=A_xy, n_exclude_inlet=int(c["excluded_layers"][0]),
+        n_exclude_outlet=int(c["excluded_layers"][1]), steady_state_option=int(c["steady_state_option"]),
+        convergence_criteria=float(c["convergence_criteria"]))
+
+
+@dataclasses.dataclass
+class SPCheckpoint:
+    ntime_next: int
+    force_z: float
+    rho_in: float
+    f: np.ndarray                           # (19, nz+2, ny+2, nx+2)
+    f_convec: Optional[np.ndarray] = None   # (19, ny+2, nx+2)
+
+
+def read_checkpoint_sp(path: str, nx: int, ny: int, nz: int, convective: bool) -> SPCheckpoint:
+    """Single-phase checkpoint (save_checkpoint, IO.F90:490-547): int32 ntime+1, f64 force_z, f64 rho_in,
+    f0..f18 over (0:nx+1, 0:ny+1, 0:nz+1), then f_convec_bc over (0:nx+1, 0:ny+1, 0:18) when outlet_BC = 1."""
+    npdf = (nx + 2) * (ny + 2) * (nz + 2)
+    nc = (nx + 2) * (ny + 2)
+    with open(path, "rb") as fh:
+        raw = fh.read()
+    ntime_next, force_z, rho_in = struct.unpack("<idd", raw[:20])
+    off = 20
+    f = np.frombuffer(raw, dtype="<f8", offset=off, count=19 * npdf).reshape(19, nz + 2, ny + 2, nx + 2)
+    off += 19 * npdf * 8
+    ck = SPCheckpoint(ntime_next, force_z, rho_in, f.copy())
+    if convective:
+        cf = np.frombuffer(raw, dtype="<f8", offset=off, count=19 * nc).reshape(19, ny + 2, nx + 2)
+        off += 19 * nc * 8
+        ck.f_convec = cf.copy()
+    if off != len(raw):
+        raise ValueError(f"checkpoint size mismatch: parsed {off} of {len(raw)} bytes")
+    return ck
+
+
+def write_checkpoint_sp(path: str, ck: SPCheckpoint) -> None:
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<idd", ck.ntime_next, ck.force_z, ck.rho_in))
+        fh.write(np.ascontiguousarray(ck.f, dtype="<f8").tobytes())
+        if ck.f_convec is not None:
+            fh.write(np.ascontiguousarray(ck.f_convec, dtype="<f8").tobytes())

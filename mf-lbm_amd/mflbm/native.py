@@ -30,6 +30,14 @@ class Params(C.Structure):
         "uin_avg", "relaxation")]
 
 
+class SPParams(C.Structure):
+    """mflbm_sp_params (single-phase solver)"""
+    _fields_ = [(n, c_int) for n in (
+        "nx", "ny", "nz", "jper", "kper", "wall_x_min", "wall_x_max", "wall_y_min", "wall_y_max",
+        "wall_z_min", "wall_z_max", "inlet_bc", "outlet_bc", "n_exclude_inlet", "n_exclude_outlet",
+        "mrt_preset")] + [(n, c_dbl) for n in ("la_nu", "force_z", "rho_in", "rho_out", "uin_avg", "relaxation")]
+
+
 class MonitorSums(C.Structure):
     """mflbm_monitor_sums"""
     _fields_ = [(n, c_dbl) for n in ("umax2", "usq1", "usq2", "vol1_full", "vol2_full", "dphi_max",
@@ -77,6 +85,25 @@ _ENTRY_POINTS = {
     "timing_enable": (c_int, [C.c_void_p, c_int]),
 }
 
+# single-phase context entry points: name -> (restype, argtypes), bound as prefix + "sp_" + name
+_SP_ENTRY_POINTS = {
+    "create": (c_int, [C.POINTER(SPParams), C.POINTER(C.c_void_p)]),
+    "destroy": (None, [C.c_void_p]),
+    "last_error": (C.c_char_p, [C.c_void_p]),
+    "set_scalars": (c_int, [C.c_void_p, c_dbl, c_dbl, c_dbl, c_dbl]),
+    "upload_geometry": (c_int, [C.c_void_p, P_i8, P_dbl]),
+    "upload_state": (c_int, [C.c_void_p, C.POINTER(P_dbl), P_dbl]),
+    "download_state": (c_int, [C.c_void_p, C.POINTER(P_dbl), P_dbl]),
+    "step": (c_int, [C.c_void_p, c_int]),
+    "run": (c_int, [C.c_void_p, c_int, c_int]),
+    "compute_macro": (c_int, [C.c_void_p]),
+    "monitor": (c_int, [C.c_void_p, P_dbl, P_dbl]),
+    "download_macro": (c_int, [C.c_void_p, P_dbl, P_dbl, P_dbl, P_dbl]),
+    "synchronize": (c_int, [C.c_void_p]),
+    "timing": (c_int, [C.c_void_p, c_int, P_dbl, C.POINTER(C.c_int64)]),
+    "timing_enable": (c_int, [C.c_void_p, c_int]),
+}
+
 _HOST_ENTRY_POINTS = {
     "mflbm_create_error": (C.c_char_p, []),
     "mflbm_setup_walls": (c_int, [C.POINTER(Params), P_i8, P_i8]),
@@ -87,16 +114,22 @@ _HOST_ENTRY_POINTS = {
     "mflbm_init_fields": (c_int, [C.POINTER(Params), c_int, c_dbl, P_dbl, P_dbl, P_dbl, C.POINTER(P_dbl),
                                   C.POINTER(P_dbl), P_dbl, P_dbl, P_dbl, P_dbl]),
     "mflbm_inlet_profile": (c_int, [C.POINTER(Params), c_dbl, c_dbl, c_dbl, P_dbl]),
+    "mflbm_sp_init_fields": (c_int, [C.POINTER(SPParams), P_dbl, P_dbl, P_dbl, P_dbl, C.POINTER(P_dbl), P_dbl]),
 }
 
 # every symbol include/mflbm.h declares
-EXPORTED = ["mflbm_" + n for n in _ENTRY_POINTS] + list(_HOST_ENTRY_POINTS)
+EXPORTED = (["mflbm_" + n for n in _ENTRY_POINTS] + ["mflbm_sp_" + n for n in _SP_ENTRY_POINTS] +
+            list(_HOST_ENTRY_POINTS))
 
 
-def bind(lib: C.CDLL, prefix: str, names=None):
-    """Attach restype/argtypes to ``prefix + name`` for the context entry points; returns a dict."""
+def bind(lib: C.CDLL, prefix: str, names=None, single_phase: bool = False):
+    """Attach restype/argtypes to ``prefix + name`` for the context entry points (two-phase, or with
+    ``single_phase`` the ``prefix + "sp_" + name`` ones); returns a dict keyed by ``name``."""
     out = {}
-    for name, (res, args) in _ENTRY_POINTS.items():
+    table = _SP_ENTRY_POINTS if single_phase else _ENTRY_POINTS
+    if single_phase:
+        prefix = prefix + "sp_"
+    for name, (res, args) in table.items():
         if names is not None and name not in names:
             continue
         if not hasattr(lib, prefix + name):

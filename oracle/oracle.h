@@ -8,6 +8,9 @@
  * of the step from the reference source; every function cites the lines it follows.  The entry points
  * mirror the product ABI (include/mflbm.h) so one test harness drives both.
  *
+ * The single-phase program (singlephase_3D/0.src) has its own restatement in oracle_sp.cpp behind the
+ * oracle_sp_* entry points, mirroring the mflbm_sp_* part of the ABI.
+ *
  * Parity pins: the transition from the reference's own initial state is compared against checkpoints of
  * the unmodified reference (fixtures in tests/golden, sha256 of the exact bytes), on periodic, velocity/
  * convective, Zou-He pressure and y/z-periodic cases.
@@ -52,6 +55,31 @@ int oracle_compute_macro(oracle_ctx* c);
 int oracle_monitor(oracle_ctx* c, double* layers, oracle_monitor_sums* sums, int32_t update_phi_old);
 int oracle_download_macro(oracle_ctx* c, double* u, double* v, double* w, double* rho, double* phi);
 int oracle_download_gradient(oracle_ctx* c, double* cn_x, double* cn_y, double* cn_z, double* c_norm, double* curv);
+
+/* single-phase solver (singlephase_3D/0.src), oracle_sp.cpp; same layout and meaning as mflbm_sp_params */
+typedef struct oracle_sp_params {
+    int32_t nx, ny, nz;
+    int32_t jper, kper;
+    int32_t wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max;
+    int32_t inlet_bc, outlet_bc;
+    int32_t n_exclude_inlet, n_exclude_outlet;
+    int32_t mrt_preset;
+    double la_nu, force_z, rho_in, rho_out, uin_avg, relaxation;
+} oracle_sp_params;
+
+typedef struct oracle_sp_ctx oracle_sp_ctx;
+
+int oracle_sp_create(const oracle_sp_params* p, oracle_sp_ctx** out);
+void oracle_sp_destroy(oracle_sp_ctx* c);
+int oracle_sp_set_scalars(oracle_sp_ctx* c, double force_z, double rho_in, double uin_avg, double relaxation);
+int oracle_sp_upload_geometry(oracle_sp_ctx* c, const int8_t* walls_h2, const double* w_in);
+int oracle_sp_upload_state(oracle_sp_ctx* c, const double* const* f, const double* f_convec);
+int oracle_sp_download_state(oracle_sp_ctx* c, double* const* f, double* f_convec);
+int oracle_sp_step(oracle_sp_ctx* c, int32_t ntime);
+int oracle_sp_run(oracle_sp_ctx* c, int32_t ntime0, int32_t nsteps);
+int oracle_sp_compute_macro(oracle_sp_ctx* c);
+int oracle_sp_monitor(oracle_sp_ctx* c, double* layers, double* umax2);
+int oracle_sp_download_macro(oracle_sp_ctx* c, double* u, double* v, double* w, double* rho);
 
 #ifdef __cplusplus
 }

@@ -8,10 +8,12 @@ import os
 import numpy as np
 
 from mflbm import Case
+from mflbm.singlephase import SPCase
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SMALL_CASES = ["periodic_tube", "open_velocity", "open_pressure", "porous_plate", "periodic_yz_drop"]
 C1_CASE = "c1_tube_sphere_48x48x128"
+SP_CASES = ["sp_periodic_obstacle", "sp_open_velocity", "sp_open_pressure", "sp_periodic_yz"]
 
 
 def sha(a: np.ndarray) -> str:
@@ -68,3 +70,15 @@ def assert_matches_reference(z, case: Case, st, n: int) -> None:
 def max_rel_diff(a: np.ndarray, b: np.ndarray) -> float:
     scale = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / scale)
+
+
+def load_sp(name: str):
+    """Single-phase fixture and its case."""
+    z, ctl = load_fixture(name)
+    return z, SPCase(ctl, z["walls"])
+
+
+def assert_sp_matches(z, case: SPCase, st, n: int) -> None:
+    assert sha(st.f) == str(z[f"sha_{n}_f"]), f"step {n}: populations differ from the reference checkpoint"
+    if case.rp.outlet_bc == 1:
+        assert sha(st.f_convec) == str(z[f"sha_{n}_convec"]), f"step {n}: convective history differs"

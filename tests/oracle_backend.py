@@ -5,6 +5,7 @@ import ctypes as C
 import os
 
 from mflbm import native
+from mflbm.singlephase import SPSimulation
 from mflbm.solver import Simulation
 
 REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
@@ -29,6 +30,14 @@ def set_threads(n: int) -> int:
 
 
 class OracleSimulation(Simulation):
+    prefix = "oracle_"
+
+    def _load(self):
+        return load_oracle()
+
+
+class SPOracleSimulation(SPSimulation):
+    """Single-phase oracle (oracle_sp_* entry points) behind the GPU driver class."""
     prefix = "oracle_"
 
     def _load(self):
