@@ -134,6 +134,54 @@ inline void build_compact(const Layout& L, const std::vector<int8_t>& wp, bool o
             }
 }
 
+// Link slots.  In the AA update the slot (x + e_r, opp(r)) of a non-fluid neighbour of fluid node x is read and
+// written only by x: pulled as x's incoming population opp(r) at an odd step and refilled with x's outgoing
+// population r.  Such slots are kept behind the compact slots of slice opp(r), packed per direction in
+// fluid-node order, so the 64 lanes of a wave find their slots of one direction contiguous.  Per wave w of 64
+// consecutive fluid nodes and direction r (entry 18 w + r - 1):
+//   wmask: bit l set when fluid node 64 w + l has a non-fluid neighbour in direction r
+//   wbase: number of such links of direction r among the fluid nodes before the wave
+//   link slot of (64 w + l, r) = nc + wbase + popcount(wmask & (2^l - 1))
+struct LinkSet {
+    std::vector<unsigned long long> wmask;
+    std::vector<int> wbase;
+    long nlink[19] = {0};
+};
+
+inline void build_links(const Layout& L, const CompactSet& cs, LinkSet& ls) {
+    const int nfl = cs.nfl;
+    const int nwaves = (nfl + 63) / 64;
+    ls.wmask.assign((size_t)nwaves * 18 + 1, 0ull);
+    ls.wbase.assign((size_t)nwaves * 18 + 1, 0);
+    for (int r = 0; r < 19; ++r) ls.nlink[r] = 0;
+    for (int m = 0; m < nfl; ++m) {
+        const size_t w = (size_t)(m >> 6);
+        if ((m & 63) == 0)
+            for (int r = 1; r < 19; ++r) ls.wbase[18 * w + r - 1] = (int)ls.nlink[r];
+        for (int r = 1; r < 19; ++r)
+            if (cs.cidx[cs.cpos[m] + L.off(r)] >= nfl) {
+                ls.wmask[18 * w + r - 1] |= 1ull << (m & 63);
+                ls.nlink[r]++;
+            }
+    }
+}
+
+// link slot of fluid node x (compact index) in direction r, or -1 when x + e_r is a fluid node
+__device__ __forceinline__ int link_slot(int nc, const int* wbase, const unsigned long long* wmask, int x, int r) {
+    const int e = 18 * (x >> 6) + r - 1;
+    const unsigned long long wm = wmask[e];
+    const unsigned l = (unsigned)x & 63u;
+    if (!((wm >> l) & 1ull)) return -1;
+    return nc + wbase[e] + __popcll(wm & ((1ull << l) - 1ull));
+}
+
+// Slot access with a scalar slice base and a 32-bit per-lane byte offset (one global load/store with an SGPR
+// base instead of a 64-bit vector address per slot); slices are kept below 2^32 bytes.
+__device__ __forceinline__ double slot_ld(const double* slice, unsigned i) {
+    return *(const double*)((const char*)slice + i * 8u);
+}
+__device__ __forceinline__ void slot_st(double* slice, unsigned i, double v) { *(double*)((char*)slice + i * 8u) = v; }
+
 // message of the last failed create call (contexts keep their own)
 inline thread_local std::string g_create_error;
 

@@ -28,15 +28,27 @@ using namespace mflbm_dev;
 
 namespace {
 
+// base of each population slice: fluid 1 direction q at p[q], fluid 2 at p[19 + q]
+struct SlotTable {
+    double* p[38];
+};
+
 // device view of the state passed to kernels by value
 struct Dev {
     Layout L;
-    double* fc;       // compact populations: fluid 1 at fc + q*cs, fluid 2 at fc + (19+q)*cs
-    long cs;
+    // Population slice q holds the slots of the compact nodes at [0, nc) and, from index nc, the link slots
+    // of direction r = opp(q): for a fluid node x whose neighbour x + e_r is not a fluid node, the slot
+    // (x + e_r, opp(r)) is read and written only by x (the AA update pulls it as the incoming population
+    // opp(r) and pushes the outgoing population r into it), so it is kept in x's order instead of at the
+    // neighbour's compact index.  Link slots of direction r are packed in fluid-node order: a wave of 64
+    // consecutive fluid nodes finds its links of one direction contiguous (index nc + wbase + rank in wave).
+    SlotTable S;
     const int* map;   // padded offset -> compact index, -1 where nothing is stored
     const int* cpos;  // compact index -> padded offset
     int nf;           // interior fluid nodes
     int nc;           // all compact nodes
+    const unsigned long long* wmask;   // per wave and direction: fluid nodes with a link (device_common.h)
+    const int* wbase;                  // per wave and direction: links before the wave
     double* force;    // per fluid node: cn_x, cn_y, cn_z, 0.5*gamma*curv*c_norm (stride fs)
     long fs;
     double* phi;
@@ -52,15 +64,24 @@ struct Dev {
     double* cphi;     // (nx+2)(ny+2)
 };
 
-// population accessor by (direction, padded offset) through the compact map
+__device__ __forceinline__ int link_index(const Dev& d, int x, int r) { return link_slot(d.nc, d.wbase, d.wmask, x, r); }
+
+// population accessor by (direction, padded offset) through the compact map.  Slot q of a support node o
+// faces the node o + e_q; when that node is fluid the slot is its link slot of direction opp(q).
 struct Pop {
-    double* base;
-    long cs;
-    const int* map;
-    __device__ __forceinline__ double& operator()(int q, int o) const { return base[q * cs + map[o]]; }
+    const Dev* d;
+    int c0;   // 0 fluid 1, 19 fluid 2
+    __device__ __forceinline__ double& operator()(int q, int o) const {
+        const int c = d->map[o];
+        if (c >= d->nf) {
+            const int x = d->map[o + d->L.off(q)];
+            if (x >= 0 && x < d->nf) return d->S.p[c0 + q][link_index(*d, x, lat::OPP[q])];
+        }
+        return d->S.p[c0 + q][c];
+    }
 };
-__device__ __forceinline__ Pop fpop(const Dev& d) { return Pop{d.fc, d.cs, d.map}; }
-__device__ __forceinline__ Pop gpop(const Dev& d) { return Pop{d.fc + 19 * d.cs, d.cs, d.map}; }
+__device__ __forceinline__ Pop fpop(const Dev& d) { return Pop{&d, 0}; }
+__device__ __forceinline__ Pop gpop(const Dev& d) { return Pop{&d, 19}; }
 __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
 
@@ -76,14 +97,21 @@ __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i +
 #ifndef MFLBM_COLLIDE_NT
 #define MFLBM_COLLIDE_NT 0
 #endif
-__device__ __forceinline__ double ld_pop(const double* p) {
+
+// Population slots.  The 38 slice base pointers travel in the kernel arguments, so each is a scalar value
+// the compiler cannot relate to the others: every access becomes one global load/store with a scalar base
+// and a 32-bit vector byte offset, instead of a 64-bit vector address per slot kept live across the collision.
+// upload_geometry guarantees that every slice is below 2^32 bytes.
+__device__ __forceinline__ double ld_slot(const double* slice, unsigned m) {
+    const double* p = (const double*)((const char*)slice + m * 8u);
 #if MFLBM_COLLIDE_NT
     return __builtin_nontemporal_load(p);
 #else
     return *p;
 #endif
 }
-__device__ __forceinline__ void st_pop(double* p, double v) {
+__device__ __forceinline__ void st_slot(double* slice, unsigned m, double v) {
+    double* p = (double*)((char*)slice + m * 8u);
 #if MFLBM_COLLIDE_NT
     __builtin_nontemporal_store(v, p);
 #else
@@ -91,48 +119,68 @@ __device__ __forceinline__ void st_pop(double* p, double v) {
 #endif
 }
 
+#ifndef MFLBM_FUSED_CURVATURE
+#define MFLBM_FUSED_CURVATURE 1
+#endif
+__device__ __forceinline__ double curvature_at(const Dev& d, int n);
+
 // ODD = false: kernel_even_color (Kernel_multiphase.F90:371-725): read swapped own slots, write own slots.
 // ODD = true : kernel_odd_color  (Kernel_multiphase.F90:6-362): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
+// The CSF force input is evaluated here from the normal field the colour-gradient chain left (curvature,
+// Phase_gradient.F90:112-201, and 0.5*gamma*curv*c_norm, Kernel_multiphase.F90:116).
 template <bool ODD>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MFLBM_COLLIDE_WAVES, 8)))
 k_collide(Dev d, CollideParams P, int per) {
+    const SlotTable& S = d.S;
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
-    const long cs = d.cs;
-    double* __restrict__ fa = d.fc;
-    double* __restrict__ ga = d.fc + 19 * cs;
+#if MFLBM_FUSED_CURVATURE
+    const int nn = d.cpos[m];
+    const double cnx = d.cnx[nn], cny = d.cny[nn], cnz = d.cnz[nn];
+    const double tf = 0.5 * P.gamma * curvature_at(d, nn) * d.cnorm[nn];
+    // keep the stencil phase and the population phase apart so their registers are not live together
+    __builtin_amdgcn_sched_barrier(0);
+#else
+    const double cnx = d.force[m], cny = d.force[d.fs + m], cnz = d.force[2 * d.fs + m], tf = d.force[3 * d.fs + m];
+#endif
     double a[19], b[19];
-    int nb[19];
-    a[0] = ld_pop(fa + m);
-    b[0] = ld_pop(ga + m);
+    unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
+    a[0] = ld_slot(S.p[0], m);
+    b[0] = ld_slot(S.p[19], m);
     if (ODD) {
         const int n = d.cpos[m];
+        // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
+        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
+        const unsigned lane = (unsigned)m & 63u;
+        const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-        for (int r = 1; r < 19; ++r) nb[r] = d.map[n + d.L.off(r)];
+        for (int r = 1; r < 19; ++r) {
+            const unsigned long long wm = d.wmask[e0 + r];
+            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
+            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[n + d.L.off(r)];
+        }
 #pragma unroll
-        for (int q = 1; q < 19; ++q) {
-            const long s = q * cs + nb[lat::OPP[q]];   // slot q of the upwind node x - e_q
-            a[q] = ld_pop(fa + s);
-            b[q] = ld_pop(ga + s);
+        for (int q = 1; q < 19; ++q) {   // slot q of the upwind node x - e_q = x + e_opp(q)
+            a[q] = ld_slot(S.p[q], nb[lat::OPP[q]]);
+            b[q] = ld_slot(S.p[19 + q], nb[lat::OPP[q]]);
         }
     } else {
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
-            const long s = lat::OPP[q] * cs + m;
-            a[q] = ld_pop(fa + s);
-            b[q] = ld_pop(ga + s);
+            a[q] = ld_slot(S.p[lat::OPP[q]], m);
+            b[q] = ld_slot(S.p[19 + lat::OPP[q]], m);
         }
     }
-    const double cnx = d.force[m], cny = d.force[d.fs + m], cnz = d.force[2 * d.fs + m], tf = d.force[3 * d.fs + m];
     const double phi = collide_node_tf(a, b, cnx, cny, cnz, tf, P);
     d.phi[d.cpos[m]] = phi;
-    st_pop(fa + m, a[0]);
-    st_pop(ga + m, b[0]);
+    st_slot(S.p[0], m, a[0]);
+    st_slot(S.p[19], m, b[0]);
 #pragma unroll
-    for (int q = 1; q < 19; ++q) {
-        const long s = ODD ? lat::OPP[q] * cs + nb[q] : q * cs + m;   // odd: slot opp(q) of x + e_q
-        st_pop(fa + s, a[q]);
-        st_pop(ga + s, b[q]);
+    for (int q = 1; q < 19; ++q) {   // odd: slot opp(q) of x + e_q
+        const int qs = ODD ? lat::OPP[q] : q;
+        const unsigned ms = ODD ? nb[q] : (unsigned)m;
+        st_slot(S.p[qs], ms, a[q]);
+        st_slot(S.p[19 + qs], ms, b[q]);
     }
 }
 
@@ -141,13 +189,33 @@ __global__ void k_gather(Dev d, const double* __restrict__ dense, long ps) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.nc) return;
     const int n = d.cpos[t];
-    for (int q = 0; q < 38; ++q) d.fc[q * d.cs + t] = dense[q * ps + n];
+    for (int q = 0; q < 38; ++q) d.S.p[q][t] = dense[q * ps + n];
 }
 __global__ void k_scatter(Dev d, double* __restrict__ dense, long ps) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.nc) return;
     const int n = d.cpos[t];
-    for (int q = 0; q < 38; ++q) dense[q * ps + n] = d.fc[q * d.cs + t];
+    for (int q = 0; q < 38; ++q) dense[q * ps + n] = d.S.p[q][t];
+}
+// link slots <-> dense staging (after k_gather / after k_scatter: a link slot holds the current value)
+template <bool TO_DENSE>
+__global__ void k_links(Dev d, double* __restrict__ dense, long ps) {
+    const int m = blockIdx.x * BLOCK + threadIdx.x;
+    if (m >= d.nf) return;
+    const int n = d.cpos[m];
+    for (int r = 1; r < 19; ++r) {
+        const int i = link_index(d, m, r);
+        if (i < 0) continue;
+        const long o = n + d.L.off(r);
+        const int q = lat::OPP[r];
+        if (TO_DENSE) {
+            dense[q * ps + o] = d.S.p[q][i];
+            dense[(19 + q) * ps + o] = d.S.p[19 + q][i];
+        } else {
+            d.S.p[q][i] = dense[q * ps + o];
+            d.S.p[19 + q][i] = dense[(19 + q) * ps + o];
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -735,7 +803,7 @@ __global__ void k_macro(Dev d, double force_z, double* u, double* v, double* w, 
     if (m >= d.nf) return;
     const int n = d.cpos[m];
     double ft[19];
-    for (int q = 0; q < 19; ++q) ft[q] = d.fc[q * d.cs + m] + d.fc[(19 + q) * d.cs + m];
+    for (int q = 0; q < 19; ++q) ft[q] = d.S.p[q][m] + d.S.p[19 + q][m];
     const double om = 1.0;
     rho[n] = (ft[0] + ft[1] + ft[2] + ft[3] + ft[4] + ft[5] + ft[6] + ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] +
               ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18]) * om;
@@ -857,13 +925,15 @@ struct mflbm_ctx {
     double* phi_old = nullptr;
     double* red = nullptr;      // monitor layer sums and partials
     // geometry-dependent device buffers
-    double* compact = nullptr;  // 38 compact population arrays
+    double* compact = nullptr;  // 38 population slices (compact slots + link slots)
     double* force = nullptr;    // 4 compact force arrays
     int* map = nullptr;
     int* ibuf = nullptr;        // cpos, normal list, solid lists, fluid list positions
     int* wbuf = nullptr;        // per normal-list node: index into the fluid boundary list or -1
     unsigned* mbuf = nullptr;   // solid list masks
     double* dbuf = nullptr;     // solid weights, fluid list normals / cos / sin
+    unsigned long long* wmask = nullptr;  // per wave and direction: fluid nodes with a link
+    int* wbase = nullptr;       // per wave of fluid nodes: link counts before it
     SolidList sl{}, sl1{};
     FluidList fl{};
     NodeList nrm{};
@@ -888,7 +958,7 @@ int launch_gradient(mflbm_ctx* c) {
     LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
     LAUNCH(c, k_normals, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
     LAUNCH(c, k_normal_extrapolate, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
-    LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
+    if (!MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     return MFLBM_OK;
 }
 
@@ -954,7 +1024,7 @@ int launch_step(mflbm_ctx* c, int ntime) {
 }
 
 void free_geometry(mflbm_ctx* c) {
-    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf};
+    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
@@ -964,6 +1034,8 @@ void free_geometry(mflbm_ctx* c) {
     c->wbuf = nullptr;
     c->mbuf = nullptr;
     c->dbuf = nullptr;
+    c->wmask = nullptr;
+    c->wbase = nullptr;
     c->have_geom = false;
     c->have_state = false;
 }
@@ -1098,6 +1170,10 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     const std::vector<int>& cidx = cset.cidx;
     const std::vector<int>& cpos = cset.cpos;
     const int nfl = cset.nfl;
+    LinkSet links;
+    build_links(L, cset, links);
+    const std::vector<int>& wbase = links.wbase;
+    const long* nlink = links.nlink;
     const int nc = (int)cpos.size();
     // nodes whose normal is computed: walls == 0 in [-1, n+2]^3
     std::vector<int> nrm;
@@ -1138,11 +1214,20 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     }
     // device buffers
     free_geometry(c);
-    const long cs = (((long)nc + 63) / 64) * 64;
+    // slice q: nc compact slots, then the link slots of direction opp(q); each slice 256-byte aligned
+    long soff[39];
+    soff[0] = 0;
+    for (int q = 0; q < 38; ++q) {
+        const int qq = q % 19;
+        const long len = (((long)nc + (qq ? nlink[lat::OPP[qq]] : 0) + 31) / 32) * 32;
+        if (len * 8 >= (1L << 32))
+            return fail(c, MFLBM_ERR_ARG, "population slice of %ld slots exceeds 32-bit byte offsets", len);
+        soff[q + 1] = soff[q] + len;
+    }
     const long fs = (((long)nfl + 63) / 64) * 64;
     const size_t ni = (size_t)nc + nrm.size() + ns + s1pos.size() + nf + 1;
     const size_t nd = (size_t)ns + s1w.size() + fd.size() + 1;
-    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * cs * 38));
+    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * soff[38]));
     HIPCHK(c, hipMalloc(&c->force, sizeof(double) * fs * 4));
     HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
     HIPCHK(c, hipMalloc(&c->ibuf, sizeof(int) * ni));
@@ -1150,7 +1235,11 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     if (!wb.empty()) HIPCHK(c, hipMemcpy(c->wbuf, wb.data(), sizeof(int) * wb.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->mbuf, sizeof(unsigned) * ((size_t)ns + s1mask.size() + 1)));
     HIPCHK(c, hipMalloc(&c->dbuf, sizeof(double) * nd));
-    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * cs * 38));
+    HIPCHK(c, hipMalloc(&c->wmask, sizeof(unsigned long long) * links.wmask.size()));
+    HIPCHK(c, hipMalloc(&c->wbase, sizeof(int) * wbase.size()));
+    HIPCHK(c, hipMemcpy(c->wmask, links.wmask.data(), sizeof(unsigned long long) * links.wmask.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->wbase, wbase.data(), sizeof(int) * wbase.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * soff[38]));
     HIPCHK(c, hipMemset(c->force, 0, sizeof(double) * fs * 4));
     // normals at solid nodes are only ever written by the extrapolation; start them from zero so no value
     // of a previous geometry survives (cn_x, cn_y, cn_z, c_norm, curv, u, v, w, rho)
@@ -1177,14 +1266,15 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMemcpy(c->walls, wp.data(), (size_t)L.total, hipMemcpyHostToDevice));
     if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
     Dev& d = c->d;
-    d.fc = c->compact;
-    d.cs = cs;
+    for (int q = 0; q < 38; ++q) d.S.p[q] = c->compact + soff[q];
     d.map = c->map;
     d.cpos = c->ibuf;
     d.nf = nfl;
     d.nc = nc;
     d.force = c->force;
     d.fs = fs;
+    d.wmask = c->wmask;
+    d.wbase = c->wbase;
     const int* ip = c->ibuf + nc;
     c->nrm = NodeList{(int)nrm.size(), ip};
     ip += nrm.size();
@@ -1221,6 +1311,7 @@ extern "C" int mflbm_upload_state(mflbm_ctx* c, const double* const* f, const do
     if (phi_convec) HIPCHK(c, hipMemcpy(c->d.cphi, phi_convec, sizeof(double) * plane, hipMemcpyHostToDevice));
     HIPCHK(c, hipDeviceSynchronize());
     LAUNCH(c, k_gather, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
+    LAUNCH(c, k_links<false>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     c->phi_old_valid = false;
@@ -1233,6 +1324,7 @@ extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* cons
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     const Layout& L = c->L;
     LAUNCH(c, k_scatter, nblocks(c->d.nc), c->d, c->dense_pdf, c->ps);
+    LAUNCH(c, k_links<true>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
     for (int q = 0; q < 38; ++q) {
@@ -1278,6 +1370,8 @@ extern "C" int mflbm_run(mflbm_ctx* c, int32_t ntime0, int32_t nsteps) {
 extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    // the force input of the macroscopic velocity comes from the current normal field
+    if (MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     LAUNCH(c, k_macro, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
     LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
     return MFLBM_OK;

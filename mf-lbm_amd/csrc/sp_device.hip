@@ -25,24 +25,33 @@ namespace {
 
 struct SDev {
     Layout L;
-    double* fc;       // compact populations, direction q at fc + q*cs
-    long cs;
+    // slice q: compact slots [0, nc), then the link slots of direction opp(q) (device_common.h)
+    double* S[19];
     const int* map;   // padded offset -> compact index
     const int* cpos;  // compact index -> padded offset
     int nf;           // interior fluid nodes
     int nc;           // all compact nodes
+    const unsigned long long* wmask;   // per wave and direction: fluid nodes with a link (device_common.h)
+    const int* wbase;                  // per wave and direction: links before the wave
     const int8_t* walls;
     double* w_in;     // (nx+2)(ny+2), reference layout
     double* cf;       // convective-outlet history: 19 planes (nx+2)(ny+2)
 };
 
+// population accessor by (direction, padded offset); a support node's slot q facing a fluid node is that
+// node's link slot of direction opp(q)
 struct SPop {
-    double* base;
-    long cs;
-    const int* map;
-    __device__ __forceinline__ double& operator()(int q, int o) const { return base[q * cs + map[o]]; }
+    const SDev* d;
+    __device__ __forceinline__ double& operator()(int q, int o) const {
+        const int c = d->map[o];
+        if (c >= d->nf) {
+            const int x = d->map[o + d->L.off(q)];
+            if (x >= 0 && x < d->nf) return d->S[q][link_slot(d->nc, d->wbase, d->wmask, x, lat::OPP[q])];
+        }
+        return d->S[q][c];
+    }
 };
-__device__ __forceinline__ SPop spop(const SDev& d) { return SPop{d.fc, d.cs, d.map}; }
+__device__ __forceinline__ SPop spop(const SDev& d) { return SPop{&d}; }
 __device__ __forceinline__ int plane_at(const SDev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
 // ODD = false: kernel_even (Kernel.F90:206-400): read the swapped own slots, write the own slots.
@@ -51,27 +60,33 @@ template <bool ODD>
 __global__ void __launch_bounds__(BLOCK) k_sp_collide(SDev d, SPCollideParams P, int per) {
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
-    const long cs = d.cs;
-    double* __restrict__ fa = d.fc;
     double a[19];
-    int nb[19];
-    a[0] = fa[m];
+    unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
+    a[0] = slot_ld(d.S[0], m);
     if (ODD) {
         const int n = d.cpos[m];
+        // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
+        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
+        const unsigned lane = (unsigned)m & 63u;
+        const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-        for (int r = 1; r < 19; ++r) nb[r] = d.map[n + d.L.off(r)];
+        for (int r = 1; r < 19; ++r) {
+            const unsigned long long wm = d.wmask[e0 + r];
+            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
+            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[n + d.L.off(r)];
+        }
 #pragma unroll
-        for (int q = 1; q < 19; ++q) a[q] = fa[q * cs + nb[lat::OPP[q]]];   // slot q of the upwind node x - e_q
+        for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[q], nb[lat::OPP[q]]);   // slot q of the upwind node x - e_q
     } else {
 #pragma unroll
-        for (int q = 1; q < 19; ++q) a[q] = fa[lat::OPP[q] * cs + m];
+        for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[lat::OPP[q]], m);
     }
     collide_single(a, P);
-    fa[m] = a[0];
+    slot_st(d.S[0], m, a[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
-        const long s = ODD ? lat::OPP[q] * cs + nb[q] : q * cs + m;   // odd: slot opp(q) of x + e_q
-        fa[s] = a[q];
+        if (ODD) slot_st(d.S[lat::OPP[q]], nb[q], a[q]);   // slot opp(q) of x + e_q
+        else slot_st(d.S[q], m, a[q]);
     }
 }
 
@@ -80,13 +95,28 @@ __global__ void k_sp_gather(SDev d, const double* __restrict__ dense, long ps) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.nc) return;
     const int n = d.cpos[t];
-    for (int q = 0; q < 19; ++q) d.fc[q * d.cs + t] = dense[q * ps + n];
+    for (int q = 0; q < 19; ++q) d.S[q][t] = dense[q * ps + n];
 }
 __global__ void k_sp_scatter(SDev d, double* __restrict__ dense, long ps) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.nc) return;
     const int n = d.cpos[t];
-    for (int q = 0; q < 19; ++q) dense[q * ps + n] = d.fc[q * d.cs + t];
+    for (int q = 0; q < 19; ++q) dense[q * ps + n] = d.S[q][t];
+}
+// link slots <-> dense staging (after gather / after scatter)
+template <bool TO_DENSE>
+__global__ void k_sp_links(SDev d, double* __restrict__ dense, long ps) {
+    const int m = blockIdx.x * BLOCK + threadIdx.x;
+    if (m >= d.nf) return;
+    const int n = d.cpos[m];
+    for (int r = 1; r < 19; ++r) {
+        const int i = link_slot(d.nc, d.wbase, d.wmask, m, r);
+        if (i < 0) continue;
+        const long o = n + d.L.off(r);
+        const int q = lat::OPP[r];
+        if (TO_DENSE) dense[q * ps + o] = d.S[q][i];
+        else d.S[q][i] = dense[q * ps + o];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -346,10 +376,9 @@ __global__ void k_sp_outlet_pressure_after(SDev d, double rho_out) {
 __global__ void k_sp_macro(SDev d, double force_z, double* u, double* v, double* w, double* rho) {
     const int m = blockIdx.x * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
-    const long cs = d.cs;
     double f[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = d.fc[q * cs + m];
+    for (int q = 0; q < 19; ++q) f[q] = d.S[q][m];
     const double fx = 0.0, fy = 0.0, fz = force_z;
     const int o = d.cpos[m];
     rho[o] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] +
@@ -409,9 +438,11 @@ struct mflbm_sp_ctx {
     int8_t* walls = nullptr;
     double* planes = nullptr;    // w_in, cf (19)
     double* red = nullptr;       // monitor layer results (3*nz)
-    double* compact = nullptr;
+    double* compact = nullptr;  // 19 population slices (compact slots + link slots)
     int* map = nullptr;
     int* cpos = nullptr;
+    unsigned long long* wmask = nullptr;
+    int* wbase = nullptr;
     bool have_geom = false, have_state = false;
     bool timing = false;
     std::vector<hipEvent_t> ev;  // 3 events per timed step: start, after the lattice update, end
@@ -474,12 +505,14 @@ int sp_launch_step(mflbm_sp_ctx* c, int ntime) {
 }
 
 void sp_free_geometry(mflbm_sp_ctx* c) {
-    void* bufs[] = {c->compact, c->map, c->cpos};
+    void* bufs[] = {c->compact, c->map, c->cpos, c->wmask, c->wbase};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
     c->map = nullptr;
     c->cpos = nullptr;
+    c->wmask = nullptr;
+    c->wbase = nullptr;
     c->have_geom = false;
     c->have_state = false;
 }
@@ -587,24 +620,38 @@ extern "C" int mflbm_sp_upload_geometry(mflbm_sp_ctx* c, const int8_t* walls_h2,
     CompactSet cset;
     build_compact(L, wp, sp_open_z(p), p.kper == 1, p.jper == 1, {}, cset);
     const int nc = (int)cset.cpos.size();
+    LinkSet links;
+    build_links(L, cset, links);
+    long soff[20];
+    soff[0] = 0;
+    for (int q = 0; q < 19; ++q) {
+        const long len = (((long)nc + (q ? links.nlink[lat::OPP[q]] : 0) + 31) / 32) * 32;
+        if (len * 8 >= (1L << 32))
+            return fail(c, MFLBM_ERR_ARG, "population slice of %ld slots exceeds 32-bit byte offsets", len);
+        soff[q + 1] = soff[q] + len;
+    }
     sp_free_geometry(c);
-    const long cs = (((long)nc + 63) / 64) * 64;
-    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * cs * 19));
+    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * soff[19]));
     HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
     HIPCHK(c, hipMalloc(&c->cpos, sizeof(int) * ((size_t)nc + 1)));
-    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * cs * 19));
+    HIPCHK(c, hipMalloc(&c->wmask, sizeof(unsigned long long) * links.wmask.size()));
+    HIPCHK(c, hipMalloc(&c->wbase, sizeof(int) * links.wbase.size()));
+    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * soff[19]));
     HIPCHK(c, hipMemset(c->fields, 0, sizeof(double) * c->ps * 4));
     HIPCHK(c, hipMemcpy(c->map, cset.cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
     if (nc > 0) HIPCHK(c, hipMemcpy(c->cpos, cset.cpos.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->wmask, links.wmask.data(), sizeof(unsigned long long) * links.wmask.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->wbase, links.wbase.data(), sizeof(int) * links.wbase.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->walls, wp.data(), (size_t)L.total, hipMemcpyHostToDevice));
     if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
     SDev& d = c->d;
-    d.fc = c->compact;
-    d.cs = cs;
+    for (int q = 0; q < 19; ++q) d.S[q] = c->compact + soff[q];
     d.map = c->map;
     d.cpos = c->cpos;
     d.nf = cset.nfl;
     d.nc = nc;
+    d.wmask = c->wmask;
+    d.wbase = c->wbase;
     HIPCHK(c, hipDeviceSynchronize());
     c->have_geom = true;
     return MFLBM_OK;
@@ -626,6 +673,7 @@ extern "C" int mflbm_sp_upload_state(mflbm_sp_ctx* c, const double* const* f, co
     if (f_convec) HIPCHK(c, hipMemcpy(c->d.cf, f_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
     HIPCHK(c, hipDeviceSynchronize());
     LAUNCH(c, k_sp_gather, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
+    LAUNCH(c, k_sp_links<false>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     return MFLBM_OK;
@@ -636,6 +684,7 @@ extern "C" int mflbm_sp_download_state(mflbm_sp_ctx* c, double* const* f, double
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     const Layout& L = c->L;
     LAUNCH(c, k_sp_scatter, nblocks(c->d.nc), c->d, c->dense_pdf, c->ps);
+    LAUNCH(c, k_sp_links<true>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
     for (int q = 0; q < 19 && f; ++q) {

@@ -17,6 +17,17 @@ module mflbm_c_binding
         real(c_double) :: rho_in, rho_out, uin_avg, relaxation
     end type mflbm_params
 
+    ! single-phase program (singlephase_3D/0.src)
+    type, bind(c) :: mflbm_sp_params
+        integer(c_int32_t) :: nx, ny, nz
+        integer(c_int32_t) :: jper, kper
+        integer(c_int32_t) :: wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max
+        integer(c_int32_t) :: inlet_bc, outlet_bc
+        integer(c_int32_t) :: n_exclude_inlet, n_exclude_outlet
+        integer(c_int32_t) :: mrt_preset
+        real(c_double) :: la_nu, force_z, rho_in, rho_out, uin_avg, relaxation
+    end type mflbm_sp_params
+
     type, bind(c) :: mflbm_monitor_sums
         real(c_double) :: umax2, usq1, usq2, vol1_full, vol2_full, dphi_max, pre_w, pre_nw
         integer(c_int64_t) :: n_w, n_nw, outlet_phase1
@@ -105,6 +116,71 @@ module mflbm_c_binding
             type(mflbm_params), intent(in) :: params
             integer(c_int8_t), intent(in) :: walls_global(*)
             integer(c_int8_t), intent(inout) :: walls_h2(*)
+        end function
+        ! ---- single-phase solver: main_iteration_kernel (singlephase_3D/0.src/Main.F90:291-422) -> mflbm_sp_step
+        integer(c_int) function mflbm_sp_create(params, ctx) bind(c, name="mflbm_sp_create")
+            import :: c_int, c_ptr, mflbm_sp_params
+            type(mflbm_sp_params), intent(in) :: params
+            type(c_ptr), intent(out) :: ctx
+        end function
+        subroutine mflbm_sp_destroy(ctx) bind(c, name="mflbm_sp_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end subroutine
+        integer(c_int) function mflbm_sp_set_scalars(ctx, force_z, rho_in, uin_avg, relaxation) &
+                bind(c, name="mflbm_sp_set_scalars")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: force_z, rho_in, uin_avg, relaxation
+        end function
+        integer(c_int) function mflbm_sp_upload_geometry(ctx, walls_h2, w_in) bind(c, name="mflbm_sp_upload_geometry")
+            import :: c_int, c_ptr, c_int8_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int8_t), intent(in) :: walls_h2(*)
+            real(c_double), intent(in) :: w_in(*)
+        end function
+        ! f: 19 addresses (c_loc(f0) ... c_loc(f18))
+        integer(c_int) function mflbm_sp_upload_state(ctx, f, f_convec) bind(c, name="mflbm_sp_upload_state")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            type(c_ptr), intent(in) :: f(19)
+            real(c_double), intent(in) :: f_convec(*)
+        end function
+        integer(c_int) function mflbm_sp_download_state(ctx, f, f_convec) bind(c, name="mflbm_sp_download_state")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            type(c_ptr), intent(in) :: f(19)
+            real(c_double), intent(inout) :: f_convec(*)
+        end function
+        integer(c_int) function mflbm_sp_step(ctx, ntime) bind(c, name="mflbm_sp_step")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: ntime
+        end function
+        integer(c_int) function mflbm_sp_run(ctx, ntime0, nsteps) bind(c, name="mflbm_sp_run")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: ntime0, nsteps
+        end function
+        ! layers: 2*nz doubles, fl(1:nz) then pre(1:nz) of Monitor.F90:20-48
+        integer(c_int) function mflbm_sp_monitor(ctx, layers, umax2) bind(c, name="mflbm_sp_monitor")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: layers(*)
+            real(c_double), intent(out) :: umax2
+        end function
+        integer(c_int) function mflbm_sp_download_macro(ctx, u, v, w, rho) bind(c, name="mflbm_sp_download_macro")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(inout) :: u(*), v(*), w(*), rho(*)
+        end function
+        ! rho0/u0/v0/w0 may be c_null_ptr (rest state at rho = 1); f: 19 addresses; f_convec may be c_null_ptr
+        integer(c_int) function mflbm_sp_init_fields(params, rho0, u0, v0, w0, f, f_convec) &
+                bind(c, name="mflbm_sp_init_fields")
+            import :: c_int, c_ptr, mflbm_sp_params
+            type(mflbm_sp_params), intent(in) :: params
+            type(c_ptr), value :: rho0, u0, v0, w0, f_convec
+            type(c_ptr), intent(in) :: f(19)
         end function
     end interface
 end module mflbm_c_binding

@@ -2,6 +2,7 @@
 (host set-up entry point only; no GPU needed)."""
 import ctypes as C
 import os
+import re
 import shutil
 import subprocess
 
@@ -25,8 +26,10 @@ def test_fortran_binding_links_and_matches_python(tmp_path):
     subprocess.run([FLANG, os.path.join(FORT, "binding_check.f90"), str(tmp_path / "b.o"), f"-I{tmp_path}",
                     f"-L{lib_dir}", "-lmflbm", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, stdout=subprocess.PIPE, text=True).stdout
-    rc, chk = [int(t.split("=")[1]) for t in out.split()]
-    assert rc == 0
+    vals = dict(re.findall(r"(\w+)=\s*(\S+)", out))
+    rc, chk = int(vals["rc"]), int(vals["checksum"])
+    sp_rc, sp_sum = int(vals["sp_rc"]), float(vals["sp_sum"])
+    assert rc == 0 and sp_rc == 0
     # same call from Python
     lib = native.load()
     p = native.Params()
@@ -40,3 +43,18 @@ def test_fortran_binding_links_and_matches_python(tmp_path):
     kk, jj, ii = np.meshgrid(np.arange(-1, 5 + 3), np.arange(-1, 6 + 3), np.arange(-1, 8 + 3), indexing="ij")
     s = int((wh.astype(np.int64) * (ii + 3 * jj + 7 * kk + 100)).sum())
     assert s == chk
+
+    # single-phase init through the same interface
+    from mflbm.native import SPParams, dptr, ptr_array
+    q = SPParams()
+    q.nx, q.ny, q.nz, q.kper = 8, 6, 5, 1
+    q.wall_x_min = q.wall_x_max = q.wall_y_min = q.wall_y_max = 1
+    q.mrt_preset, q.la_nu = 1, 0.1
+    f = np.zeros((19, 7, 8, 10))
+    u = np.zeros((7, 8, 10))
+    u[:, :, :] = 0.01 * np.arange(10)[None, None, :]
+    assert lib.mflbm_sp_init_fields(C.byref(q), None, dptr(u), None, None, ptr_array(list(f)), None) == 0
+    t = 0.0
+    for m in range(19):
+        t = t + f[m].sum() * (m + 1)
+    assert sp_sum == pytest.approx(t, rel=1e-14)
