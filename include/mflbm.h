@@ -147,6 +147,10 @@ int mflbm_setup_walls(const mflbm_params* params, const int8_t* walls_global, in
  * (geometry_preprocessing_new, Geometry_preprocessing.F90:9-512). */
 int mflbm_geometry_new(const mflbm_params* params, const int8_t* walls_global, double theta,
                        mflbm_geometry** out);
+/* The same lists computed on the current HIP device (smoothing, marks, normals and the list extraction
+ * as kernels); results are identical to mflbm_geometry_new. */
+int mflbm_geometry_new_device(const mflbm_params* params, const int8_t* walls_global, double theta,
+                              mflbm_geometry** out);
 int mflbm_geometry_counts(const mflbm_geometry* geom, int32_t* n_solid, int32_t* n_fluid);
 int mflbm_geometry_export(const mflbm_geometry* geom, int32_t* s_ijk, int32_t* s_nfluid, int32_t* s_nbr,
                           double* s_weight, int32_t* f_ijk, double* f_nw, double* f_theta);

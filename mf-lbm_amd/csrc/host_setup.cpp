@@ -7,6 +7,7 @@
 #include <new>
 #include <vector>
 
+#include "geometry.h"
 #include "lattice.h"
 #include "mflbm.h"
 
@@ -29,13 +30,6 @@ bool bad_params(const mflbm_params* p) {
 }
 
 }  // namespace
-
-struct mflbm_geometry {
-    std::vector<int32_t> s_ijk, s_nfl, s_nbr;
-    std::vector<double> s_w;
-    std::vector<int32_t> f_ijk;
-    std::vector<double> f_nw, f_theta;
-};
 
 extern "C" int mflbm_setup_walls(const mflbm_params* p, const int8_t* wg, int8_t* walls_h2) {
     // set_walls for a single rank (Misc.F90:24-36, 130-210) followed by the periodic transport of the
@@ -87,7 +81,7 @@ extern "C" int mflbm_geometry_new(const mflbm_params* p, const int8_t* wg, doubl
     if (bad_params(p) || !wg || !out) return MFLBM_ERR_ARG;
     *out = nullptr;
     const int nx = p->nx, ny = p->ny, nz = p->nz;
-    const int GL = 10;
+    const int GL = geo::GL;
     if ((p->kper == 1 && nz < GL) || (p->jper == 1 && ny < GL)) return MFLBM_ERR_ARG;
     Grid G(nx, ny, nz, GL);
     std::vector<int8_t> t(G.size(), 0);
@@ -140,31 +134,19 @@ extern "C" int mflbm_geometry_new(const mflbm_params* p, const int8_t* wg, doubl
                         if (t[G.at(i + lat::EX[q], j + lat::EY[q], k + lat::EZ[q])] >= 1) { c = -1; break; }
                 }
             }
-    static const int IEX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
-    static const int IEY[27] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1};
-    static const int IEZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, 1, 1, -1};
-    const double WE[4] = {8.0 / 27.0, 2.0 / 27.0, 1.0 / 54.0, 1.0 / 216.0};
     for (int it = 0; it < 4; ++it) {
+        auto S1 = [&](int i, int j, int k) { return s1[G.at(i, j, k)]; };
         for (int k = 2 - GL; k <= nz + GL - 1; ++k)
             for (int j = 2 - GL; j <= ny + GL - 1; ++j)
-                for (int i = 2 - GL; i <= nx + GL - 1; ++i) {
-                    double acc = 0.0;
-                    for (int n = 0; n < 27; ++n) {
-                        int m = IEX[n] * IEX[n] + IEY[n] * IEY[n] + IEZ[n] * IEZ[n];
-                        acc = acc + s1[G.at(i + IEX[n], j + IEY[n], k + IEZ[n])] * WE[m];
-                    }
-                    s2[G.at(i, j, k)] = acc;
-                }
+                for (int i = 2 - GL; i <= nx + GL - 1; ++i) s2[G.at(i, j, k)] = geo::smooth27(S1, i, j, k);
         for (int k = 2 - GL; k <= nz + GL - 1; ++k)
             for (int j = 2 - GL; j <= ny + GL - 1; ++j)
                 for (int i = 2 - GL; i <= nx + GL - 1; ++i) s1[G.at(i, j, k)] = s2[G.at(i, j, k)];
     }
-    mflbm_geometry* geo = new (std::nothrow) mflbm_geometry();
-    if (!geo) return MFLBM_ERR_NOMEM;
+    mflbm_geometry* gm = new (std::nothrow) mflbm_geometry();
+    if (!gm) return MFLBM_ERR_NOMEM;
     const double* W = s2.data();
     auto S = [&](int i, int j, int k) { return W[G.at(i, j, k)]; };
-    const double I1 = 4.0 / 45.0, I2 = 1.0 / 21.0, I3 = 2.0 / 105.0, I4 = 5.0 / 504.0, I5 = 1.0 / 315.0,
-                 I6 = 1.0 / 630.0, I7 = 1.0 / 5040.0;
     for (int k = -3; k <= nz + 4; ++k)
         for (int j = -3; j <= ny + 4; ++j)
             for (int i = -3; i <= nx + 4; ++i) {
@@ -179,80 +161,27 @@ extern "C" int mflbm_geometry_new(const mflbm_params* p, const int8_t* wg, doubl
                             w = w + lat::W[q];
                             nbr[cnt++] = q;
                         }
-                    geo->s_ijk.push_back(i);
-                    geo->s_ijk.push_back(j);
-                    geo->s_ijk.push_back(k);
-                    geo->s_nfl.push_back(cnt);
-                    for (int q = 0; q < 18; ++q) geo->s_nbr.push_back(nbr[q]);
-                    geo->s_w.push_back(w);
+                    gm->s_ijk.push_back(i);
+                    gm->s_ijk.push_back(j);
+                    gm->s_ijk.push_back(k);
+                    gm->s_nfl.push_back(cnt);
+                    for (int q = 0; q < 18; ++q) gm->s_nbr.push_back(nbr[q]);
+                    gm->s_w.push_back(w);
                 } else if (c == -1) {
                     if (i < -1 || i > nx + 2 || j < -1 || j > ny + 2 || k < -1 || k > nz + 2) continue;
-                    // ISO8 gradient of the smoothed wall field (Geometry_preprocessing.F90:230-368)
-                    double nwx = I1 * (S(i + 1, j, k) - S(i - 1, j, k)) +
-                        I2 * (S(i + 1, j + 1, k) - S(i - 1, j - 1, k) + S(i + 1, j - 1, k) - S(i - 1, j + 1, k) +
-                              S(i + 1, j, k + 1) - S(i - 1, j, k - 1) + S(i + 1, j, k - 1) - S(i - 1, j, k + 1)) +
-                        I3 * (S(i + 1, j + 1, k + 1) - S(i - 1, j - 1, k - 1) + S(i + 1, j + 1, k - 1) - S(i - 1, j - 1, k + 1) +
-                              S(i + 1, j - 1, k + 1) - S(i - 1, j + 1, k - 1) + S(i + 1, j - 1, k - 1) - S(i - 1, j + 1, k + 1)) +
-                        I4 * (S(i + 2, j, k) - S(i - 2, j, k)) +
-                        I5 * (S(i + 2, j + 1, k) - S(i - 2, j - 1, k) + S(i + 2, j - 1, k) - S(i - 2, j + 1, k) +
-                              S(i + 2, j, k + 1) - S(i - 2, j, k - 1) + S(i + 2, j, k - 1) - S(i - 2, j, k + 1) +
-                              S(i + 1, j + 2, k) - S(i - 1, j - 2, k) + S(i + 1, j - 2, k) - S(i - 1, j + 2, k) +
-                              S(i + 1, j, k + 2) - S(i - 1, j, k - 2) + S(i + 1, j, k - 2) - S(i - 1, j, k + 2)) +
-                        I6 * (S(i + 2, j + 1, k + 1) - S(i - 2, j - 1, k - 1) + S(i + 2, j + 1, k - 1) - S(i - 2, j - 1, k + 1) +
-                              S(i + 2, j - 1, k + 1) - S(i - 2, j + 1, k - 1) + S(i + 2, j - 1, k - 1) - S(i - 2, j + 1, k + 1) +
-                              S(i + 1, j + 2, k + 1) - S(i - 1, j - 2, k - 1) + S(i + 1, j + 2, k - 1) - S(i - 1, j - 2, k + 1) +
-                              S(i + 1, j - 2, k + 1) - S(i - 1, j + 2, k - 1) + S(i + 1, j - 2, k - 1) - S(i - 1, j + 2, k + 1) +
-                              S(i + 1, j + 1, k + 2) - S(i - 1, j - 1, k - 2) + S(i + 1, j + 1, k - 2) - S(i - 1, j - 1, k + 2) +
-                              S(i + 1, j - 1, k + 2) - S(i - 1, j + 1, k - 2) + S(i + 1, j - 1, k - 2) - S(i - 1, j + 1, k + 2)) +
-                        I7 * (S(i + 2, j + 2, k) - S(i - 2, j - 2, k) + S(i + 2, j - 2, k) - S(i - 2, j + 2, k) +
-                              S(i + 2, j, k + 2) - S(i - 2, j, k - 2) + S(i + 2, j, k - 2) - S(i - 2, j, k + 2));
-                    double nwy = I1 * (S(i, j + 1, k) - S(i, j - 1, k)) +
-                        I2 * (S(i + 1, j + 1, k) - S(i - 1, j - 1, k) + S(i - 1, j + 1, k) - S(i + 1, j - 1, k) +
-                              S(i, j + 1, k + 1) - S(i, j - 1, k - 1) + S(i, j + 1, k - 1) - S(i, j - 1, k + 1)) +
-                        I3 * (S(i + 1, j + 1, k + 1) - S(i - 1, j - 1, k - 1) + S(i + 1, j + 1, k - 1) - S(i - 1, j - 1, k + 1) +
-                              S(i - 1, j + 1, k - 1) - S(i + 1, j - 1, k + 1) + S(i - 1, j + 1, k + 1) - S(i + 1, j - 1, k - 1)) +
-                        I4 * (S(i, j + 2, k) - S(i, j - 2, k)) +
-                        I5 * (S(i + 2, j + 1, k) - S(i - 2, j - 1, k) + S(i - 2, j + 1, k) - S(i + 2, j - 1, k) +
-                              S(i, j + 2, k + 1) - S(i, j - 2, k - 1) + S(i, j + 2, k - 1) - S(i, j - 2, k + 1) +
-                              S(i + 1, j + 2, k) - S(i - 1, j - 2, k) + S(i - 1, j + 2, k) - S(i + 1, j - 2, k) +
-                              S(i, j + 1, k + 2) - S(i, j - 1, k - 2) + S(i, j + 1, k - 2) - S(i, j - 1, k + 2)) +
-                        I6 * (S(i + 2, j + 1, k + 1) - S(i - 2, j - 1, k - 1) + S(i + 2, j + 1, k - 1) - S(i - 2, j - 1, k + 1) +
-                              S(i - 2, j + 1, k + 1) - S(i + 2, j - 1, k - 1) + S(i - 2, j + 1, k - 1) - S(i + 2, j - 1, k + 1) +
-                              S(i + 1, j + 2, k + 1) - S(i - 1, j - 2, k - 1) + S(i + 1, j + 2, k - 1) - S(i - 1, j - 2, k + 1) +
-                              S(i - 1, j + 2, k + 1) - S(i + 1, j - 2, k - 1) + S(i - 1, j + 2, k - 1) - S(i + 1, j - 2, k + 1) +
-                              S(i + 1, j + 1, k + 2) - S(i - 1, j - 1, k - 2) + S(i + 1, j + 1, k - 2) - S(i - 1, j - 1, k + 2) +
-                              S(i - 1, j + 1, k + 2) - S(i + 1, j - 1, k - 2) + S(i - 1, j + 1, k - 2) - S(i + 1, j - 1, k + 2)) +
-                        I7 * (S(i + 2, j + 2, k) - S(i - 2, j - 2, k) + S(i - 2, j + 2, k) - S(i + 2, j - 2, k) +
-                              S(i, j + 2, k + 2) - S(i, j - 2, k - 2) + S(i, j + 2, k - 2) - S(i, j - 2, k + 2));
-                    double nwz = I1 * (S(i, j, k + 1) - S(i, j, k - 1)) +
-                        I2 * (S(i, j + 1, k + 1) - S(i, j - 1, k - 1) + S(i, j - 1, k + 1) - S(i, j + 1, k - 1) +
-                              S(i + 1, j, k + 1) - S(i - 1, j, k - 1) + S(i - 1, j, k + 1) - S(i + 1, j, k - 1)) +
-                        I3 * (S(i + 1, j + 1, k + 1) - S(i - 1, j - 1, k - 1) + S(i + 1, j - 1, k + 1) - S(i - 1, j + 1, k - 1) +
-                              S(i - 1, j + 1, k + 1) - S(i + 1, j - 1, k - 1) + S(i - 1, j - 1, k + 1) - S(i + 1, j + 1, k - 1)) +
-                        I4 * (S(i, j, k + 2) - S(i, j, k - 2)) +
-                        I5 * (S(i, j + 1, k + 2) - S(i, j - 1, k - 2) + S(i, j - 1, k + 2) - S(i, j + 1, k - 2) +
-                              S(i + 2, j, k + 1) - S(i - 2, j, k - 1) + S(i - 2, j, k + 1) - S(i + 2, j, k - 1) +
-                              S(i, j + 2, k + 1) - S(i, j - 2, k - 1) + S(i, j - 2, k + 1) - S(i, j + 2, k - 1) +
-                              S(i + 1, j, k + 2) - S(i - 1, j, k - 2) + S(i - 1, j, k + 2) - S(i + 1, j, k - 2)) +
-                        I6 * (S(i + 2, j + 1, k + 1) - S(i - 2, j - 1, k - 1) + S(i + 2, j - 1, k + 1) - S(i - 2, j + 1, k - 1) +
-                              S(i - 2, j + 1, k + 1) - S(i + 2, j - 1, k - 1) + S(i - 2, j - 1, k + 1) - S(i + 2, j + 1, k - 1) +
-                              S(i + 1, j + 2, k + 1) - S(i - 1, j - 2, k - 1) + S(i + 1, j - 2, k + 1) - S(i - 1, j + 2, k - 1) +
-                              S(i - 1, j + 2, k + 1) - S(i + 1, j - 2, k - 1) + S(i - 1, j - 2, k + 1) - S(i + 1, j + 2, k - 1) +
-                              S(i + 1, j + 1, k + 2) - S(i - 1, j - 1, k - 2) + S(i + 1, j - 1, k + 2) - S(i - 1, j + 1, k - 2) +
-                              S(i - 1, j + 1, k + 2) - S(i + 1, j - 1, k - 2) + S(i - 1, j - 1, k + 2) - S(i + 1, j + 1, k - 2)) +
-                        I7 * (S(i, j + 2, k + 2) - S(i, j - 2, k - 2) + S(i, j - 2, k + 2) - S(i, j + 2, k - 2) +
-                              S(i + 2, j, k + 2) - S(i - 2, j, k - 2) + S(i - 2, j, k + 2) - S(i + 2, j, k - 2));
+                    double nwx, nwy, nwz;
+                    geo::iso8_gradient(S, i, j, k, nwx, nwy, nwz);
                     double tmp = 1.0 / (std::sqrt(nwx * nwx + nwy * nwy + nwz * nwz) + lat::EPS);
-                    geo->f_ijk.push_back(i);
-                    geo->f_ijk.push_back(j);
-                    geo->f_ijk.push_back(k);
-                    geo->f_nw.push_back(nwx * tmp);
-                    geo->f_nw.push_back(nwy * tmp);
-                    geo->f_nw.push_back(nwz * tmp);
-                    geo->f_theta.push_back(theta);
+                    gm->f_ijk.push_back(i);
+                    gm->f_ijk.push_back(j);
+                    gm->f_ijk.push_back(k);
+                    gm->f_nw.push_back(nwx * tmp);
+                    gm->f_nw.push_back(nwy * tmp);
+                    gm->f_nw.push_back(nwz * tmp);
+                    gm->f_theta.push_back(theta);
                 }
             }
-    *out = geo;
+    *out = gm;
     return MFLBM_OK;
 }
 

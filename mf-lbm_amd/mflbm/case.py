@@ -86,7 +86,10 @@ def _check(rc: int, what: str) -> None:
 
 
 class Case:
-    def __init__(self, control: Dict[str, object], sample: np.ndarray):
+    """``geometry="device"`` builds the boundary lists on the GPU (mflbm_geometry_new_device, identical
+    results); the default is the host routine."""
+
+    def __init__(self, control: Dict[str, object], sample: np.ndarray, geometry: str = "host"):
         lib = native.load()
         self.control = dict(control)
         self.rp = io.derive_params(control)
@@ -102,7 +105,10 @@ class Case:
             # the geometry pass wraps 10 ghost layers around periodic directions
             # (Geometry_preprocessing.F90:54-83), which needs at least 10 nodes along them
             raise ValueError("a periodic direction needs at least 10 lattice nodes")
-        _check(lib.mflbm_geometry_new(C.byref(self.params), i8ptr(wg), rp.theta, C.byref(geo)), "mflbm_geometry_new")
+        if geometry not in ("host", "device"):
+            raise ValueError("geometry must be 'host' or 'device'")
+        fn = lib.mflbm_geometry_new_device if geometry == "device" else lib.mflbm_geometry_new
+        _check(fn(C.byref(self.params), i8ptr(wg), rp.theta, C.byref(geo)), fn.__name__)
         try:
             ns, nf = C.c_int32(), C.c_int32()
             lib.mflbm_geometry_counts(geo, C.byref(ns), C.byref(nf))

@@ -108,6 +108,7 @@ _HOST_ENTRY_POINTS = {
     "mflbm_create_error": (C.c_char_p, []),
     "mflbm_setup_walls": (c_int, [C.POINTER(Params), P_i8, P_i8]),
     "mflbm_geometry_new": (c_int, [C.POINTER(Params), P_i8, c_dbl, C.POINTER(C.c_void_p)]),
+    "mflbm_geometry_new_device": (c_int, [C.POINTER(Params), P_i8, c_dbl, C.POINTER(C.c_void_p)]),
     "mflbm_geometry_counts": (c_int, [C.c_void_p, P_int, P_int]),
     "mflbm_geometry_export": (c_int, [C.c_void_p, P_int, P_int, P_int, P_dbl, P_int, P_dbl, P_dbl]),
     "mflbm_geometry_free": (None, [C.c_void_p]),
