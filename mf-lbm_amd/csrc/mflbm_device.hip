@@ -1165,8 +1165,9 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     std::vector<int> extra_z;
     if ((p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2) && p.z_porous_plate >= 1 && p.z_porous_plate <= L.nz)
         extra_z = {p.z_porous_plate - 1, p.z_porous_plate, p.z_porous_plate + 1};
+    const TileOrder T{tile_rows(), L};
     CompactSet cset;
-    build_compact(L, wp, open_z(p), p.kper == 1, p.jper == 1, extra_z, cset);
+    build_compact(L, T, wp, open_z(p), p.kper == 1, p.jper == 1, extra_z, cset);
     const std::vector<int>& cidx = cset.cidx;
     const std::vector<int>& cpos = cset.cpos;
     const int nfl = cset.nfl;
@@ -1177,19 +1178,29 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     const int nc = (int)cpos.size();
     // nodes whose normal is computed: walls == 0 in [-1, n+2]^3
     std::vector<int> nrm;
-    for (int k = -1; k <= L.nz + 2; ++k)
-        for (int j = -1; j <= L.ny + 2; ++j)
-            for (int i = -1; i <= L.nx + 2; ++i)
-                if (wp[L.at(i, j, k)] == 0) nrm.push_back(L.at(i, j, k));
-    // solid list (all, and the part inside one halo layer for the normal extrapolation)
+    T.visit(-1, L.nx + 2, -1, L.ny + 2, -1, L.nz + 2, [&](int i, int j, int k) {
+        if (wp[L.at(i, j, k)] == 0) nrm.push_back(L.at(i, j, k));
+    });
+    // solid list (all, and the part inside one halo layer for the normal extrapolation), in the tile order;
+    // every entry is independent, so the order changes no result
+    std::vector<int> sperm(ns);
+    for (int t = 0; t < ns; ++t) sperm[t] = t;
+    {
+        std::vector<long> skey(ns);
+        for (int t = 0; t < ns; ++t) skey[t] = T.key(L.at(s_ijk[3 * t], s_ijk[3 * t + 1], s_ijk[3 * t + 2]));
+        std::stable_sort(sperm.begin(), sperm.end(), [&](int a, int b) { return skey[a] < skey[b]; });
+    }
     std::vector<int> spos(ns), s1pos;
-    std::vector<unsigned> s1mask;
-    std::vector<double> s1w;
-    for (int t = 0; t < ns; ++t) {
+    std::vector<unsigned> smask_o(ns), s1mask;
+    std::vector<double> sw_o(ns), s1w;
+    for (int u = 0; u < ns; ++u) {
+        const int t = sperm[u];
         const int i = s_ijk[3 * t], j = s_ijk[3 * t + 1], k = s_ijk[3 * t + 2];
-        spos[t] = L.at(i, j, k);
+        spos[u] = L.at(i, j, k);
+        smask_o[u] = smask[t];
+        sw_o[u] = s_w[t];
         if (i >= 0 && i <= L.nx + 1 && j >= 0 && j <= L.ny + 1 && k >= 0 && k <= L.nz + 1) {
-            s1pos.push_back(spos[t]);
+            s1pos.push_back(spos[u]);
             s1mask.push_back(smask[t]);
             s1w.push_back(s_w[t]);
         }
@@ -1252,10 +1263,10 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     hi.insert(hi.end(), s1pos.begin(), s1pos.end());
     hi.insert(hi.end(), fpos.begin(), fpos.end());
     hi.push_back(0);
-    std::vector<unsigned> hm(smask);
+    std::vector<unsigned> hm(smask_o);
     hm.insert(hm.end(), s1mask.begin(), s1mask.end());
     hm.push_back(0);
-    std::vector<double> hd(s_w, s_w + ns);
+    std::vector<double> hd(sw_o);
     hd.insert(hd.end(), s1w.begin(), s1w.end());
     hd.insert(hd.end(), fd.begin(), fd.end());
     hd.push_back(0.0);

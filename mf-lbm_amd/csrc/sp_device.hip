@@ -617,8 +617,9 @@ extern "C" int mflbm_sp_upload_geometry(mflbm_sp_ctx* c, const int8_t* walls_h2,
     if (sp_open_z(p) && p.inlet_bc == 1 && !w_in) return fail(c, MFLBM_ERR_ARG, "velocity inlet needs w_in");
     std::vector<int8_t> wp(L.total, 1);
     pack_host(L, 2, walls_h2, wp.data());
+    const TileOrder T{tile_rows(), L};
     CompactSet cset;
-    build_compact(L, wp, sp_open_z(p), p.kper == 1, p.jper == 1, {}, cset);
+    build_compact(L, T, wp, sp_open_z(p), p.kper == 1, p.jper == 1, {}, cset);
     const int nc = (int)cset.cpos.size();
     LinkSet links;
     build_links(L, cset, links);

@@ -87,9 +87,11 @@ def _check(rc: int, what: str) -> None:
 
 class Case:
     """``geometry="device"`` builds the boundary lists on the GPU (mflbm_geometry_new_device, identical
-    results); the default is the host routine."""
+    results); the default is the host routine.  ``boundary_file`` loads the lists from a boundary-info file
+    instead (geometry_preprocess_cmd 1, Geometry_preprocessing.F90:524-685)."""
 
-    def __init__(self, control: Dict[str, object], sample: np.ndarray, geometry: str = "host"):
+    def __init__(self, control: Dict[str, object], sample: np.ndarray, geometry: str = "host",
+                 boundary_file: Optional[str] = None):
         lib = native.load()
         self.control = dict(control)
         self.rp = io.derive_params(control)
@@ -100,15 +102,27 @@ class Case:
         self.walls_h2 = np.zeros((nz + 4, ny + 4, nx + 4), dtype=np.int8)
         wg = np.ascontiguousarray(self.walls_global)
         _check(lib.mflbm_setup_walls(C.byref(self.params), i8ptr(wg), i8ptr(self.walls_h2)), "mflbm_setup_walls")
-        geo = C.c_void_p()
         if (rp.kper == 1 and nz < 10) or (rp.jper == 1 and ny < 10):
             # the geometry pass wraps 10 ghost layers around periodic directions
             # (Geometry_preprocessing.F90:54-83), which needs at least 10 nodes along them
             raise ValueError("a periodic direction needs at least 10 lattice nodes")
+        if boundary_file is not None:
+            s_ijk, s_nfl, s_nbr, s_w, f_ijk, f_nw = io.read_boundary_info(boundary_file, nx, ny, nz)
+            b = BoundaryLists(s_ijk, s_nfl, s_nbr, s_w, f_ijk, f_nw, np.full(len(f_ijk), rp.theta))
+        else:
+            b = self._compute_lists(lib, wg, geometry)
+        self.lists = b
+        self.w_in = np.zeros((ny + 2, nx + 2))
+        if rp.open_z and rp.inlet_bc == 1:
+            _check(lib.mflbm_inlet_profile(C.byref(self.params), rp.uin_avg, rp.la_x, rp.la_y, dptr(self.w_in)),
+                   "mflbm_inlet_profile")
+
+    def _compute_lists(self, lib, wg: np.ndarray, geometry: str) -> "BoundaryLists":
         if geometry not in ("host", "device"):
             raise ValueError("geometry must be 'host' or 'device'")
         fn = lib.mflbm_geometry_new_device if geometry == "device" else lib.mflbm_geometry_new
-        _check(fn(C.byref(self.params), i8ptr(wg), rp.theta, C.byref(geo)), fn.__name__)
+        geo = C.c_void_p()
+        _check(fn(C.byref(self.params), i8ptr(wg), self.rp.theta, C.byref(geo)), fn.__name__)
         try:
             ns, nf = C.c_int32(), C.c_int32()
             lib.mflbm_geometry_counts(geo, C.byref(ns), C.byref(nf))
@@ -119,11 +133,7 @@ class Case:
                                              iptr(b.f_ijk), dptr(b.f_nw), dptr(b.f_theta)), "mflbm_geometry_export")
         finally:
             lib.mflbm_geometry_free(geo)
-        self.lists = b
-        self.w_in = np.zeros((ny + 2, nx + 2))
-        if rp.open_z and rp.inlet_bc == 1:
-            _check(lib.mflbm_inlet_profile(C.byref(self.params), rp.uin_avg, rp.la_x, rp.la_y, dptr(self.w_in)),
-                   "mflbm_inlet_profile")
+        return b
 
     @property
     def n_fluid(self) -> int:

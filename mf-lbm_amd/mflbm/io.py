@@ -493,3 +493,58 @@ def write_checkpoint_sp(path: str, ck: SPCheckpoint) -> None:
         fh.write(np.ascontiguousarray(ck.f, dtype="<f8").tobytes())
         if ck.f_convec is not None:
             fh.write(np.ascontiguousarray(ck.f_convec, dtype="<f8").tobytes())
+
+
+# -------------------------------------------------------------------------------------------------
+# boundary-info file (geometry_preprocess_cmd 1): the lists written by the reference's preprocessing tool
+# (preprocessing/3.wall_boundary_preprocess/3d_wall_boundary_process.f90) and read by
+# geometry_preprocessing_load (multiphase_3D/0.src/Geometry_preprocessing.F90:524-685)
+# -------------------------------------------------------------------------------------------------
+
+# stream layout: int32 n_solid, n_fluid; n_solid records {int32 ix, iy, iz, i_fluid_num, neighbor_list(18);
+# f64 la_weight} (96 bytes); n_fluid records {int32 ix, iy, iz; f64 nwx, nwy, nwz}, 40 bytes where the
+# compiler aligns the reals (gfortran: 4 bytes of padding) or 36 bytes packed (flang).  The record layout
+# follows the compiler of the tool that wrote the file (its compile.sh says so); both are read.
+_SOLID_REC = np.dtype([("ijk", "<i4", 3), ("nfl", "<i4"), ("nbr", "<i4", 18), ("w", "<f8")])
+_FLUID_REC = np.dtype([("ijk", "<i4", 3), ("pad", "<i4"), ("nw", "<f8", 3)])
+_FLUID_REC_PACKED = np.dtype([("ijk", "<i4", 3), ("nw", "<f8", 3)], align=False)
+assert _SOLID_REC.itemsize == 96 and _FLUID_REC.itemsize == 40 and _FLUID_REC_PACKED.itemsize == 36
+
+
+def write_boundary_info(path: str, s_ijk, s_nfl, s_nbr, s_w, f_ijk, f_nw, packed: bool = False) -> None:
+    s = np.zeros(len(s_nfl), dtype=_SOLID_REC)
+    s["ijk"], s["nfl"], s["nbr"], s["w"] = s_ijk, s_nfl, s_nbr, s_w
+    f = np.zeros(len(f_ijk), dtype=_FLUID_REC_PACKED if packed else _FLUID_REC)
+    f["ijk"], f["nw"] = f_ijk, f_nw
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<ii", len(s), len(f)))
+        fh.write(s.tobytes())
+        fh.write(f.tobytes())
+
+
+def read_boundary_info(path: str, nx: int, ny: int, nz: int):
+    """Read a boundary-info file and keep, as geometry_preprocessing_load does for a single rank, the solid
+    nodes within 3 ghost layers and the fluid nodes within 2 (:606-676).  Returns (s_ijk, s_nfl, s_nbr, s_w,
+    f_ijk, f_nw) as contiguous arrays; the caller assigns the contact angle to every fluid node."""
+    with open(path, "rb") as fh:
+        raw = fh.read()
+    ns, nf = struct.unpack("<ii", raw[:8])
+    rest = len(raw) - 8 - ns * 96
+    if rest == nf * 40:
+        frec = _FLUID_REC
+    elif rest == nf * 36:
+        frec = _FLUID_REC_PACKED
+    else:
+        raise ValueError(f"boundary-info file size does not match {ns} solid and {nf} fluid records")
+    s = np.frombuffer(raw, dtype=_SOLID_REC, count=ns, offset=8)
+    f = np.frombuffer(raw, dtype=frec, count=nf, offset=8 + ns * 96)
+
+    def keep(ijk, g):
+        lo = 1 - g
+        return ((ijk[:, 0] >= lo) & (ijk[:, 0] <= nx + g) & (ijk[:, 1] >= lo) & (ijk[:, 1] <= ny + g) &
+                (ijk[:, 2] >= lo) & (ijk[:, 2] <= nz + g))
+
+    ks, kf = keep(s["ijk"], 3), keep(f["ijk"], 2)
+    c = np.ascontiguousarray
+    return (c(s["ijk"][ks], dtype=np.int32), c(s["nfl"][ks], dtype=np.int32), c(s["nbr"][ks], dtype=np.int32),
+            c(s["w"][ks], dtype=np.float64), c(f["ijk"][kf], dtype=np.int32), c(f["nw"][kf], dtype=np.float64))

@@ -6,6 +6,8 @@
 # oracle/_ref/ (git-ignored):
 #   oracle/_ref/MF_LBM_multiphase.cpu   two-phase solver   (multiphase_3D/0.src/*.F90)
 #   oracle/_ref/MF_LBM_singlephase.cpu  single-phase solver (singlephase_3D/0.src/*.F90)
+#   oracle/_ref/wall_boundary_process   boundary-info preprocessing tool
+#                                       (preprocessing/3.wall_boundary_preprocess/3d_wall_boundary_process.f90)
 # The binaries run as a singleton MPI process (no mpirun needed), single thread unless OMP_NUM_THREADS
 # is raised.  Flags: -O2 on generic x86-64 (no FMA contraction is possible without -march), OpenMP on.
 set -euo pipefail
@@ -34,4 +36,7 @@ build() {   # build <name> <srcdir>
 }
 build multiphase "$REF/multiphase_3D/0.src"
 build singlephase "$REF/singlephase_3D/0.src"
-echo "built: $OUT/MF_LBM_multiphase.cpu $OUT/MF_LBM_singlephase.cpu"
+mkdir -p "$OUT/obj_tool"
+"$FC" -O2 -fopenmp -module-dir "$OUT/obj_tool" "$REF/preprocessing/3.wall_boundary_preprocess/3d_wall_boundary_process.f90" \
+    -o "$OUT/wall_boundary_process"
+echo "built: $OUT/MF_LBM_multiphase.cpu $OUT/MF_LBM_singlephase.cpu $OUT/wall_boundary_process"
