@@ -18,7 +18,8 @@ cores for a bounded number of steps of the same workload.
 
 ``--workload singlephase`` measures the single-phase program's step instead (D3Q19 MRT, one population set,
 bounce-back-optimised rates, nu = 0.1) on the same 256^3 pack, z-periodic and driven by a body force of 1e-6
-(a permeability run); 305 B per fluid node.
+(a permeability run); 305 B per fluid node.  ``--precision fp32`` stores its populations in FP32 as deviations
+from the rest equilibrium and collides in FP32 (153 B per fluid node).
 """
 from __future__ import annotations
 
@@ -35,6 +36,7 @@ sys.path.insert(0, os.path.join(REPO, "mf-lbm_amd"))
 METRIC = "MLUPS (D3Q19+D3Q7, FP64) at 256^3 porous; time-to-converged permeability"
 B_ALG = 625.0          # bytes per fluid node per step, two-phase FP64
 B_ALG_SP = 305.0       # bytes per fluid node per step, single-phase FP64
+B_ALG_SP32 = 153.0     # bytes per fluid node per step, single-phase FP32 deviation storage
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
@@ -49,14 +51,14 @@ def workload(n: int = 256):
     return Case(ctl, walls)
 
 
-def workload_sp(n: int = 256):
+def workload_sp(n: int = 256, precision: str = "fp64"):
     from mflbm import geometry
     from mflbm.singlephase import SPCase
     walls = geometry.sphere_pack(n, n, n, 0.22, 6.0, 12.0, 2024, buffer=8)
     ctl = {"lattice_dimensions": (n, n, n), "periodic_indicator": (0, 0, 1), "domain_wall_status_z": (0, 0),
            "MPI_async_layers_num": (0, 0, 4), "fluid_viscosity": 0.1, "body_force_0": 1e-6,
            "MRT_collision_parameter_preset": 1}
-    return SPCase(ctl, walls)
+    return SPCase(ctl, walls, precision)
 
 
 def cpu_baseline(case, st, steps: int, threads: int, single_phase: bool = False):
@@ -153,7 +155,7 @@ def run_twophase(args) -> dict:
 def run_singlephase(args) -> dict:
     from mflbm.singlephase import SPSimulation
     t0 = time.time()
-    case = workload_sp(args.size)
+    case = workload_sp(args.size, args.precision)
     st = case.initial_state()
     print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
     sim = SPSimulation(case, state=st)
@@ -161,18 +163,74 @@ def run_singlephase(args) -> dict:
     mon = sim.monitor()
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
+    fp32 = args.precision == "fp32"
+    tag = "FP32 deviation storage" if fp32 else "FP64"
     return {
-        "metric": "MLUPS (single-phase D3Q19 MRT, FP64) at 256^3 porous",
+        "metric": f"MLUPS (single-phase D3Q19 MRT, {tag}) at 256^3 porous",
         "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32" if fp32 else "f64", "data": "synthetic",
         "config": {"workload": f"single-phase permeability run, {args.size}^3 seeded sphere pack porosity 0.22, "
-                               "z-periodic, body force 1e-6, nu 0.1, FP64",
+                               f"z-periodic, body force 1e-6, nu 0.1, {tag}",
                    "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3},
-        "roofline": roofline(B_ALG_SP, case.n_fluid, means[0], traffic_from_profiles(f"sp_sphere_pack_{args.size}"),
+        "roofline": roofline(B_ALG_SP32 if fp32 else B_ALG_SP, case.n_fluid, means[0],
+                             traffic_from_profiles(f"sp{'32' if fp32 else ''}_sphere_pack_{args.size}"),
                              "k_sp_collide (even/odd mean)", step_ms_events=round(means[1], 4)),
         "cpu_baseline": cpu_leg(args, case, st, single_phase=True),
         "check": {"flowrate": mon.flowrate, "umax": mon.umax, "failed": mon.failed},
+    }
+
+
+def run_batch(args) -> dict:
+    """96 REV cubes (sides from {48, 64, 96, 128}, sphere packs of porosity 0.15-0.35, seed 2024), nu 0.1,
+    body force 1e-6, FP32 deviation storage with autotuned tile rows and block size.  value: MLUPS of the
+    batched lattice update over all samples (all running); time_to_converged: wall time of the steady run
+    until every sample meets the flow-rate criterion 1e-7, in FP32 and in FP64, with the FP32 permeability
+    error against FP64 per sample."""
+    import numpy as np
+    from mflbm import batch
+    t0 = time.time()
+    samples = batch.rev_samples(args.batch_samples, seed=2024)
+    print(f"[bench] {len(samples)} samples generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    tune = batch.autotune(samples, "fp32")
+    print(f"[bench] autotune: tile_rows {tune['tile_rows']} block {tune['block']}", file=sys.stderr, flush=True)
+    kw = dict(la_nu=0.1, force_z=1e-6, ntime_monitor=500, tol=1e-7, tile_rows=tune["tile_rows"], block=tune["block"])
+    b = batch.REVBatch(samples, precision="fp32", **kw)
+    b.run(1, args.warmup)
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.run(1 + args.warmup, args.steps)
+    b.synchronize()
+    dt = time.perf_counter() - t1
+    nodes, pop_bytes = b.fluid_nodes, b.pop_bytes
+    b.close()
+    mlups = nodes * args.steps / dt / 1e6
+    conv = {}
+    for prec in ("fp32", "fp64"):
+        bb = batch.REVBatch(samples, precision=prec, **kw)
+        r = bb.run_steady(args.batch_max_steps)
+        bb.close()
+        conv[prec] = r
+        print(f"[bench] {prec}: steady in {r.seconds:.2f}s, {int((r.conv_step > 0).sum())}/{len(samples)} converged, "
+              f"max step {int(r.conv_step.max())}", file=sys.stderr, flush=True)
+    rel = np.abs(conv["fp32"].permeability / conv["fp64"].permeability - 1.0)
+    ms = dt / args.steps * 1e3
+    return {
+        "metric": "MLUPS (batched single-phase REVs, D3Q19 MRT, FP32 deviation storage); time-to-converged permeability",
+        "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(samples)} REV cubes, sides from {{48,64,96,128}}, sphere packs porosity "
+                               "0.15-0.35 (seed 2024), z-periodic, body force 1e-6, nu 0.1, FP32 deviation storage, "
+                               "one lattice-update launch per step for all samples",
+                   "fluid_nodes": nodes, "samples": len(samples), "autotune": {k: tune[k] for k in ("tile_rows", "block")}},
+        "roofline": roofline(B_ALG_SP32, nodes, ms, None, "k_spb_collide + periodic exchange (wall time per step)"),
+        "time_to_converged": {
+            p: {"seconds": round(conv[p].seconds, 3), "converged": int((conv[p].conv_step > 0).sum()),
+                "max_step": int(conv[p].conv_step.max())} for p in conv},
+        "permeability_error_fp32_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
+        "autotune_timings": tune["timings"],
+        "cpu_baseline": None,
     }
 
 
@@ -184,13 +242,21 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("twophase", "singlephase"), default="twophase")
+    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch"), default="twophase")
+    ap.add_argument("--batch-samples", type=int, default=96)
+    ap.add_argument("--batch-max-steps", type=int, default=100000)
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
+                    help="single-phase storage: fp64 (reference arithmetic) or fp32 deviations")
     args = ap.parse_args()
+    if args.precision == "fp32" and args.workload == "twophase":
+        print("--precision fp32 applies to the single-phase workload", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
               "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    out = run_singlephase(args) if args.workload == "singlephase" else run_twophase(args)
+    run = {"singlephase": run_singlephase, "batch": run_batch}.get(args.workload, run_twophase)
+    out = run(args)
     print(json.dumps(out))
 
 

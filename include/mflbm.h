@@ -185,6 +185,9 @@ typedef struct mflbm_sp_params {
     int32_t outlet_bc;         /* 1 convective, 2 Zou-He pressure */
     int32_t n_exclude_inlet, n_exclude_outlet;
     int32_t mrt_preset;        /* MRT_collision_parameter_preset: 1 bounce-back optimised, else SRT */
+    int32_t precision;         /* 0: FP64 populations, the reference arithmetic (bit-identical);
+                                * 1: FP32 deviations from the rest equilibrium, FP32 collision, FP64 boundaries,
+                                *    macroscopic fields and reductions (no reference counterpart) */
     double la_nu;              /* fluid_viscosity */
     double force_z;            /* body force along z (force_z0) */
     double rho_in, rho_out;    /* Zou-He densities (rho_out = 1) */
@@ -226,6 +229,49 @@ int mflbm_sp_timing_enable(mflbm_sp_ctx* ctx, int32_t enable);
  * convective-outlet history f_convec_bc(:, :, q) = f_q(:, :, nz) (may be NULL). */
 int mflbm_sp_init_fields(const mflbm_sp_params* params, const double* rho0, const double* u0, const double* v0,
                          const double* w0, double* const* f, double* f_convec);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Batched single-phase permeability runs (no reference counterpart: the reference runs one sample per job
+ * through singlephase_3D/0.src/Main.F90:124-230).  Every sample is a complete single-phase problem with
+ * solid x walls, z periodic, y walls or y periodic, a body force along z and no open boundaries; all samples
+ * share the rates and the force.  One lattice-update launch per step covers all samples.  A steady run
+ * replays a captured monitor period (ntime_monitor steps + the flow-rate test of Monitor.F90:145-156 per
+ * sample) until every sample has converged; a converged sample stops and keeps its state.
+ * --------------------------------------------------------------------------------------------------------- */
+typedef struct mflbm_spb_params {
+    int32_t precision;         /* 0 FP64 populations (the reference arithmetic), 1 FP32 deviations */
+    int32_t mrt_preset;        /* as mflbm_sp_params */
+    int32_t ntime_monitor;     /* steps per convergence test (even: the flow rate is read after even steps) */
+    int32_t tile_rows;         /* compact node order: padded y rows per tile (0: MFLBM_TILE_Y or 32) */
+    int32_t block;             /* threads per block of the lattice update: 64, 128, 256 or 512 (0: 256) */
+    int32_t reserved;          /* 0 */
+    double la_nu;              /* fluid_viscosity */
+    double force_z;            /* body force along z */
+    double tol;                /* convergence_criteria: |Q - Q_prev| / (|Q| + 1e-14) < tol */
+} mflbm_spb_params;
+typedef struct mflbm_spb_ctx mflbm_spb_ctx;
+int mflbm_spb_create(const mflbm_spb_params* params, mflbm_spb_ctx** out);
+void mflbm_spb_destroy(mflbm_spb_ctx* ctx);
+const char* mflbm_spb_last_error(const mflbm_spb_ctx* ctx);
+/* walls with halo 2 as mflbm_setup_walls returns them for (nx, ny, nz, jper, kper = 1), x walls */
+int mflbm_spb_add_sample(mflbm_spb_ctx* ctx, int32_t nx, int32_t ny, int32_t nz, int32_t jper, const int8_t* walls_h2);
+/* lays out and uploads every sample; initial state: fluid at rest, rho 1 (initialization_new) */
+int mflbm_spb_commit(mflbm_spb_ctx* ctx);
+int mflbm_spb_info(mflbm_spb_ctx* ctx, int32_t* nsamples, int64_t* fluid_nodes, int64_t* pop_bytes);
+int mflbm_spb_sample_fluid(mflbm_spb_ctx* ctx, int32_t sample, int64_t* fluid_nodes);
+/* one sample's populations in the layout of mflbm_sp_upload_state / _download_state */
+int mflbm_spb_upload_state(mflbm_spb_ctx* ctx, int32_t sample, const double* const* f);
+int mflbm_spb_download_state(mflbm_spb_ctx* ctx, int32_t sample, double* const* f);
+/* plain steps of every running sample (kernel_odd / kernel_even by the parity of ntime, periodic wrap) */
+int mflbm_spb_run(mflbm_spb_ctx* ctx, int32_t ntime0, int32_t nsteps);
+/* up to max_periods monitor periods; returns early when every sample has stopped; nactive = samples left */
+int mflbm_spb_run_steady(mflbm_spb_ctx* ctx, int32_t max_periods, int32_t* nactive);
+/* per sample: step of convergence (> 0), of a NaN flow rate (< 0) or 0 while running; the last flow rate
+ * Q = sum(w) / nz; periods = monitor periods done */
+int mflbm_spb_results(mflbm_spb_ctx* ctx, int32_t* conv_step, double* flowrate, int32_t* periods);
+/* restarts the monitor: every sample runs again, period count and previous flow rates cleared */
+int mflbm_spb_reset_monitor(mflbm_spb_ctx* ctx);
+int mflbm_spb_synchronize(mflbm_spb_ctx* ctx);
 
 #ifdef __cplusplus
 }

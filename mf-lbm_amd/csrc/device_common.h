@@ -211,10 +211,14 @@ __device__ __forceinline__ int link_slot(int nc, const int* wbase, const unsigne
 
 // Slot access with a scalar slice base and a 32-bit per-lane byte offset (one global load/store with an SGPR
 // base instead of a 64-bit vector address per slot); slices are kept below 2^32 bytes.
-__device__ __forceinline__ double slot_ld(const double* slice, unsigned i) {
-    return *(const double*)((const char*)slice + i * 8u);
+template <class T>
+__device__ __forceinline__ T slot_ld(const T* slice, unsigned i) {
+    return *(const T*)((const char*)slice + i * (unsigned)sizeof(T));
 }
-__device__ __forceinline__ void slot_st(double* slice, unsigned i, double v) { *(double*)((char*)slice + i * 8u) = v; }
+template <class T>
+__device__ __forceinline__ void slot_st(T* slice, unsigned i, T v) {
+    *(T*)((char*)slice + i * (unsigned)sizeof(T)) = v;
+}
 
 // message of the last failed create call (contexts keep their own)
 inline thread_local std::string g_create_error;

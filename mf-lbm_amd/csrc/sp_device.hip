@@ -1,5 +1,8 @@
-// gfx950 implementation of MF-LBM's single-phase time step (D3Q19 MRT, FP64, singlephase_3D/0.src) and its
+// gfx950 implementation of MF-LBM's single-phase time step (D3Q19 MRT, singlephase_3D/0.src) and its
 // C ABI (the mflbm_sp_* entry points of mflbm.h).
+//
+// precision 0 keeps FP64 populations (the reference arithmetic), precision 1 FP32 deviations from the rest
+// equilibrium (sp_kernels.h); the open boundaries and macroscopic fields below run in FP64 in both modes.
 //
 // Storage follows the two-phase path (device_common.h): the 19 population arrays are compact over the
 // interior fluid nodes plus the support nodes the update, the open boundaries and the periodic exchange
@@ -18,179 +21,13 @@
 #include "device_common.h"
 #include "lattice.h"
 #include "mflbm.h"
+#include "sp_kernels.h"
 
 using namespace mflbm_dev;
 
 namespace {
 
-struct SDev {
-    Layout L;
-    // slice q: compact slots [0, nc), then the link slots of direction opp(q) (device_common.h)
-    double* S[19];
-    const int* map;   // padded offset -> compact index
-    const int* cpos;  // compact index -> padded offset
-    int nf;           // interior fluid nodes
-    int nc;           // all compact nodes
-    const unsigned long long* wmask;   // per wave and direction: fluid nodes with a link (device_common.h)
-    const int* wbase;                  // per wave and direction: links before the wave
-    const int8_t* walls;
-    double* w_in;     // (nx+2)(ny+2), reference layout
-    double* cf;       // convective-outlet history: 19 planes (nx+2)(ny+2)
-};
-
-// population accessor by (direction, padded offset); a support node's slot q facing a fluid node is that
-// node's link slot of direction opp(q)
-struct SPop {
-    const SDev* d;
-    __device__ __forceinline__ double& operator()(int q, int o) const {
-        const int c = d->map[o];
-        if (c >= d->nf) {
-            const int x = d->map[o + d->L.off(q)];
-            if (x >= 0 && x < d->nf) return d->S[q][link_slot(d->nc, d->wbase, d->wmask, x, lat::OPP[q])];
-        }
-        return d->S[q][c];
-    }
-};
-__device__ __forceinline__ SPop spop(const SDev& d) { return SPop{&d}; }
-__device__ __forceinline__ int plane_at(const SDev& d, int i, int j) { return i + j * (d.L.nx + 2); }
-
-// ODD = false: kernel_even (Kernel.F90:206-400): read the swapped own slots, write the own slots.
-// ODD = true : kernel_odd  (Kernel.F90:5-200): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
-template <bool ODD>
-__global__ void __launch_bounds__(BLOCK) k_sp_collide(SDev d, SPCollideParams P, int per) {
-    const int m = xcd_block(per) * BLOCK + threadIdx.x;
-    if (m >= d.nf) return;
-    double a[19];
-    unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
-    a[0] = slot_ld(d.S[0], m);
-    if (ODD) {
-        const int n = d.cpos[m];
-        // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
-        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
-        const unsigned lane = (unsigned)m & 63u;
-        const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-        for (int r = 1; r < 19; ++r) {
-            const unsigned long long wm = d.wmask[e0 + r];
-            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
-            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[n + d.L.off(r)];
-        }
-#pragma unroll
-        for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[q], nb[lat::OPP[q]]);   // slot q of the upwind node x - e_q
-    } else {
-#pragma unroll
-        for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[lat::OPP[q]], m);
-    }
-    collide_single(a, P);
-    slot_st(d.S[0], m, a[0]);
-#pragma unroll
-    for (int q = 1; q < 19; ++q) {
-        if (ODD) slot_st(d.S[lat::OPP[q]], nb[q], a[q]);   // slot opp(q) of x + e_q
-        else slot_st(d.S[q], m, a[q]);
-    }
-}
-
-// dense staging <-> compact storage
-__global__ void k_sp_gather(SDev d, const double* __restrict__ dense, long ps) {
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= d.nc) return;
-    const int n = d.cpos[t];
-    for (int q = 0; q < 19; ++q) d.S[q][t] = dense[q * ps + n];
-}
-__global__ void k_sp_scatter(SDev d, double* __restrict__ dense, long ps) {
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= d.nc) return;
-    const int n = d.cpos[t];
-    for (int q = 0; q < 19; ++q) dense[q * ps + n] = d.S[q][t];
-}
-// link slots <-> dense staging (after gather / after scatter)
-template <bool TO_DENSE>
-__global__ void k_sp_links(SDev d, double* __restrict__ dense, long ps) {
-    const int m = blockIdx.x * BLOCK + threadIdx.x;
-    if (m >= d.nf) return;
-    const int n = d.cpos[m];
-    for (int r = 1; r < 19; ++r) {
-        const int i = link_slot(d.nc, d.wbase, d.wmask, m, r);
-        if (i < 0) continue;
-        const long o = n + d.L.off(r);
-        const int q = lat::OPP[r];
-        if (TO_DENSE) dense[q * ps + o] = d.S[q][i];
-        else d.S[q][i] = dense[q * ps + o];
-    }
-}
-
-// ------------------------------------------------------------------------------------------------------
-// periodic wrap: the reference's MPI exchange of one rank with itself (Mpi.F90:73-448)
-// ------------------------------------------------------------------------------------------------------
-__constant__ int SZP[5] = {5, 11, 12, 15, 16};
-__constant__ int SZM[5] = {6, 14, 13, 18, 17};
-__constant__ int SYP[5] = {3, 7, 8, 15, 17};
-__constant__ int SYM[5] = {4, 10, 9, 16, 18};
-
-__device__ __forceinline__ void cp(const SDev& d, int q, int dst, int src) {
-    const SPop f = spop(d);
-    f(q, dst) = f(q, src);
-}
-// after an even step (mpi_pdf_pull_async_pack / _sync_update, Mpi.F90:73-256)
-__global__ void k_sp_pull_z(SDev d) {
-    const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= L.nx * L.ny) return;
-    const int i = t % L.nx + 1, j = t / L.nx + 1;
-    for (int m = 0; m < 5; ++m) {
-        cp(d, SZP[m], L.at(i, j, 0), L.at(i, j, L.nz));
-        cp(d, SZM[m], L.at(i, j, L.nz + 1), L.at(i, j, 1));
-    }
-}
-__global__ void k_sp_pull_y(SDev d) {
-    const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= L.nx * L.nz) return;
-    const int i = t % L.nx + 1, k = t / L.nx + 1;
-    for (int m = 0; m < 5; ++m) {
-        cp(d, SYP[m], L.at(i, 0, k), L.at(i, L.ny, k));
-        cp(d, SYM[m], L.at(i, L.ny + 1, k), L.at(i, 1, k));
-    }
-}
-__global__ void k_sp_pull_edge(SDev d) {
-    const Layout L = d.L;
-    const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
-    if (i > L.nx) return;
-    cp(d, 18, L.at(i, L.ny + 1, L.nz + 1), L.at(i, 1, 1));
-    cp(d, 16, L.at(i, L.ny + 1, 0), L.at(i, 1, L.nz));
-    cp(d, 17, L.at(i, 0, L.nz + 1), L.at(i, L.ny, 1));
-    cp(d, 15, L.at(i, 0, 0), L.at(i, L.ny, L.nz));
-}
-// after an odd step (mpi_pdf_push_async_pack / _sync_update, Mpi.F90:264-448): faces, then x-edges
-__global__ void k_sp_push_z(SDev d) {
-    const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= L.nx * L.ny) return;
-    const int i = t % L.nx + 1, j = t / L.nx + 1;
-    for (int m = 0; m < 5; ++m) {
-        cp(d, SZM[m], L.at(i, j, 1), L.at(i, j, L.nz + 1));
-        cp(d, SZP[m], L.at(i, j, L.nz), L.at(i, j, 0));
-    }
-}
-__global__ void k_sp_push_y(SDev d) {
-    const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= L.nx * L.nz) return;
-    const int i = t % L.nx + 1, k = t / L.nx + 1;
-    for (int m = 0; m < 5; ++m) {
-        cp(d, SYM[m], L.at(i, 1, k), L.at(i, L.ny + 1, k));
-        cp(d, SYP[m], L.at(i, L.ny, k), L.at(i, 0, k));
-    }
-}
-__global__ void k_sp_push_edge(SDev d) {
-    const Layout L = d.L;
-    const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
-    if (i > L.nx) return;
-    cp(d, 15, L.at(i, L.ny, L.nz), L.at(i, 0, 0));
-    cp(d, 17, L.at(i, L.ny, 1), L.at(i, 0, L.nz + 1));
-    cp(d, 16, L.at(i, 1, L.nz), L.at(i, L.ny + 1, 0));
-    cp(d, 18, L.at(i, 1, 1), L.at(i, L.ny + 1, L.nz + 1));
-}
+using namespace mflbm_sp;
 
 // ------------------------------------------------------------------------------------------------------
 // open boundaries (Boundary.F90), one thread per (i, j) of the face; `x * (1 - w) + y * w` blends are kept
@@ -198,7 +35,8 @@ __global__ void k_sp_push_edge(SDev d) {
 // ------------------------------------------------------------------------------------------------------
 
 // inlet_bounce_back_velocity_BC_before_odd (Boundary.F90:5-37)
-__global__ void k_sp_inlet_velocity_before(SDev d, double relaxation) {
+template <class T>
+__global__ void k_sp_inlet_velocity_before(SDev<T> d, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
@@ -207,7 +45,7 @@ __global__ void k_sp_inlet_velocity_before(SDev d, double relaxation) {
     const double om = (double)(1 - wi), wd = (double)wi;
     const double s = d.w_in[plane_at(d, i, j)] * relaxation;
     const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     int o;
     o = L.at(i, j, 0);     A(5, o) = (A(6, L.at(i, j, 1)) + c1 * s) * om + A(5, o) * wd;
     o = L.at(i - 1, j, 0); A(11, o) = (A(14, L.at(i, j, 1)) + c2 * s) * om + A(11, o) * wd;
@@ -216,7 +54,8 @@ __global__ void k_sp_inlet_velocity_before(SDev d, double relaxation) {
     o = L.at(i, j + 1, 0); A(16, o) = (A(17, L.at(i, j, 1)) + c2 * s) * om + A(16, o) * wd;
 }
 // inlet_bounce_back_velocity_BC_after_odd (Boundary.F90:40-70)
-__global__ void k_sp_inlet_velocity_after(SDev d, double relaxation) {
+template <class T>
+__global__ void k_sp_inlet_velocity_after(SDev<T> d, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
@@ -225,7 +64,7 @@ __global__ void k_sp_inlet_velocity_after(SDev d, double relaxation) {
     const double om = (double)(1 - wi), wd = (double)wi;
     const double s = d.w_in[plane_at(d, i, j)] * relaxation;
     const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     const int o = L.at(i, j, 1);
     A(6, o) = (A(5, L.at(i, j, 0)) + c1 * s) * om + A(6, o) * wd;
     A(13, o) = (A(12, L.at(i + 1, j, 0)) + c2 * s) * om + A(13, o) * wd;
@@ -234,14 +73,15 @@ __global__ void k_sp_inlet_velocity_after(SDev d, double relaxation) {
     A(18, o) = (A(15, L.at(i, j - 1, 0)) + c2 * s) * om + A(18, o) * wd;
 }
 // inlet_Zou_He_pressure_BC_before_odd (Boundary.F90:76-132)
-__global__ void k_sp_inlet_pressure_before(SDev d, double rho_in, double relaxation) {
+template <class T>
+__global__ void k_sp_inlet_pressure_before(SDev<T> d, double rho_in, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     const int wi = d.walls[L.at(i, j, 1)];
     const double om = (double)(1 - wi), wd = (double)wi;
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     const double s = (rho_in - (A(0, L.at(i, j, 1)) + A(1, L.at(i - 1, j, 1)) + A(2, L.at(i + 1, j, 1)) +
                                 A(3, L.at(i, j - 1, 1)) + A(4, L.at(i, j + 1, 1)) + A(7, L.at(i - 1, j - 1, 1)) +
                                 A(8, L.at(i + 1, j - 1, 1)) + A(9, L.at(i - 1, j + 1, 1)) + A(10, L.at(i + 1, j + 1, 1)) +
@@ -259,14 +99,15 @@ __global__ void k_sp_inlet_pressure_before(SDev d, double rho_in, double relaxat
     o = L.at(i, j + 1, 0); A(16, o) = (A(17, L.at(i, j - 1, 2)) + lat::ONE6 * s + tny) * om + A(16, o) * wd;
 }
 // inlet_Zou_He_pressure_BC_after_odd (Boundary.F90:135-183)
-__global__ void k_sp_inlet_pressure_after(SDev d, double rho_in, double relaxation) {
+template <class T>
+__global__ void k_sp_inlet_pressure_after(SDev<T> d, double rho_in, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     const int wi = d.walls[L.at(i, j, 1)];
     const double om = (double)(1 - wi), wd = (double)wi;
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     const int o = L.at(i, j, 1);
     const double s = (rho_in - (A(0, o) + A(2, o) + A(1, o) + A(4, o) + A(3, o) + A(8, o) + A(7, o) + A(10, o) + A(9, o) +
                                 2.0 * (A(5, o) + A(11, o) + A(12, o) + A(15, o) + A(16, o)))) * relaxation;
@@ -279,7 +120,8 @@ __global__ void k_sp_inlet_pressure_after(SDev d, double rho_in, double relaxati
     A(18, o) = (A(15, o) + lat::ONE6 * s - tny) * om + A(18, o) * wd;
 }
 // outlet_convective_BC_before_odd (Boundary.F90:192-230)
-__global__ void k_sp_outlet_convective_before(SDev d, double u_convec) {
+template <class T>
+__global__ void k_sp_outlet_convective_before(SDev<T> d, double u_convec) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
@@ -293,7 +135,7 @@ __global__ void k_sp_outlet_convective_before(SDev d, double u_convec) {
     const int QS[5] = {6, 13, 14, 17, 18};
     const int DI[5] = {0, -1, 1, 0, 0};
     const int DJ[5] = {0, 0, 0, -1, 1};
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     for (int m = 0; m < 5; ++m) {
         const int o = L.at(i + DI[m], j + DJ[m], nz + 1);
         const int in = L.at(i + DI[m], j + DJ[m], nz);
@@ -302,7 +144,8 @@ __global__ void k_sp_outlet_convective_before(SDev d, double u_convec) {
     for (int m = 0; m < 5; ++m) d.cf[pc + QS[m] * pl] = A(QS[m], L.at(i + DI[m], j + DJ[m], nz + 1));
 }
 // outlet_convective_BC_after_odd (Boundary.F90:233-267)
-__global__ void k_sp_outlet_convective_after(SDev d, double u_convec) {
+template <class T>
+__global__ void k_sp_outlet_convective_after(SDev<T> d, double u_convec) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
@@ -316,20 +159,21 @@ __global__ void k_sp_outlet_convective_after(SDev d, double u_convec) {
     const int QW[5] = {5, 11, 12, 15, 16};
     const int QH[5] = {6, 14, 13, 18, 17};
     const int o = L.at(i, j, nz), om1 = L.at(i, j, nz - 1);
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     for (int m = 0; m < 5; ++m)
         A(QW[m], o) = ((d.cf[pc + QH[m] * pl] + u_convec * A(QW[m], om1)) * temp) * om + A(QW[m], o) * wd;
     for (int m = 0; m < 5; ++m) d.cf[pc + QH[m] * pl] = A(QW[m], o);
 }
 // outlet_Zou_He_pressure_BC_before_odd (Boundary.F90:276-330)
-__global__ void k_sp_outlet_pressure_before(SDev d, double rho_out) {
+template <class T>
+__global__ void k_sp_outlet_pressure_before(SDev<T> d, double rho_out) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     const int a0 = L.at(i, j, nz), am = L.at(i - 1, j, nz), ap = L.at(i + 1, j, nz), bm = L.at(i, j - 1, nz),
               bp = L.at(i, j + 1, nz), mm = L.at(i - 1, j - 1, nz), pm = L.at(i + 1, j - 1, nz),
               mp = L.at(i - 1, j + 1, nz), pp = L.at(i + 1, j + 1, nz);
@@ -347,14 +191,15 @@ __global__ void k_sp_outlet_pressure_before(SDev d, double rho_out) {
     o = L.at(i, j + 1, nz + 1); A(18, o) = (A(15, c4) - lat::ONE6 * s + tny) * om + A(18, o) * wd;
 }
 // outlet_Zou_He_pressure_BC_after_odd (Boundary.F90:333-389)
-__global__ void k_sp_outlet_pressure_after(SDev d, double rho_out) {
+template <class T>
+__global__ void k_sp_outlet_pressure_after(SDev<T> d, double rho_out) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
-    const SPop A = spop(d);
+    const SPop<T> A = spop(d);
     const int o = L.at(i, j, nz);
     const double s = (A(0, o) + A(2, o) + A(1, o) + A(4, o) + A(3, o) + A(8, o) + A(7, o) + A(10, o) + A(9, o) +
                       2.0 * (A(6, o) + A(14, o) + A(13, o) + A(18, o) + A(17, o))) - rho_out;
@@ -373,12 +218,13 @@ __global__ void k_sp_outlet_pressure_after(SDev d, double rho_out) {
 
 // compute_macro_vars (Misc.F90:368-423): rho = sum f, u = sum e f - F/2 at fluid nodes.  Solid nodes hold 0
 // (the fields are cleared when the geometry is uploaded and nothing else writes them).
-__global__ void k_sp_macro(SDev d, double force_z, double* u, double* v, double* w, double* rho) {
+template <class T>
+__global__ void k_sp_macro(SDev<T> d, double force_z, double* u, double* v, double* w, double* rho) {
     const int m = blockIdx.x * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     double f[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = d.S[q][m];
+    for (int q = 0; q < 19; ++q) f[q] = Store<T>::get(d.S[q][m], q);
     const double fx = 0.0, fy = 0.0, fz = force_z;
     const int o = d.cpos[m];
     rho[o] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] +
@@ -390,9 +236,8 @@ __global__ void k_sp_macro(SDev d, double force_z, double* u, double* v, double*
 
 // monitor (Monitor.F90:20-48), one block per z layer: fl(k) = sum of w, pre(k) = sum of rho over the layer,
 // and the layer's max of u^2+v^2+w^2.  The sums are tree reductions (the reference adds in x-fastest order).
-__global__ void __launch_bounds__(BLOCK) k_sp_monitor_layers(SDev d, const double* u, const double* v, const double* w,
+__global__ void __launch_bounds__(BLOCK) k_sp_monitor_layers(Layout L, const double* u, const double* v, const double* w,
                                                              const double* rho, double* out) {
-    const Layout L = d.L;
     const int k = blockIdx.x + 1;
     __shared__ double s_fl[BLOCK], s_pre[BLOCK], s_um[BLOCK];
     double fl = 0.0, pre = 0.0, um = 0.0;
@@ -429,8 +274,11 @@ bool sp_open_z(const mflbm_sp_params& p) { return p.kper == 0 && p.wall_z_min ==
 struct mflbm_sp_ctx {
     mflbm_sp_params p{};
     SPCollideParams cp{};
+    SPCollideParamsT<float> cp32{};
+    bool fp32 = false;           // precision == 1: FP32 deviation storage
     Layout L{};
-    SDev d{};
+    SDev<double> d{};            // slice pointers of the FP64 mode; the other fields are shared by both views
+    SDev<float> d32{};
     hipStream_t stream = nullptr;
     long ps = 0;                 // stride of the dense arrays
     double* dense_pdf = nullptr; // 19 dense arrays: upload/download staging
@@ -438,7 +286,7 @@ struct mflbm_sp_ctx {
     int8_t* walls = nullptr;
     double* planes = nullptr;    // w_in, cf (19)
     double* red = nullptr;       // monitor layer results (3*nz)
-    double* compact = nullptr;  // 19 population slices (compact slots + link slots)
+    void* compact = nullptr;     // 19 population slices (compact slots + link slots), double or float
     int* map = nullptr;
     int* cpos = nullptr;
     unsigned long long* wmask = nullptr;
@@ -454,10 +302,58 @@ struct mflbm_sp_ctx {
 
 namespace {
 
-int sp_launch_step(mflbm_sp_ctx* c, int ntime) {
+template <class T>
+const SDev<T>& view(const mflbm_sp_ctx* c);
+template <>
+const SDev<double>& view<double>(const mflbm_sp_ctx* c) { return c->d; }
+template <>
+const SDev<float>& view<float>(const mflbm_sp_ctx* c) { return c->d32; }
+template <class T>
+const SPCollideParamsT<T>& rates(const mflbm_sp_ctx* c);
+template <>
+const SPCollideParamsT<double>& rates<double>(const mflbm_sp_ctx* c) { return c->cp; }
+template <>
+const SPCollideParamsT<float>& rates<float>(const mflbm_sp_ctx* c) { return c->cp32; }
+
+template <class T>
+int sp_launch_step_t(mflbm_sp_ctx* c, int ntime, hipEvent_t* ev) {
     const mflbm_sp_params& p = c->p;
     const Layout& L = c->L;
+    const SDev<T>& d = view<T>(c);
     const bool odd = (ntime % 2) != 0;
+    const int per = xcd_per(d.nf);
+    if (odd) LAUNCH(c, (k_sp_collide<true, T>), NXCD * per, d, rates<T>(c), per);
+    else LAUNCH(c, (k_sp_collide<false, T>), NXCD * per, d, rates<T>(c), per);
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    if (p.kper == 1 || p.jper == 1) {
+        if (!odd) {
+            if (p.kper == 1) LAUNCH(c, k_sp_pull_z<T>, nblocks((long)L.nx * L.ny), d);
+            if (p.jper == 1) LAUNCH(c, k_sp_pull_y<T>, nblocks((long)L.nx * L.nz), d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_pull_edge<T>, nblocks(L.nx), d);
+        } else {
+            if (p.kper == 1) LAUNCH(c, k_sp_push_z<T>, nblocks((long)L.nx * L.ny), d);
+            if (p.jper == 1) LAUNCH(c, k_sp_push_y<T>, nblocks((long)L.nx * L.nz), d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_push_edge<T>, nblocks(L.nx), d);
+        }
+    }
+    if (sp_open_z(p)) {
+        const unsigned gface = nblocks((long)L.nx * L.ny);
+        if (!odd) {
+            if (p.inlet_bc == 1) LAUNCH(c, k_sp_inlet_velocity_before<T>, gface, d, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_sp_inlet_pressure_before<T>, gface, d, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_sp_outlet_convective_before<T>, gface, d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_sp_outlet_pressure_before<T>, gface, d, p.rho_out);
+        } else {
+            if (p.inlet_bc == 1) LAUNCH(c, k_sp_inlet_velocity_after<T>, gface, d, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_sp_inlet_pressure_after<T>, gface, d, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_sp_outlet_convective_after<T>, gface, d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_sp_outlet_pressure_after<T>, gface, d, p.rho_out);
+        }
+    }
+    return MFLBM_OK;
+}
+
+int sp_launch_step(mflbm_sp_ctx* c, int ntime) {
     hipEvent_t* ev = nullptr;
     if (c->timing) {
         if (c->ev_used + 3 > c->ev.size()) {
@@ -471,36 +367,32 @@ int sp_launch_step(mflbm_sp_ctx* c, int ntime) {
         c->ev_used += 3;
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
     }
-    const int per = xcd_per(c->d.nf);
-    if (odd) LAUNCH(c, k_sp_collide<true>, NXCD * per, c->d, c->cp, per);
-    else LAUNCH(c, k_sp_collide<false>, NXCD * per, c->d, c->cp, per);
-    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    if (p.kper == 1 || p.jper == 1) {
-        if (!odd) {
-            if (p.kper == 1) LAUNCH(c, k_sp_pull_z, nblocks((long)L.nx * L.ny), c->d);
-            if (p.jper == 1) LAUNCH(c, k_sp_pull_y, nblocks((long)L.nx * L.nz), c->d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_pull_edge, nblocks(L.nx), c->d);
-        } else {
-            if (p.kper == 1) LAUNCH(c, k_sp_push_z, nblocks((long)L.nx * L.ny), c->d);
-            if (p.jper == 1) LAUNCH(c, k_sp_push_y, nblocks((long)L.nx * L.nz), c->d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_push_edge, nblocks(L.nx), c->d);
-        }
-    }
-    if (sp_open_z(p)) {
-        const unsigned gface = nblocks((long)L.nx * L.ny);
-        if (!odd) {
-            if (p.inlet_bc == 1) LAUNCH(c, k_sp_inlet_velocity_before, gface, c->d, p.relaxation);
-            else if (p.inlet_bc == 2) LAUNCH(c, k_sp_inlet_pressure_before, gface, c->d, p.rho_in, p.relaxation);
-            if (p.outlet_bc == 1) LAUNCH(c, k_sp_outlet_convective_before, gface, c->d, p.uin_avg);
-            else if (p.outlet_bc == 2) LAUNCH(c, k_sp_outlet_pressure_before, gface, c->d, p.rho_out);
-        } else {
-            if (p.inlet_bc == 1) LAUNCH(c, k_sp_inlet_velocity_after, gface, c->d, p.relaxation);
-            else if (p.inlet_bc == 2) LAUNCH(c, k_sp_inlet_pressure_after, gface, c->d, p.rho_in, p.relaxation);
-            if (p.outlet_bc == 1) LAUNCH(c, k_sp_outlet_convective_after, gface, c->d, p.uin_avg);
-            else if (p.outlet_bc == 2) LAUNCH(c, k_sp_outlet_pressure_after, gface, c->d, p.rho_out);
-        }
-    }
+    const int rc = c->fp32 ? sp_launch_step_t<float>(c, ntime, ev) : sp_launch_step_t<double>(c, ntime, ev);
+    if (rc) return rc;
     if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    return MFLBM_OK;
+}
+
+template <class T>
+int sp_gather_t(mflbm_sp_ctx* c) {
+    const SDev<T>& d = view<T>(c);
+    LAUNCH(c, k_sp_gather<T>, nblocks(d.nc), d, (const double*)c->dense_pdf, c->ps);
+    LAUNCH(c, (k_sp_links<false, T>), nblocks(d.nf), d, c->dense_pdf, c->ps);
+    return MFLBM_OK;
+}
+template <class T>
+int sp_scatter_t(mflbm_sp_ctx* c) {
+    const SDev<T>& d = view<T>(c);
+    LAUNCH(c, k_sp_scatter<T>, nblocks(d.nc), d, c->dense_pdf, c->ps);
+    LAUNCH(c, (k_sp_links<true, T>), nblocks(d.nf), d, c->dense_pdf, c->ps);
+    return MFLBM_OK;
+}
+template <class T>
+int sp_macro_t(mflbm_sp_ctx* c) {
+    const long ps = c->ps;
+    const SDev<T>& d = view<T>(c);
+    LAUNCH(c, k_sp_macro<T>, nblocks(d.nf), d, c->p.force_z, c->fields, c->fields + ps, c->fields + 2 * ps,
+           c->fields + 3 * ps);
     return MFLBM_OK;
 }
 
@@ -545,12 +437,16 @@ extern "C" int mflbm_sp_create(const mflbm_sp_params* prm, mflbm_sp_ctx** out) {
     if (prm->wall_x_min != 1 || prm->wall_x_max != 1)
         return fail(nullptr, MFLBM_ERR_ARG, "x needs solid walls at both ends (the reference rejects anything else)");
     if (!(prm->la_nu > 0.0)) return fail(nullptr, MFLBM_ERR_ARG, "viscosity must be positive");
+    if (prm->precision != 0 && prm->precision != 1)
+        return fail(nullptr, MFLBM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 deviations), got %d", prm->precision);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MFLBM_ERR_NODEV, "no HIP device visible");
     mflbm_sp_ctx* c = new (std::nothrow) mflbm_sp_ctx();
     if (!c) return fail(nullptr, MFLBM_ERR_NOMEM, "context allocation failed");
     c->p = *prm;
     c->cp = sp_collide_params(prm->la_nu, prm->mrt_preset, prm->force_z);
+    c->cp32 = to_float(c->cp);
+    c->fp32 = prm->precision == 1;
     Layout& L = c->L;
     L = make_layout(prm->nx, prm->ny, prm->nz);
     if (L.total > (1L << 31) - 1) {
@@ -576,11 +472,15 @@ extern "C" int mflbm_sp_create(const mflbm_sp_params* prm, mflbm_sp_ctx** out) {
         hipMemset(c->fields, 0, sizeof(double) * ps * 4) != hipSuccess || hipMemset(c->walls, 1, (size_t)ps) != hipSuccess ||
         hipMemset(c->planes, 0, sizeof(double) * plane * 20) != hipSuccess)
         return bail(fail(c, MFLBM_ERR_HIP, "hipMemset failed"));
-    SDev& d = c->d;
+    SDev<double>& d = c->d;
     d.L = L;
     d.walls = c->walls;
     d.w_in = c->planes;
     d.cf = c->planes + plane;
+    c->d32.L = L;
+    c->d32.walls = c->walls;
+    c->d32.w_in = d.w_in;
+    c->d32.cf = d.cf;
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(c, MFLBM_ERR_HIP, "hipDeviceSynchronize failed"));
     *out = c;
     return MFLBM_OK;
@@ -606,6 +506,7 @@ extern "C" int mflbm_sp_set_scalars(mflbm_sp_ctx* c, double force_z, double rho_
     c->p.uin_avg = uin_avg;
     c->p.relaxation = relaxation;
     c->cp.force_z = force_z;
+    c->cp32.force_z = (float)force_z;
     return MFLBM_OK;
 }
 
@@ -623,21 +524,22 @@ extern "C" int mflbm_sp_upload_geometry(mflbm_sp_ctx* c, const int8_t* walls_h2,
     const int nc = (int)cset.cpos.size();
     LinkSet links;
     build_links(L, cset, links);
+    const long esz = c->fp32 ? (long)sizeof(float) : (long)sizeof(double);
     long soff[20];
     soff[0] = 0;
     for (int q = 0; q < 19; ++q) {
-        const long len = (((long)nc + (q ? links.nlink[lat::OPP[q]] : 0) + 31) / 32) * 32;
-        if (len * 8 >= (1L << 32))
+        const long len = (((long)nc + (q ? links.nlink[lat::OPP[q]] : 0) + 63) / 64) * 64;
+        if (len * esz >= (1L << 32))
             return fail(c, MFLBM_ERR_ARG, "population slice of %ld slots exceeds 32-bit byte offsets", len);
         soff[q + 1] = soff[q] + len;
     }
     sp_free_geometry(c);
-    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * soff[19]));
+    HIPCHK(c, hipMalloc(&c->compact, esz * soff[19]));
     HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
     HIPCHK(c, hipMalloc(&c->cpos, sizeof(int) * ((size_t)nc + 1)));
     HIPCHK(c, hipMalloc(&c->wmask, sizeof(unsigned long long) * links.wmask.size()));
     HIPCHK(c, hipMalloc(&c->wbase, sizeof(int) * links.wbase.size()));
-    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * soff[19]));
+    HIPCHK(c, hipMemset(c->compact, 0, esz * soff[19]));
     HIPCHK(c, hipMemset(c->fields, 0, sizeof(double) * c->ps * 4));
     HIPCHK(c, hipMemcpy(c->map, cset.cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
     if (nc > 0) HIPCHK(c, hipMemcpy(c->cpos, cset.cpos.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
@@ -645,14 +547,18 @@ extern "C" int mflbm_sp_upload_geometry(mflbm_sp_ctx* c, const int8_t* walls_h2,
     HIPCHK(c, hipMemcpy(c->wbase, links.wbase.data(), sizeof(int) * links.wbase.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->walls, wp.data(), (size_t)L.total, hipMemcpyHostToDevice));
     if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
-    SDev& d = c->d;
-    for (int q = 0; q < 19; ++q) d.S[q] = c->compact + soff[q];
-    d.map = c->map;
-    d.cpos = c->cpos;
-    d.nf = cset.nfl;
-    d.nc = nc;
-    d.wmask = c->wmask;
-    d.wbase = c->wbase;
+    SDev<double>& d = c->d;
+    SDev<float>& d32 = c->d32;
+    for (int q = 0; q < 19; ++q) {
+        d.S[q] = c->fp32 ? nullptr : (double*)c->compact + soff[q];
+        d32.S[q] = c->fp32 ? (float*)c->compact + soff[q] : nullptr;
+    }
+    d.map = d32.map = c->map;
+    d.cpos = d32.cpos = c->cpos;
+    d.nf = d32.nf = cset.nfl;
+    d.nc = d32.nc = nc;
+    d.wmask = d32.wmask = c->wmask;
+    d.wbase = d32.wbase = c->wbase;
     HIPCHK(c, hipDeviceSynchronize());
     c->have_geom = true;
     return MFLBM_OK;
@@ -673,8 +579,8 @@ extern "C" int mflbm_sp_upload_state(mflbm_sp_ctx* c, const double* const* f, co
     const size_t plane = (size_t)(L.nx + 2) * (L.ny + 2);
     if (f_convec) HIPCHK(c, hipMemcpy(c->d.cf, f_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
     HIPCHK(c, hipDeviceSynchronize());
-    LAUNCH(c, k_sp_gather, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
-    LAUNCH(c, k_sp_links<false>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
+    const int rc = c->fp32 ? sp_gather_t<float>(c) : sp_gather_t<double>(c);
+    if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     return MFLBM_OK;
@@ -684,8 +590,8 @@ extern "C" int mflbm_sp_download_state(mflbm_sp_ctx* c, double* const* f, double
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     const Layout& L = c->L;
-    LAUNCH(c, k_sp_scatter, nblocks(c->d.nc), c->d, c->dense_pdf, c->ps);
-    LAUNCH(c, k_sp_links<true>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
+    const int rc = c->fp32 ? sp_scatter_t<float>(c) : sp_scatter_t<double>(c);
+    if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
     for (int q = 0; q < 19 && f; ++q) {
@@ -718,10 +624,7 @@ extern "C" int mflbm_sp_run(mflbm_sp_ctx* c, int32_t ntime0, int32_t nsteps) {
 extern "C" int mflbm_sp_compute_macro(mflbm_sp_ctx* c) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
-    const long ps = c->ps;
-    LAUNCH(c, k_sp_macro, nblocks(c->d.nf), c->d, c->p.force_z, c->fields, c->fields + ps, c->fields + 2 * ps,
-           c->fields + 3 * ps);
-    return MFLBM_OK;
+    return c->fp32 ? sp_macro_t<float>(c) : sp_macro_t<double>(c);
 }
 
 extern "C" int mflbm_sp_monitor(mflbm_sp_ctx* c, double* layers, double* umax2) {
@@ -730,7 +633,7 @@ extern "C" int mflbm_sp_monitor(mflbm_sp_ctx* c, double* layers, double* umax2) 
     if (rc) return rc;
     const Layout& L = c->L;
     const long ps = c->ps;
-    LAUNCH(c, k_sp_monitor_layers, (unsigned)L.nz, c->d, c->fields, c->fields + ps, c->fields + 2 * ps, c->fields + 3 * ps,
+    LAUNCH(c, k_sp_monitor_layers, (unsigned)L.nz, L, c->fields, c->fields + ps, c->fields + 2 * ps, c->fields + 3 * ps,
            c->red);
     std::vector<double> h((size_t)L.nz * 3);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));

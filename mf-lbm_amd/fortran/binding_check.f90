@@ -40,7 +40,7 @@ contains
         q%jper = 0; q%kper = 1
         q%wall_x_min = 1; q%wall_x_max = 1; q%wall_y_min = 1; q%wall_y_max = 1
         q%wall_z_min = 0; q%wall_z_max = 0
-        q%inlet_bc = 0; q%outlet_bc = 0; q%mrt_preset = 1; q%la_nu = 0.1d0
+        q%inlet_bc = 0; q%outlet_bc = 0; q%mrt_preset = 1; q%precision = 0; q%la_nu = 0.1d0
         allocate(f(0:nx+1, 0:ny+1, 0:nz+1, 0:18), u(0:nx+1, 0:ny+1, 0:nz+1))
         do i = 0, nx + 1
             u(i, :, :) = 0.01d0 * i

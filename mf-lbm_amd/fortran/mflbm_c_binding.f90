@@ -25,6 +25,7 @@ module mflbm_c_binding
         integer(c_int32_t) :: inlet_bc, outlet_bc
         integer(c_int32_t) :: n_exclude_inlet, n_exclude_outlet
         integer(c_int32_t) :: mrt_preset
+        integer(c_int32_t) :: precision   ! 0 FP64 (reference arithmetic), 1 FP32 deviation storage
         real(c_double) :: la_nu, force_z, rho_in, rho_out, uin_avg, relaxation
     end type mflbm_sp_params
 

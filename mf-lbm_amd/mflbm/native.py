@@ -35,7 +35,14 @@ class SPParams(C.Structure):
     _fields_ = [(n, c_int) for n in (
         "nx", "ny", "nz", "jper", "kper", "wall_x_min", "wall_x_max", "wall_y_min", "wall_y_max",
         "wall_z_min", "wall_z_max", "inlet_bc", "outlet_bc", "n_exclude_inlet", "n_exclude_outlet",
-        "mrt_preset")] + [(n, c_dbl) for n in ("la_nu", "force_z", "rho_in", "rho_out", "uin_avg", "relaxation")]
+        "mrt_preset", "precision")] + [(n, c_dbl) for n in (
+        "la_nu", "force_z", "rho_in", "rho_out", "uin_avg", "relaxation")]
+
+
+class SPBParams(C.Structure):
+    """mflbm_spb_params (batched single-phase runs)"""
+    _fields_ = [(n, c_int) for n in ("precision", "mrt_preset", "ntime_monitor", "tile_rows", "block", "reserved")] + \
+        [(n, c_dbl) for n in ("la_nu", "force_z", "tol")]
 
 
 class MonitorSums(C.Structure):
@@ -104,6 +111,24 @@ _SP_ENTRY_POINTS = {
     "timing_enable": (c_int, [C.c_void_p, c_int]),
 }
 
+# batched single-phase entry points, bound as prefix + "spb_" + name
+_SPB_ENTRY_POINTS = {
+    "create": (c_int, [C.POINTER(SPBParams), C.POINTER(C.c_void_p)]),
+    "destroy": (None, [C.c_void_p]),
+    "last_error": (C.c_char_p, [C.c_void_p]),
+    "add_sample": (c_int, [C.c_void_p, c_int, c_int, c_int, c_int, P_i8]),
+    "commit": (c_int, [C.c_void_p]),
+    "info": (c_int, [C.c_void_p, P_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "sample_fluid": (c_int, [C.c_void_p, c_int, C.POINTER(C.c_int64)]),
+    "upload_state": (c_int, [C.c_void_p, c_int, C.POINTER(P_dbl)]),
+    "download_state": (c_int, [C.c_void_p, c_int, C.POINTER(P_dbl)]),
+    "run": (c_int, [C.c_void_p, c_int, c_int]),
+    "run_steady": (c_int, [C.c_void_p, c_int, P_int]),
+    "results": (c_int, [C.c_void_p, P_int, P_dbl, P_int]),
+    "reset_monitor": (c_int, [C.c_void_p]),
+    "synchronize": (c_int, [C.c_void_p]),
+}
+
 _HOST_ENTRY_POINTS = {
     "mflbm_create_error": (C.c_char_p, []),
     "mflbm_setup_walls": (c_int, [C.POINTER(Params), P_i8, P_i8]),
@@ -120,15 +145,18 @@ _HOST_ENTRY_POINTS = {
 
 # every symbol include/mflbm.h declares
 EXPORTED = (["mflbm_" + n for n in _ENTRY_POINTS] + ["mflbm_sp_" + n for n in _SP_ENTRY_POINTS] +
-            list(_HOST_ENTRY_POINTS))
+            ["mflbm_spb_" + n for n in _SPB_ENTRY_POINTS] + list(_HOST_ENTRY_POINTS))
 
 
-def bind(lib: C.CDLL, prefix: str, names=None, single_phase: bool = False):
-    """Attach restype/argtypes to ``prefix + name`` for the context entry points (two-phase, or with
-    ``single_phase`` the ``prefix + "sp_" + name`` ones); returns a dict keyed by ``name``."""
+def bind(lib: C.CDLL, prefix: str, names=None, single_phase: bool = False, batch: bool = False):
+    """Attach restype/argtypes to ``prefix + name`` for the context entry points (two-phase, with
+    ``single_phase`` the ``prefix + "sp_" + name`` ones, with ``batch`` the ``prefix + "spb_" + name`` ones);
+    returns a dict keyed by ``name``."""
     out = {}
-    table = _SP_ENTRY_POINTS if single_phase else _ENTRY_POINTS
-    if single_phase:
+    table = _SPB_ENTRY_POINTS if batch else (_SP_ENTRY_POINTS if single_phase else _ENTRY_POINTS)
+    if batch:
+        prefix = prefix + "spb_"
+    elif single_phase:
         prefix = prefix + "sp_"
     for name, (res, args) in table.items():
         if names is not None and name not in names:

@@ -47,7 +47,14 @@ def _check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed with code {rc}")
 
 
-def make_sp_params(rp: io.SPRunParams) -> native.SPParams:
+PRECISION = {"fp64": 0, "fp32": 1}
+
+
+def make_sp_params(rp: io.SPRunParams, precision: str = "fp64") -> native.SPParams:
+    """precision "fp64": the reference arithmetic; "fp32": populations stored in FP32 as deviations from the
+    rest equilibrium and collided in FP32 (boundaries, macroscopic fields and reductions stay FP64)."""
+    if precision not in PRECISION:
+        raise ValueError(f"precision must be one of {sorted(PRECISION)}, got {precision!r}")
     p = native.SPParams()
     p.nx, p.ny, p.nz = rp.nx, rp.ny, rp.nz
     p.jper, p.kper = rp.jper, rp.kper
@@ -57,6 +64,7 @@ def make_sp_params(rp: io.SPRunParams) -> native.SPParams:
     p.inlet_bc, p.outlet_bc = rp.inlet_bc, rp.outlet_bc
     p.n_exclude_inlet, p.n_exclude_outlet = rp.n_exclude_inlet, rp.n_exclude_outlet
     p.mrt_preset = rp.mrt_preset
+    p.precision = PRECISION[precision]
     p.la_nu, p.force_z = rp.la_nu, rp.force_z
     p.rho_in, p.rho_out, p.uin_avg = rp.rho_in, rp.rho_out, rp.uin_avg
     p.relaxation = 1.0
@@ -64,14 +72,15 @@ def make_sp_params(rp: io.SPRunParams) -> native.SPParams:
 
 
 class SPCase:
-    def __init__(self, control: Dict[str, object], sample: np.ndarray):
+    def __init__(self, control: Dict[str, object], sample: np.ndarray, precision: str = "fp64"):
         lib = native.load()
         self.control = dict(control)
         self.rp = io.derive_sp_params(control)
+        self.precision = precision
         rp = self.rp
         nx, ny, nz = rp.nx, rp.ny, rp.nz
         self.walls_global = io.global_walls(sample, nx, ny, nz, rp.wall_x, rp.wall_y, rp.wall_z)
-        self.params = make_sp_params(rp)
+        self.params = make_sp_params(rp, precision)
         # set_walls and the inlet profile are the same routines in both programs; they read only the
         # lattice and wall fields of mflbm_params
         wp = native.Params()

@@ -64,6 +64,7 @@ typedef struct oracle_sp_params {
     int32_t inlet_bc, outlet_bc;
     int32_t n_exclude_inlet, n_exclude_outlet;
     int32_t mrt_preset;
+    int32_t precision;   /* ignored: the oracle computes in FP64 whatever the device path stores */
     double la_nu, force_z, rho_in, rho_out, uin_avg, relaxation;
 } oracle_sp_params;
 

@@ -72,10 +72,10 @@ def max_rel_diff(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.abs(a - b).max() / scale)
 
 
-def load_sp(name: str):
+def load_sp(name: str, precision: str = "fp64"):
     """Single-phase fixture and its case."""
     z, ctl = load_fixture(name)
-    return z, SPCase(ctl, z["walls"])
+    return z, SPCase(ctl, z["walls"], precision)
 
 
 def assert_sp_matches(z, case: SPCase, st, n: int) -> None:
