@@ -29,6 +29,12 @@ module mflbm_c_binding
         real(c_double) :: la_nu, force_z, rho_in, rho_out, uin_avg, relaxation
     end type mflbm_sp_params
 
+    ! batched single-phase runs (no reference counterpart; many samples per job)
+    type, bind(c) :: mflbm_spb_params
+        integer(c_int32_t) :: precision, mrt_preset, ntime_monitor, tile_rows, block, reserved
+        real(c_double) :: la_nu, force_z, tol
+    end type mflbm_spb_params
+
     type, bind(c) :: mflbm_monitor_sums
         real(c_double) :: umax2, usq1, usq2, vol1_full, vol2_full, dphi_max, pre_w, pre_nw
         integer(c_int64_t) :: n_w, n_nw, outlet_phase1
@@ -181,6 +187,46 @@ module mflbm_c_binding
             import :: c_int, c_ptr, mflbm_sp_params
             type(mflbm_sp_params), intent(in) :: params
             type(c_ptr), value :: rho0, u0, v0, w0, f_convec
+            type(c_ptr), intent(in) :: f(19)
+        end function
+        ! ---- batched single-phase runs
+        integer(c_int) function mflbm_spb_create(params, ctx) bind(c, name="mflbm_spb_create")
+            import :: c_int, c_ptr, mflbm_spb_params
+            type(mflbm_spb_params), intent(in) :: params
+            type(c_ptr), intent(out) :: ctx
+        end function
+        subroutine mflbm_spb_destroy(ctx) bind(c, name="mflbm_spb_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end subroutine
+        ! walls_h2: the sample's walls with halo 2 (mflbm_setup_walls, kper = 1)
+        integer(c_int) function mflbm_spb_add_sample(ctx, nx, ny, nz, jper, walls_h2) bind(c, name="mflbm_spb_add_sample")
+            import :: c_int, c_ptr, c_int32_t, c_int8_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nx, ny, nz, jper
+            integer(c_int8_t), intent(in) :: walls_h2(*)
+        end function
+        integer(c_int) function mflbm_spb_commit(ctx) bind(c, name="mflbm_spb_commit")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function mflbm_spb_run_steady(ctx, max_periods, nactive) bind(c, name="mflbm_spb_run_steady")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: max_periods
+            integer(c_int32_t), intent(out) :: nactive
+        end function
+        integer(c_int) function mflbm_spb_results(ctx, conv_step, flowrate, periods) bind(c, name="mflbm_spb_results")
+            import :: c_int, c_ptr, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(inout) :: conv_step(*)
+            real(c_double), intent(inout) :: flowrate(*)
+            integer(c_int32_t), intent(out) :: periods
+        end function
+        integer(c_int) function mflbm_spb_download_state(ctx, sample, f) bind(c, name="mflbm_spb_download_state")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: sample
             type(c_ptr), intent(in) :: f(19)
         end function
     end interface
