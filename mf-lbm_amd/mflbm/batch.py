@@ -24,7 +24,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from . import geometry, native
+from . import geometry, io, native
 from .native import i8ptr
 
 PRECISION = {"fp64": 0, "fp32": 1}
@@ -53,6 +53,14 @@ def area_xy(nx: int, ny: int) -> float:
     return la_x * la_y
 
 
+def global_walls(sample: REVSample) -> np.ndarray:
+    """walls_global of the sample's run (set_walls, Misc.F90:18-109): solid x walls at i = 1 and nx, solid y
+    walls at j = 1 and ny unless y is periodic, z periodic."""
+    nx, ny, nz = sample.shape
+    wy = (0, 0) if sample.jper else (1, 1)
+    return io.global_walls(sample.walls, nx, ny, nz, (1, 1), wy, (0, 0))
+
+
 def walls_h2(sample: REVSample) -> np.ndarray:
     """The sample's walls with halo 2 (set_walls through mflbm_setup_walls): x walls, z periodic."""
     lib = native.load()
@@ -63,7 +71,7 @@ def walls_h2(sample: REVSample) -> np.ndarray:
     wp.wall_y_min = wp.wall_y_max = 0 if sample.jper else 1
     wp.wall_z_min = wp.wall_z_max = 0
     out = np.zeros((nz + 4, ny + 4, nx + 4), dtype=np.int8)
-    wg = np.ascontiguousarray(sample.walls, dtype=np.int8)
+    wg = np.ascontiguousarray(global_walls(sample), dtype=np.int8)
     rc = lib.mflbm_setup_walls(C.byref(wp), i8ptr(wg), i8ptr(out))
     if rc != 0:
         raise RuntimeError(f"mflbm_setup_walls failed with code {rc}")

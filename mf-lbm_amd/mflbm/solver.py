@@ -90,7 +90,10 @@ class Simulation:
     def _load(self):
         return native.load()
 
-    def __init__(self, case: Case, state: Optional[State] = None, gradient: bool = True):
+    def __init__(self, case: Case, state: Optional[State] = None, gradient: bool = True,
+                 device_init: bool = False):
+        """``state`` is uploaded when given; otherwise the new-run initial state is built on the host and
+        uploaded, or with ``device_init`` computed on the device (mflbm_init_state_device, at rest)."""
         self.case = case
         self.lib = self._load()
         self.fn = native.bind(self.lib, self.prefix)
@@ -105,7 +108,10 @@ class Simulation:
         self._call("upload_geometry", i8ptr(case.walls_h2), len(b.s_nfl), iptr(b.s_ijk), iptr(b.s_nfl),
                    iptr(b.s_nbr), dptr(b.s_w), len(b.f_theta), iptr(b.f_ijk), dptr(b.f_nw), dptr(b.f_theta),
                    dptr(case.w_in))
-        self.upload(state if state is not None else case.initial_state())
+        if state is not None or not device_init:
+            self.upload(state if state is not None else case.initial_state())
+        else:
+            self._call("init_state_device", case.rp.init_option, case.rp.interface_z0)
         if gradient:
             self.color_gradient()
 
