@@ -18,8 +18,9 @@ cores for a bounded number of steps of the same workload.
 
 ``--workload singlephase`` measures the single-phase program's step instead (D3Q19 MRT, one population set,
 bounce-back-optimised rates, nu = 0.1) on the same 256^3 pack, z-periodic and driven by a body force of 1e-6
-(a permeability run); 305 B per fluid node.  ``--precision fp32`` stores its populations in FP32 as deviations
-from the rest equilibrium and collides in FP32 (153 B per fluid node).
+(a permeability run); 305 B per fluid node.  ``--precision fp32`` stores the populations in FP32 (single phase:
+deviations from the rest equilibrium, 153 B per fluid node; two phase: f + g - REST and f - g, 321 B) and
+collides in FP32; ``--precision mixed`` keeps the FP32 storage and collides in FP64.
 """
 from __future__ import annotations
 
@@ -35,12 +36,13 @@ sys.path.insert(0, os.path.join(REPO, "mf-lbm_amd"))
 # the headline metric as the benchmark definition names it; MLUPS of the two-phase step
 METRIC = "MLUPS (D3Q19+D3Q7, FP64) at 256^3 porous; time-to-converged permeability"
 B_ALG = 625.0          # bytes per fluid node per step, two-phase FP64
+B_ALG32 = 321.0        # two-phase FP32 storage: 2 x 19 x 4 B x (read + write) + phi 8 + 8 + wall mask 1
 B_ALG_SP = 305.0       # bytes per fluid node per step, single-phase FP64
 B_ALG_SP32 = 153.0     # bytes per fluid node per step, single-phase FP32 deviation storage
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def workload(n: int = 256):
+def workload(n: int = 256, precision: str = "fp64"):
     from mflbm import Case, geometry
     walls = geometry.sphere_pack(n, n, n, 0.22, 6.0, 12.0, 2024, buffer=8)
     ctl = {"lattice_dimensions": (n, n, n), "periodic_indicator": (0, 0, 0), "domain_wall_status_z": (0, 0),
@@ -48,7 +50,7 @@ def workload(n: int = 256):
            "fluid2_viscosity": 0.05, "surface_tension": 0.05, "theta": 150.0, "capillary_number": 1e-4,
            "initial_fluid_distribution_option": 1, "initial_interface_position": 8.0,
            "saturation_injection": 1.0, "excluded_layers": (8, 8)}
-    return Case(ctl, walls)
+    return Case(ctl, walls, precision=precision)
 
 
 def workload_sp(n: int = 256, precision: str = "fp64"):
@@ -127,7 +129,7 @@ def roofline(b_alg, n_fluid, kernel_ms, traffic, kernel, **extra):
 def run_twophase(args) -> dict:
     from mflbm import Simulation
     t0 = time.time()
-    case = workload(args.size)
+    case = workload(args.size, args.precision)
     st = case.initial_state()
     print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
     sim = Simulation(case, state=st)
@@ -135,19 +137,23 @@ def run_twophase(args) -> dict:
     mon = sim.monitor()
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
+    fp32 = args.precision != "fp64"
     return {
-        "metric": METRIC,
+        "metric": METRIC if not fp32 else f"MLUPS (two-phase D3Q19 colour model, {args.precision} storage) at 256^3 porous",
         "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32" if fp32 else "f64", "data": "synthetic",
         "config": {"workload": f"two-phase drainage, {args.size}^3 seeded sphere pack porosity 0.22, 8-layer "
-                               "buffers, velocity inlet 1e-4 + convective outlet, 150 deg, FP64; the reference's "
+                               "buffers, velocity inlet 1e-4 + convective outlet, 150 deg, "
+                               + ({"fp32": "FP32 population storage", "mixed": "FP32 storage, FP64 collision"}
+                                  .get(args.precision, "FP64")) + "; the reference's "
                                "colour model (two D3Q19 population sets: the reference has no D3Q7 lattice)",
                    "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3},
-        "roofline": roofline(B_ALG, case.n_fluid, means[0], traffic_from_profiles(f"sphere_pack_{args.size}"),
+        "roofline": roofline(B_ALG32 if fp32 else B_ALG, case.n_fluid, means[0],
+                             traffic_from_profiles(f"{'fp32_' if fp32 else ''}sphere_pack_{args.size}"),
                              "k_collide (even/odd mean)", gradient_chain_ms=round(means[1], 4),
                              step_ms_events=round(means[2], 4)),
-        "cpu_baseline": cpu_leg(args, case, st),
+        "cpu_baseline": cpu_leg(args, workload(args.size) if fp32 else case, st),
         "check": {"saturation": mon.saturation, "umax": mon.umax, "failed": mon.failed},
     }
 
@@ -163,8 +169,8 @@ def run_singlephase(args) -> dict:
     mon = sim.monitor()
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
-    fp32 = args.precision == "fp32"
-    tag = "FP32 deviation storage" if fp32 else "FP64"
+    fp32 = args.precision != "fp64"
+    tag = {"fp64": "FP64", "fp32": "FP32 deviation storage", "mixed": "FP32 deviation storage, FP64 collision"}[args.precision]
     return {
         "metric": f"MLUPS (single-phase D3Q19 MRT, {tag}) at 256^3 porous",
         "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -192,10 +198,11 @@ def run_batch(args) -> dict:
     t0 = time.time()
     samples = batch.rev_samples(args.batch_samples, seed=2024)
     print(f"[bench] {len(samples)} samples generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    tune = batch.autotune(samples, "fp32")
+    tune = batch.autotune(samples, args.precision if args.precision != "fp64" else "mixed")
     print(f"[bench] autotune: tile_rows {tune['tile_rows']} block {tune['block']}", file=sys.stderr, flush=True)
     kw = dict(la_nu=0.1, force_z=1e-6, ntime_monitor=500, tol=1e-7, tile_rows=tune["tile_rows"], block=tune["block"])
-    b = batch.REVBatch(samples, precision="fp32", **kw)
+    p32 = args.precision if args.precision != "fp64" else "mixed"
+    b = batch.REVBatch(samples, precision=p32, **kw)
     b.run(1, args.warmup)
     b.synchronize()
     t1 = time.perf_counter()
@@ -206,17 +213,17 @@ def run_batch(args) -> dict:
     b.close()
     mlups = nodes * args.steps / dt / 1e6
     conv = {}
-    for prec in ("fp32", "fp64"):
+    for prec in (p32, "fp64"):
         bb = batch.REVBatch(samples, precision=prec, **kw)
         r = bb.run_steady(args.batch_max_steps)
         bb.close()
         conv[prec] = r
         print(f"[bench] {prec}: steady in {r.seconds:.2f}s, {int((r.conv_step > 0).sum())}/{len(samples)} converged, "
               f"max step {int(r.conv_step.max())}", file=sys.stderr, flush=True)
-    rel = np.abs(conv["fp32"].permeability / conv["fp64"].permeability - 1.0)
+    rel = np.abs(conv[p32].permeability / conv["fp64"].permeability - 1.0)
     ms = dt / args.steps * 1e3
     return {
-        "metric": "MLUPS (batched single-phase REVs, D3Q19 MRT, FP32 deviation storage); time-to-converged permeability",
+        "metric": f"MLUPS (batched single-phase REVs, D3Q19 MRT, {p32} FP32 deviation storage); time-to-converged permeability",
         "value": round(mlups, 3), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
@@ -228,10 +235,61 @@ def run_batch(args) -> dict:
         "time_to_converged": {
             p: {"seconds": round(conv[p].seconds, 3), "converged": int((conv[p].conv_step > 0).sum()),
                 "max_step": int(conv[p].conv_step.max())} for p in conv},
-        "permeability_error_fp32_vs_fp64": {"max": float(rel.max()), "median": float(np.median(rel))},
+        "permeability_error_vs_fp64": {"precision": p32, "max": float(rel.max()), "median": float(np.median(rel))},
         "autotune_timings": tune["timings"],
         "cpu_baseline": None,
     }
+
+
+def run_fp32_report(args) -> dict:
+    """FP32 storage against FP64 on the 256^3 drainage (saturation, capillary number and the largest velocity at
+    the report steps) and on a static drop (spurious-current magnitude: the largest velocity of a drop at rest,
+    64^3, radius 16, 90 degrees, after args.drop_steps steps)."""
+    from mflbm import Case, Simulation
+    report = [int(x) for x in args.report_steps.split(",")]
+    out = {"drainage": {}, "drop": {}, "ms_per_step": {}}
+    precs = ("fp64", "fp32", "mixed")
+    for prec in precs:
+        case = workload(args.size, prec)
+        sim = Simulation(case, device_init=True)
+        t, rows = 0, []
+        t0 = time.perf_counter()
+        for n in report:
+            sim.run(t + 1, n - t)
+            t = n
+            m = sim.monitor()
+            rows.append({"step": n, "saturation": m.saturation, "ca": m.ca, "umax": m.umax})
+            print(f"[bench] {prec} step {n}: sat {m.saturation:.8f} ca {m.ca:.6e} umax {m.umax:.6e}",
+                  file=sys.stderr, flush=True)
+        out["ms_per_step"][prec] = (time.perf_counter() - t0) / report[-1] * 1e3
+        out["drainage"][prec] = rows
+        sim.close()
+        ctl = {"lattice_dimensions": (64, 64, 64), "periodic_indicator": (0, 1, 1), "domain_wall_status_y": (0, 0),
+               "domain_wall_status_z": (0, 0), "MPI_async_layers_num": (0, 4, 4), "fluid1_viscosity": 1.0 / 6.0,
+               "fluid2_viscosity": 1.0 / 6.0, "surface_tension": 0.01, "theta": 90.0, "body_force_0": 0.0,
+               "initial_fluid_distribution_option": 5, "initial_interface_position": 16.0}
+        import numpy as np
+        drop = Case(ctl, np.zeros((64, 64, 64), np.int8), precision=prec)
+        sim = Simulation(drop, device_init=True)
+        sim.run(1, args.drop_steps)
+        m = sim.monitor()
+        out["drop"][prec] = {"steps": args.drop_steps, "umax": m.umax, "saturation": m.saturation}
+        sim.close()
+    err = {}
+    for prec in precs[1:]:
+        err[prec] = []
+        for a, b in zip(out["drainage"][prec], out["drainage"]["fp64"]):
+            err[prec].append({"step": a["step"], "saturation_abs": abs(a["saturation"] - b["saturation"]),
+                              "ca_rel": abs(a["ca"] / b["ca"] - 1.0) if b["ca"] else None,
+                              "umax_rel": abs(a["umax"] / b["umax"] - 1.0) if b["umax"] else None})
+    d64 = out["drop"]["fp64"]["umax"]
+    spur = {"fp64": d64}
+    for prec in precs[1:]:
+        d = out["drop"][prec]["umax"]
+        spur[prec] = {"umax": d, "rel": abs(d / d64 - 1.0) if d64 else None}
+    return {"metric": "FP32 storage vs FP64: two-phase 256^3 drainage and static drop", "unit": "report",
+            "value": None, "n_gpus": 1, "data": "synthetic", "vs_fp64": err, "spurious_current": spur,
+            "detail": out}
 
 
 def main() -> None:
@@ -242,20 +300,20 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch"), default="twophase")
+    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report"), default="twophase")
     ap.add_argument("--batch-samples", type=int, default=96)
+    ap.add_argument("--report-steps", default="5000,10000,20000")
+    ap.add_argument("--drop-steps", type=int, default=4000)
     ap.add_argument("--batch-max-steps", type=int, default=100000)
-    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
-                    help="single-phase storage: fp64 (reference arithmetic) or fp32 deviations")
+    ap.add_argument("--precision", choices=("fp64", "fp32", "mixed"), default="fp64",
+                    help="population storage: fp64 (reference arithmetic) or fp32")
     args = ap.parse_args()
-    if args.precision == "fp32" and args.workload == "twophase":
-        print("--precision fp32 applies to the single-phase workload", file=sys.stderr)
-        sys.exit(2)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
               "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    run = {"singlephase": run_singlephase, "batch": run_batch}.get(args.workload, run_twophase)
+    run = {"singlephase": run_singlephase, "batch": run_batch, "fp32report": run_fp32_report}.get(args.workload,
+                                                                                                 run_twophase)
     out = run(args)
     print(json.dumps(out))
 

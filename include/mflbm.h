@@ -48,6 +48,11 @@ typedef struct mflbm_params {
     int32_t porous_plate_cmd;           /* 0 off, 1 block fluid 1, 2 block fluid 2 */
     int32_t z_porous_plate;             /* plate position (1-based z) */
     int32_t n_exclude_inlet, n_exclude_outlet;
+    int32_t precision;                  /* 0: FP64 populations, the reference arithmetic (bit-identical);
+                                         * 1: FP32 storage of f + g - REST and f - g with FP32 collision and
+                                         *    recolouring; 2: the same storage collided in FP64.  phi, the
+                                         *    gradient chain, boundaries and monitors stay FP64 (1 and 2 have
+                                         *    no reference counterpart) */
     double la_nui1, la_nui2;            /* 1/nu of fluid 1 and fluid 2 */
     double gamma;                       /* surface tension */
     double beta;                        /* recolouring parameter */
@@ -107,6 +112,11 @@ int mflbm_upload_state(mflbm_ctx* ctx, const double* const* f, const double* con
                        const double* f_convec, const double* g_convec, const double* phi_convec);
 int mflbm_download_state(mflbm_ctx* ctx, double* const* f, double* const* g, double* phi,
                          double* f_convec, double* g_convec, double* phi_convec);
+/* The initial state of a new run at rest computed on the device instead of uploaded
+ * (initialization_new_multi / _pdf, Init_multiphase.F90:243-470; equals mflbm_init_fields with
+ * u = v = w = NULL followed by mflbm_upload_state): phi by initial_fluid_distribution_option 1..5, equilibrium
+ * populations at rho = 1, convective-outlet history when outlet_bc == 1. */
+int mflbm_init_state_device(mflbm_ctx* ctx, int32_t option, double interface_z0);
 
 /* ---------------------------------------------------------------- hot path ------------------------- */
 
@@ -186,8 +196,9 @@ typedef struct mflbm_sp_params {
     int32_t n_exclude_inlet, n_exclude_outlet;
     int32_t mrt_preset;        /* MRT_collision_parameter_preset: 1 bounce-back optimised, else SRT */
     int32_t precision;         /* 0: FP64 populations, the reference arithmetic (bit-identical);
-                                * 1: FP32 deviations from the rest equilibrium, FP32 collision, FP64 boundaries,
-                                *    macroscopic fields and reductions (no reference counterpart) */
+                                * 1: FP32 deviations from the rest equilibrium, FP32 collision; 2: the same
+                                *    storage collided in FP64.  Boundaries, macroscopic fields and reductions
+                                *    stay FP64 (1 and 2 have no reference counterpart) */
     double la_nu;              /* fluid_viscosity */
     double force_z;            /* body force along z (force_z0) */
     double rho_in, rho_out;    /* Zou-He densities (rho_out = 1) */
@@ -239,7 +250,7 @@ int mflbm_sp_init_fields(const mflbm_sp_params* params, const double* rho0, cons
  * sample) until every sample has converged; a converged sample stops and keeps its state.
  * --------------------------------------------------------------------------------------------------------- */
 typedef struct mflbm_spb_params {
-    int32_t precision;         /* 0 FP64 populations (the reference arithmetic), 1 FP32 deviations */
+    int32_t precision;         /* as mflbm_sp_params.precision: 0 FP64, 1 FP32, 2 FP32 storage with FP64 collision */
     int32_t mrt_preset;        /* as mflbm_sp_params */
     int32_t ntime_monitor;     /* steps per convergence test (even: the flow rate is read after even steps) */
     int32_t tile_rows;         /* compact node order: padded y rows per tile (0: MFLBM_TILE_Y or 32) */

@@ -184,3 +184,64 @@ MFLBM_HD inline double collide_node_tf(double* __restrict__ a, double* __restric
     b[18] = ft18 - a[18];
     return phi;
 }
+
+// ------------------------------------------------------------------------------------------------------
+// FP32 storage of the two-colour state: per direction q the total deviation s_q = f_q + g_q - REST_q from the
+// MRT model's rest equilibrium (collide_sp.h) and the colour difference a_q = f_q - g_q.  Streaming, bounce-back
+// and the periodic exchange move f_q and g_q together, hence s_q and a_q too.  The collision acts on the total
+// population only (its equilibria are linear in the density and u is the momentum itself), so it runs on s_q
+// unchanged (mrt_relax); the recolouring f' = (rho1/rho) ft' + W tmp (e.cn), g' = ft' - f' becomes
+// a' = f' - g' = phi ft' + 2 W tmp (e.cn) with ft' = REST + s', phi = (rho1 - rho2)/rho, tmp = rho1 rho2 beta/rho.
+// rho = 1 + sum s, rho1 - rho2 = sum a.  No reference counterpart (the FP64 path above is the reference
+// arithmetic and the oracle for this one).
+// ------------------------------------------------------------------------------------------------------
+#include "collide_sp.h"
+
+template <class C>
+struct CollideParamsDev {
+    C gamma, beta, force_z, la_nui1, la_nui2;
+};
+
+// in: s[q], a[q] of the incoming populations; cn unit normal, tf = 0.5*gamma*curv*c_norm.  out: s[q], a[q] of
+// the outgoing recoloured populations; returns phi.  C is the arithmetic: float for the FP32 mode, double for
+// the mixed mode (FP32 storage, FP64 collision; the kernel rounds s and a to FP32 once on store).
+template <class C>
+MFLBM_HD inline C collide_node_dev(C* __restrict__ s, C* __restrict__ a, C cnx, C cny, C cnz, C tf,
+                                   const CollideParamsDev<C>& P) {
+    C dr = s[0], dl = a[0];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        dr += s[q];
+        dl += a[q];
+    }
+    const C den = C(1.0) + dr;
+    const C rho1 = C(0.5) * (den + dl), rho2 = C(0.5) * (den - dl);
+    const C phi = dl / den;
+    const C fx = tf * cnx, fy = tf * cny, fz = tf * cnz + P.force_z;
+    const C omega = C(1.0) / (C(6.0) / ((C(1.0) + phi) * P.la_nui1 + (C(1.0) - phi) * P.la_nui2) + C(0.5));
+    const SPCollideParamsT<C> R{C(lat::S_E), C(lat::S_E2), C(lat::S_Q), omega, C(lat::S_PI), C(lat::S_T), C(0.0)};
+    mrt_relax<C>(s, R, fx, fy, fz);
+    const C t2 = C(2.0) * (rho1 * rho2 * P.beta / den);
+    const C k1 = C(lat::W1) * t2, k2 = C(lat::RK_W2) * t2;
+    const C r0 = C(REST0), r1 = C(REST1), r2 = C(REST2);
+    a[0] = phi * (r0 + s[0]);
+    a[1] = phi * (r1 + s[1]) + k1 * cnx;
+    a[2] = phi * (r1 + s[2]) - k1 * cnx;
+    a[3] = phi * (r1 + s[3]) + k1 * cny;
+    a[4] = phi * (r1 + s[4]) - k1 * cny;
+    a[5] = phi * (r1 + s[5]) + k1 * cnz;
+    a[6] = phi * (r1 + s[6]) - k1 * cnz;
+    a[7] = phi * (r2 + s[7]) + k2 * (cnx + cny);
+    a[8] = phi * (r2 + s[8]) + k2 * (-cnx + cny);
+    a[9] = phi * (r2 + s[9]) + k2 * (cnx - cny);
+    a[10] = phi * (r2 + s[10]) + k2 * (-cnx - cny);
+    a[11] = phi * (r2 + s[11]) + k2 * (cnx + cnz);
+    a[12] = phi * (r2 + s[12]) + k2 * (-cnx + cnz);
+    a[13] = phi * (r2 + s[13]) + k2 * (cnx - cnz);
+    a[14] = phi * (r2 + s[14]) + k2 * (-cnx - cnz);
+    a[15] = phi * (r2 + s[15]) + k2 * (cny + cnz);
+    a[16] = phi * (r2 + s[16]) + k2 * (-cny + cnz);
+    a[17] = phi * (r2 + s[17]) + k2 * (cny - cnz);
+    a[18] = phi * (r2 + s[18]) + k2 * (-cny - cnz);
+    return phi;
+}

@@ -54,12 +54,13 @@ constexpr double REST1 = lat::MRT_C1 + 121.0 * lat::MRT_C2 - 4.0 * lat::MRT_C3 *
 constexpr double REST2 = lat::MRT_C1 - 88.0 * lat::MRT_C2 + lat::MRT_C3 * lat::MRT_E2_C1;
 MFLBM_HD constexpr double rest_weight(int q) { return q == 0 ? REST0 : (q < 7 ? REST1 : REST2); }
 
+// The MRT collision with a body force (fx, fy, fz), shared by the single-phase kernels (force (0, 0, force_z))
+// and the FP32 two-phase kernel (the CSF force; P.force_z is not read here).
 // in:  a[q] = incoming populations (q = 0..18); out: a[q] = outgoing post-collision populations.
 // T = double is the reference arithmetic; T = float the FP32 variant (every constant rounded to float).
 template <class T>
-MFLBM_HD inline void collide_single(T* __restrict__ a, const SPCollideParamsT<T>& P) {
+MFLBM_HD inline void mrt_relax(T* __restrict__ a, const SPCollideParamsT<T>& P, const T fx, const T fy, const T fz) {
     using namespace lat;
-    const T fx = T(0.0), fy = T(0.0), fz = P.force_z;
     const T s_e = P.s_e, s_e2 = P.s_e2, s_q = P.s_q, s_nu = P.s_nu, s_pi = P.s_pi, s_t = P.s_t;
     T ft0 = a[0], ft1 = a[1], ft2 = a[2], ft3 = a[3], ft4 = a[4], ft5 = a[5], ft6 = a[6], ft7 = a[7];
     T ft8 = a[8], ft9 = a[9], ft10 = a[10], ft11 = a[11], ft12 = a[12], ft13 = a[13], ft14 = a[14];
@@ -170,4 +171,9 @@ MFLBM_HD inline void collide_single(T* __restrict__ a, const SPCollideParamsT<T>
     a[16] = sum4 - sum6 + sum7 - sum8 * T(2.0) - m_pyz - m_ty - m_tz;
     a[17] = sum4 + sum6 - sum7 - sum8 * T(2.0) - m_pyz + m_ty + m_tz;
     a[18] = sum4 - sum6 - sum7 - sum8 * T(2.0) + m_pyz - m_ty + m_tz;
+}
+
+template <class T>
+MFLBM_HD inline void collide_single(T* __restrict__ a, const SPCollideParamsT<T>& P) {
+    mrt_relax<T>(a, P, T(0.0), T(0.0), P.force_z);
 }

@@ -66,22 +66,61 @@ struct Dev {
 
 __device__ __forceinline__ int link_index(const Dev& d, int x, int r) { return link_slot(d.nc, d.wbase, d.wmask, x, r); }
 
-// population accessor by (direction, padded offset) through the compact map.  Slot q of a support node o
+// index of slot (direction q, padded offset o) in slice q through the compact map.  Slot q of a support node o
 // faces the node o + e_q; when that node is fluid the slot is its link slot of direction opp(q).
-struct Pop {
+__device__ __forceinline__ int slot_index(const Dev& d, int q, int o) {
+    const int c = d.map[o];
+    if (c >= d.nf) {
+        const int x = d.map[o + d.L.off(q)];
+        if (x >= 0 && x < d.nf) return link_index(d, x, lat::OPP[q]);
+    }
+    return c;
+}
+
+// Population accessors by (direction, padded offset) for the boundary, exchange and staging code.
+// T = double: the slices hold f (slices 0..18) and g (19..37), the reference arithmetic.
+// T = float: the slices hold the total deviation s = f + g - REST (0..18) and the colour difference a = f - g
+// (19..37) (collide.h); a colour's population is read and written in FP64 through CRef, which rounds s and a
+// once per write and leaves the other colour unchanged.
+template <class T>
+struct Pop;
+template <>
+struct Pop<double> {
     const Dev* d;
     int c0;   // 0 fluid 1, 19 fluid 2
-    __device__ __forceinline__ double& operator()(int q, int o) const {
-        const int c = d->map[o];
-        if (c >= d->nf) {
-            const int x = d->map[o + d->L.off(q)];
-            if (x >= 0 && x < d->nf) return d->S.p[c0 + q][link_index(*d, x, lat::OPP[q])];
-        }
-        return d->S.p[c0 + q][c];
+    __device__ __forceinline__ double& operator()(int q, int o) const { return d->S.p[c0 + q][slot_index(*d, q, o)]; }
+};
+struct CRef {
+    float* s;
+    float* a;
+    int q;
+    int g;   // 0 fluid 1, 1 fluid 2
+    __device__ __forceinline__ operator double() const {
+        const double F = (double)*s + rest_weight(q), A = (double)*a;
+        return g ? 0.5 * (F - A) : 0.5 * (F + A);
+    }
+    __device__ __forceinline__ CRef& operator=(double v) {
+        const double F = (double)*s + rest_weight(q), A = (double)*a;
+        const double other = g ? 0.5 * (F + A) : 0.5 * (F - A);
+        *s = (float)(v + other - rest_weight(q));
+        *a = (float)(g ? other - v : v - other);
+        return *this;
+    }
+    __device__ __forceinline__ CRef& operator=(const CRef& o) { return *this = (double)o; }
+};
+template <>
+struct Pop<float> {
+    const Dev* d;
+    int c0;
+    __device__ __forceinline__ CRef operator()(int q, int o) const {
+        const int i = slot_index(*d, q, o);
+        return CRef{(float*)d->S.p[q] + i, (float*)d->S.p[19 + q] + i, q, c0 != 0};
     }
 };
-__device__ __forceinline__ Pop fpop(const Dev& d) { return Pop{&d, 0}; }
-__device__ __forceinline__ Pop gpop(const Dev& d) { return Pop{&d, 19}; }
+template <class T>
+__device__ __forceinline__ Pop<T> fpop(const Dev& d) { return Pop<T>{&d, 0}; }
+template <class T>
+__device__ __forceinline__ Pop<T> gpop(const Dev& d) { return Pop<T>{&d, 19}; }
 __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
 
@@ -94,6 +133,10 @@ __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i +
 #ifndef MFLBM_COLLIDE_WAVES
 #define MFLBM_COLLIDE_WAVES 2
 #endif
+#ifndef MFLBM_COLLIDE_WAVES_EVEN
+#define MFLBM_COLLIDE_WAVES_EVEN MFLBM_COLLIDE_WAVES
+#endif
+
 #ifndef MFLBM_COLLIDE_NT
 #define MFLBM_COLLIDE_NT 0
 #endif
@@ -129,7 +172,8 @@ __device__ __forceinline__ double curvature_at(const Dev& d, int n);
 // The CSF force input is evaluated here from the normal field the colour-gradient chain left (curvature,
 // Phase_gradient.F90:112-201, and 0.5*gamma*curv*c_norm, Kernel_multiphase.F90:116).
 template <bool ODD>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MFLBM_COLLIDE_WAVES, 8)))
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(ODD ? MFLBM_COLLIDE_WAVES : MFLBM_COLLIDE_WAVES_EVEN, 8)))
 k_collide(Dev d, CollideParams P, int per) {
     const SlotTable& S = d.S;
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
@@ -184,21 +228,97 @@ k_collide(Dev d, CollideParams P, int per) {
     }
 }
 
+// FP32 storage (s = f + g - REST, a = f - g, collide.h): the same update, the collision and recolouring in C
+// (float, or double for the mixed mode).  The normal, curvature and phi stay FP64 (the colour-gradient chain is
+// shared with the FP64 path).
+template <bool ODD, class C>
+__global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> P, double gamma, int per) {
+    const SlotTable& S = d.S;
+    const int m = xcd_block(per) * BLOCK + threadIdx.x;
+    if (m >= d.nf) return;
+    const int nn = d.cpos[m];
+    const C cnx = (C)d.cnx[nn], cny = (C)d.cny[nn], cnz = (C)d.cnz[nn];
+    const C tf = (C)(0.5 * gamma * curvature_at(d, nn) * d.cnorm[nn]);
+    __builtin_amdgcn_sched_barrier(0);
+    C s[19], a[19];
+    unsigned nb[19];
+    s[0] = slot_ld((const float*)S.p[0], (unsigned)m);
+    a[0] = slot_ld((const float*)S.p[19], (unsigned)m);
+    if (ODD) {
+        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
+        const unsigned lane = (unsigned)m & 63u;
+        const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int r = 1; r < 19; ++r) {
+            const unsigned long long wm = d.wmask[e0 + r];
+            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
+            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[nn + d.L.off(r)];
+        }
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            s[q] = slot_ld((const float*)S.p[q], nb[lat::OPP[q]]);
+            a[q] = slot_ld((const float*)S.p[19 + q], nb[lat::OPP[q]]);
+        }
+    } else {
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            s[q] = slot_ld((const float*)S.p[lat::OPP[q]], (unsigned)m);
+            a[q] = slot_ld((const float*)S.p[19 + lat::OPP[q]], (unsigned)m);
+        }
+    }
+    const C phi = collide_node_dev<C>(s, a, cnx, cny, cnz, tf, P);
+    d.phi[nn] = (double)phi;
+    slot_st((float*)S.p[0], (unsigned)m, (float)s[0]);
+    slot_st((float*)S.p[19], (unsigned)m, (float)a[0]);
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        const int qs = ODD ? lat::OPP[q] : q;
+        const unsigned ms = ODD ? nb[q] : (unsigned)m;
+        slot_st((float*)S.p[qs], ms, (float)s[q]);
+        slot_st((float*)S.p[19 + qs], ms, (float)a[q]);
+    }
+}
+
+// slot i of direction q <-> (f, g) of the dense staging arrays at padded offset o
+template <class T>
+__device__ __forceinline__ void slot_from_dense(const Dev& d, int q, int i, const double* dense, long ps, long o) {
+    if (sizeof(T) == 8) {
+        d.S.p[q][i] = dense[q * ps + o];
+        d.S.p[19 + q][i] = dense[(19 + q) * ps + o];
+    } else {
+        const double f = dense[q * ps + o], g = dense[(19 + q) * ps + o];
+        ((float*)d.S.p[q])[i] = (float)(f + g - rest_weight(q));
+        ((float*)d.S.p[19 + q])[i] = (float)(f - g);
+    }
+}
+template <class T>
+__device__ __forceinline__ void slot_to_dense(const Dev& d, int q, int i, double* dense, long ps, long o) {
+    if (sizeof(T) == 8) {
+        dense[q * ps + o] = d.S.p[q][i];
+        dense[(19 + q) * ps + o] = d.S.p[19 + q][i];
+    } else {
+        const double F = (double)((const float*)d.S.p[q])[i] + rest_weight(q), A = (double)((const float*)d.S.p[19 + q])[i];
+        dense[q * ps + o] = 0.5 * (F + A);
+        dense[(19 + q) * ps + o] = 0.5 * (F - A);
+    }
+}
 // dense staging <-> compact storage
+template <class T>
 __global__ void k_gather(Dev d, const double* __restrict__ dense, long ps) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.nc) return;
     const int n = d.cpos[t];
-    for (int q = 0; q < 38; ++q) d.S.p[q][t] = dense[q * ps + n];
+    for (int q = 0; q < 19; ++q) slot_from_dense<T>(d, q, (int)t, dense, ps, n);
 }
+template <class T>
 __global__ void k_scatter(Dev d, double* __restrict__ dense, long ps) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.nc) return;
     const int n = d.cpos[t];
-    for (int q = 0; q < 38; ++q) dense[q * ps + n] = d.S.p[q][t];
+    for (int q = 0; q < 19; ++q) slot_to_dense<T>(d, q, (int)t, dense, ps, n);
 }
 // link slots <-> dense staging (after k_gather / after k_scatter: a link slot holds the current value)
-template <bool TO_DENSE>
+template <bool TO_DENSE, class T>
 __global__ void k_links(Dev d, double* __restrict__ dense, long ps) {
     const int m = blockIdx.x * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
@@ -208,13 +328,8 @@ __global__ void k_links(Dev d, double* __restrict__ dense, long ps) {
         if (i < 0) continue;
         const long o = n + d.L.off(r);
         const int q = lat::OPP[r];
-        if (TO_DENSE) {
-            dense[q * ps + o] = d.S.p[q][i];
-            dense[(19 + q) * ps + o] = d.S.p[19 + q][i];
-        } else {
-            d.S.p[q][i] = dense[q * ps + o];
-            d.S.p[19 + q][i] = dense[(19 + q) * ps + o];
-        }
+        if (TO_DENSE) slot_to_dense<T>(d, q, i, dense, ps, o);
+        else slot_from_dense<T>(d, q, i, dense, ps, o);
     }
 }
 
@@ -228,72 +343,82 @@ __constant__ int ZM[5] = {6, 14, 13, 18, 17};
 __constant__ int YP[5] = {3, 7, 8, 15, 17};
 __constant__ int YM[5] = {4, 10, 9, 16, 18};
 
+// the stored values of both slices move unchanged (same direction on both sides)
+template <class T>
 __device__ __forceinline__ void copy_both(const Dev& d, int q, int dst, int src) {
-    const Pop f = fpop(d), g = gpop(d);
-    f(q, dst) = f(q, src);
-    g(q, dst) = g(q, src);
+    T* f = (T*)d.S.p[q];
+    T* g = (T*)d.S.p[19 + q];
+    const int a = slot_index(d, q, dst), b = slot_index(d, q, src);
+    f[a] = f[b];
+    g[a] = g[b];
 }
 // after an even step: outgoing boundary-layer populations into the opposite halo
 // (mpi_pdf_pull_async_pack / _sync_update, Mpi.F90:101-346)
+template <class T>
 __global__ void k_wrap_pull_z(Dev d) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     for (int m = 0; m < 5; ++m) {
-        copy_both(d, ZP[m], L.at(i, j, 0), L.at(i, j, L.nz));
-        copy_both(d, ZM[m], L.at(i, j, L.nz + 1), L.at(i, j, 1));
+        copy_both<T>(d, ZP[m], L.at(i, j, 0), L.at(i, j, L.nz));
+        copy_both<T>(d, ZM[m], L.at(i, j, L.nz + 1), L.at(i, j, 1));
     }
 }
+template <class T>
 __global__ void k_wrap_pull_y(Dev d) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.nz) return;
     const int i = t % L.nx + 1, k = t / L.nx + 1;
     for (int m = 0; m < 5; ++m) {
-        copy_both(d, YP[m], L.at(i, 0, k), L.at(i, L.ny, k));
-        copy_both(d, YM[m], L.at(i, L.ny + 1, k), L.at(i, 1, k));
+        copy_both<T>(d, YP[m], L.at(i, 0, k), L.at(i, L.ny, k));
+        copy_both<T>(d, YM[m], L.at(i, L.ny + 1, k), L.at(i, 1, k));
     }
 }
+template <class T>
 __global__ void k_wrap_pull_edge(Dev d) {
     const Layout L = d.L;
     const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
     if (i > L.nx) return;
-    copy_both(d, 18, L.at(i, L.ny + 1, L.nz + 1), L.at(i, 1, 1));
-    copy_both(d, 16, L.at(i, L.ny + 1, 0), L.at(i, 1, L.nz));
-    copy_both(d, 17, L.at(i, 0, L.nz + 1), L.at(i, L.ny, 1));
-    copy_both(d, 15, L.at(i, 0, 0), L.at(i, L.ny, L.nz));
+    copy_both<T>(d, 18, L.at(i, L.ny + 1, L.nz + 1), L.at(i, 1, 1));
+    copy_both<T>(d, 16, L.at(i, L.ny + 1, 0), L.at(i, 1, L.nz));
+    copy_both<T>(d, 17, L.at(i, 0, L.nz + 1), L.at(i, L.ny, 1));
+    copy_both<T>(d, 15, L.at(i, 0, 0), L.at(i, L.ny, L.nz));
 }
 // after an odd step: populations pushed into the halo go back to the periodic image
 // (mpi_pdf_push_async_pack / _sync_update, Mpi.F90:354-598); faces first, x-parallel edges last
+template <class T>
 __global__ void k_wrap_push_z(Dev d) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     for (int m = 0; m < 5; ++m) {
-        copy_both(d, ZM[m], L.at(i, j, 1), L.at(i, j, L.nz + 1));
-        copy_both(d, ZP[m], L.at(i, j, L.nz), L.at(i, j, 0));
+        copy_both<T>(d, ZM[m], L.at(i, j, 1), L.at(i, j, L.nz + 1));
+        copy_both<T>(d, ZP[m], L.at(i, j, L.nz), L.at(i, j, 0));
     }
 }
+template <class T>
 __global__ void k_wrap_push_y(Dev d) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.nz) return;
     const int i = t % L.nx + 1, k = t / L.nx + 1;
     for (int m = 0; m < 5; ++m) {
-        copy_both(d, YM[m], L.at(i, 1, k), L.at(i, L.ny + 1, k));
-        copy_both(d, YP[m], L.at(i, L.ny, k), L.at(i, 0, k));
+        copy_both<T>(d, YM[m], L.at(i, 1, k), L.at(i, L.ny + 1, k));
+        copy_both<T>(d, YP[m], L.at(i, L.ny, k), L.at(i, 0, k));
     }
 }
+template <class T>
 __global__ void k_wrap_push_edge(Dev d) {
     const Layout L = d.L;
     const int i = blockIdx.x * BLOCK + threadIdx.x + 1;
     if (i > L.nx) return;
-    copy_both(d, 15, L.at(i, L.ny, L.nz), L.at(i, 0, 0));
-    copy_both(d, 17, L.at(i, L.ny, 1), L.at(i, 0, L.nz + 1));
-    copy_both(d, 16, L.at(i, 1, L.nz), L.at(i, L.ny + 1, 0));
-    copy_both(d, 18, L.at(i, 1, 1), L.at(i, L.ny + 1, L.nz + 1));
+    copy_both<T>(d, 15, L.at(i, L.ny, L.nz), L.at(i, 0, 0));
+    copy_both<T>(d, 17, L.at(i, L.ny, 1), L.at(i, 0, L.nz + 1));
+    copy_both<T>(d, 16, L.at(i, 1, L.nz), L.at(i, L.ny + 1, 0));
+    copy_both<T>(d, 18, L.at(i, 1, 1), L.at(i, L.ny + 1, L.nz + 1));
 }
 // phi halo of 4 layers in the periodic directions (MPI_phi_async_pack / _sync_update, Mpi.F90:608-867):
 // each halo node of the y-z plane (x interior) takes the value of its periodic image
@@ -330,6 +455,7 @@ __device__ __forceinline__ void set_inlet_phi(const Dev& d, int i, int j, double
 }
 
 // inlet_bounce_back_velocity_BC_before_odd (Boundary_multiphase_inlet.F90:6-53)
+template <class T>
 __global__ void k_inlet_velocity_before(Dev d, double phi_inlet, double sa_inject, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -343,7 +469,7 @@ __global__ void k_inlet_velocity_before(Dev d, double phi_inlet, double sa_injec
     tmp2 = tmp2 - tmp1;
     const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     for (int c = 0; c < 2; ++c) {
-        const Pop A = c == 0 ? fpop(d) : gpop(d);
+        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double s = c == 0 ? tmp1 : tmp2;
         int o;
         o = L.at(i, j, 0);         A(5, o) = (A(6, L.at(i, j, 1)) + c1 * s) * om + A(5, o) * wd;
@@ -354,6 +480,7 @@ __global__ void k_inlet_velocity_before(Dev d, double phi_inlet, double sa_injec
     }
 }
 // inlet_bounce_back_velocity_BC_after_odd (Boundary_multiphase_inlet.F90:56-102)
+template <class T>
 __global__ void k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -368,7 +495,7 @@ __global__ void k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject
     const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     const int o = L.at(i, j, 1);
     for (int c = 0; c < 2; ++c) {
-        const Pop A = c == 0 ? fpop(d) : gpop(d);
+        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double s = c == 0 ? tmp1 : tmp2;
         A(6, o) = (A(5, L.at(i, j, 0)) + c1 * s) * om + A(6, o) * wd;
         A(13, o) = (A(12, L.at(i + 1, j, 0)) + c2 * s) * om + A(13, o) * wd;
@@ -378,6 +505,7 @@ __global__ void k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject
     }
 }
 // inlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_inlet.F90:115-209)
+template <class T>
 __global__ void k_inlet_pressure_before(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -390,7 +518,7 @@ __global__ void k_inlet_pressure_before(Dev d, double phi_inlet, double sa_injec
     const double tmpRho1 = rho_in * sa_inject;
     tmpRho2 = tmpRho2 - tmpRho1;
     for (int c = 0; c < 2; ++c) {
-        const Pop A = c == 0 ? fpop(d) : gpop(d);
+        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double rr = c == 0 ? tmpRho1 : tmpRho2;
         const double s = (rr - (A(0, L.at(i, j, 1)) + A(1, L.at(i - 1, j, 1)) + A(2, L.at(i + 1, j, 1)) +
                                 A(3, L.at(i, j - 1, 1)) + A(4, L.at(i, j + 1, 1)) + A(7, L.at(i - 1, j - 1, 1)) +
@@ -410,6 +538,7 @@ __global__ void k_inlet_pressure_before(Dev d, double phi_inlet, double sa_injec
     }
 }
 // inlet_Zou_He_pressure_BC_after_odd (Boundary_multiphase_inlet.F90:212-294)
+template <class T>
 __global__ void k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -423,7 +552,7 @@ __global__ void k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject
     tmpRho2 = tmpRho2 - tmpRho1;
     const int o = L.at(i, j, 1);
     for (int c = 0; c < 2; ++c) {
-        const Pop A = c == 0 ? fpop(d) : gpop(d);
+        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double rr = c == 0 ? tmpRho1 : tmpRho2;
         const double s = (rr - (A(0, o) + A(2, o) + A(1, o) + A(4, o) + A(3, o) + A(8, o) + A(7, o) + A(10, o) + A(9, o) +
                                 2.0 * (A(5, o) + A(11, o) + A(12, o) + A(15, o) + A(16, o)))) * relaxation;
@@ -438,6 +567,7 @@ __global__ void k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject
 }
 
 // outlet_convective_BC_before_odd (Boundary_multiphase_outlet.F90:7-65)
+template <class T>
 __global__ void k_outlet_convective_before(Dev d, double u_convec) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -462,18 +592,19 @@ __global__ void k_outlet_convective_before(Dev d, double u_convec) {
     for (int c = 0; c < 2; ++c) {
         double* hist = c == 0 ? d.cf : d.cg;
         for (int m = 0; m < 5; ++m) {
-            const Pop A = c == 0 ? fpop(d) : gpop(d);
+            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
             const int o = L.at(i + DI[m], j + DJ[m], nz + 1);
             const int in = L.at(i + DI[m], j + DJ[m], nz);
             A(QS[m], o) = ((hist[pc + QS[m] * pl] + u_convec * A(QS[m], in)) * temp) * om + A(QS[m], o) * wd;
         }
         for (int m = 0; m < 5; ++m) {
-            const Pop A = c == 0 ? fpop(d) : gpop(d);
+            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
             hist[pc + QS[m] * pl] = A(QS[m], L.at(i + DI[m], j + DJ[m], nz + 1));
         }
     }
 }
 // outlet_convective_BC_after_odd (Boundary_multiphase_outlet.F90:68-127)
+template <class T>
 __global__ void k_outlet_convective_after(Dev d, double u_convec) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -498,16 +629,17 @@ __global__ void k_outlet_convective_after(Dev d, double u_convec) {
     for (int c = 0; c < 2; ++c) {
         double* hist = c == 0 ? d.cf : d.cg;
         for (int m = 0; m < 5; ++m) {
-            const Pop A = c == 0 ? fpop(d) : gpop(d);
+            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
             A(QW[m], o) = ((hist[pc + QH[m] * pl] + u_convec * A(QW[m], om1)) * temp) * om + A(QW[m], o) * wd;
         }
         for (int m = 0; m < 5; ++m) {
-            const Pop A = c == 0 ? fpop(d) : gpop(d);
+            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
             hist[pc + QH[m] * pl] = A(QW[m], o);
         }
     }
 }
 // outlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_outlet.F90:139-228)
+template <class T>
 __global__ void k_outlet_pressure_before(Dev d, double rho_out) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -521,7 +653,7 @@ __global__ void k_outlet_pressure_before(Dev d, double rho_out) {
     ph[L.at(i, j, nz + 2)] = pz;
     ph[L.at(i, j, nz + 3)] = pz;
     ph[L.at(i, j, nz + 4)] = pz;
-    const Pop f = fpop(d), g = gpop(d);
+    const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
     const int a0 = L.at(i, j, nz), am = L.at(i - 1, j, nz), ap = L.at(i + 1, j, nz), bm = L.at(i, j - 1, nz),
               bp = L.at(i, j + 1, nz), mm = L.at(i - 1, j - 1, nz), pm = L.at(i + 1, j - 1, nz),
               mp = L.at(i - 1, j + 1, nz), pp = L.at(i + 1, j + 1, nz);
@@ -534,7 +666,7 @@ __global__ void k_outlet_pressure_before(Dev d, double rho_out) {
     const double tmp2 = tmp1 * 0.5 * (1.0 - pz);
     tmp1 = tmp1 - tmp2;
     for (int c = 0; c < 2; ++c) {
-        const Pop A = c == 0 ? f : g;
+        const Pop<T> A = c == 0 ? f : g;
         const double s = c == 0 ? tmp1 : tmp2;
         const double tnx = 0.5 * (A(1, am) + A(7, mm) + A(9, mp) - (A(2, ap) + A(8, pm) + A(10, pp)));
         const double tny = 0.5 * (A(3, bm) + A(7, mm) + A(8, pm) - (A(4, bp) + A(10, pp) + A(9, mp)));
@@ -547,6 +679,7 @@ __global__ void k_outlet_pressure_before(Dev d, double rho_out) {
     }
 }
 // outlet_Zou_He_pressure_BC_after_odd (Boundary_multiphase_outlet.F90:231-311)
+template <class T>
 __global__ void k_outlet_pressure_after(Dev d, double rho_out) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -560,7 +693,7 @@ __global__ void k_outlet_pressure_after(Dev d, double rho_out) {
     ph[L.at(i, j, nz + 2)] = pz;
     ph[L.at(i, j, nz + 3)] = pz;
     ph[L.at(i, j, nz + 4)] = pz;
-    const Pop f = fpop(d), g = gpop(d);
+    const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
     const int o = L.at(i, j, nz);
     double tmp1 = (f(0, o) + f(2, o) + f(1, o) + f(4, o) + f(3, o) + f(8, o) + f(7, o) + f(10, o) + f(9, o) +
                    2.0 * (f(6, o) + f(14, o) + f(13, o) + f(18, o) + f(17, o)) + g(0, o) + g(2, o) + g(1, o) + g(4, o) +
@@ -569,7 +702,7 @@ __global__ void k_outlet_pressure_after(Dev d, double rho_out) {
     const double tmp2 = tmp1 * 0.5 * (1.0 - pz);
     tmp1 = tmp1 - tmp2;
     for (int c = 0; c < 2; ++c) {
-        const Pop A = c == 0 ? f : g;
+        const Pop<T> A = c == 0 ? f : g;
         const double s = c == 0 ? tmp1 : tmp2;
         const double tnx = 0.5 * (A(2, o) + A(8, o) + A(10, o) - (A(1, o) + A(7, o) + A(9, o)));
         const double tny = 0.5 * (A(4, o) + A(10, o) + A(9, o) - (A(3, o) + A(7, o) + A(8, o)));
@@ -583,13 +716,14 @@ __global__ void k_outlet_pressure_after(Dev d, double rho_out) {
 
 // porous_plate_BC_before_odd / _after_odd (Boundary_multiphase_other.F90:8-184); `blocked` is the colour
 // reflected by the plate (0 = fluid 1, 1 = fluid 2), the other colour passes straight through
+template <class T>
 __global__ void k_porous_plate(Dev d, int zp, int blocked, int after) {
     const Layout L = d.L;
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
-    const Pop B = blocked == 0 ? fpop(d) : gpop(d);
-    const Pop P = blocked == 0 ? gpop(d) : fpop(d);
+    const Pop<T> B = blocked == 0 ? fpop<T>(d) : gpop<T>(d);
+    const Pop<T> P = blocked == 0 ? gpop<T>(d) : fpop<T>(d);
     if (!after) {
         B(6, L.at(i, j, zp)) = B(5, L.at(i, j, zp - 1));
         B(13, L.at(i - 1, j, zp)) = B(12, L.at(i, j, zp - 1));
@@ -798,12 +932,14 @@ __global__ void k_curvature_dense(Dev d) {
 // ------------------------------------------------------------------------------------------------------
 
 // u, v, w, rho at the fluid nodes; solid nodes hold 0 as the reference's (1 - wall) factor gives
+template <class T>
 __global__ void k_macro(Dev d, double force_z, double* u, double* v, double* w, double* rho) {
     const int m = blockIdx.x * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const int n = d.cpos[m];
     double ft[19];
-    for (int q = 0; q < 19; ++q) ft[q] = d.S.p[q][m] + d.S.p[19 + q][m];
+    for (int q = 0; q < 19; ++q)
+        ft[q] = sizeof(T) == 8 ? d.S.p[q][m] + d.S.p[19 + q][m] : (double)((const float*)d.S.p[q])[m] + rest_weight(q);
     const double om = 1.0;
     rho[n] = (ft[0] + ft[1] + ft[2] + ft[3] + ft[4] + ft[5] + ft[6] + ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] +
               ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18]) * om;
@@ -893,6 +1029,69 @@ __global__ void __launch_bounds__(BLOCK) k_monitor_layers(Dev d, const double* u
         for (int m = 7; m < NPART + 3; ++m) part[(m - 7) * L.nz + (k - 1)] = out[m];
     }
 }
+// Initial state of a new run at rest, on the device (initialization_new_multi / _pdf, Init_multiphase.F90:
+// 243-470, the same expressions as mflbm_init_fields with u = 0): phi by initial_fluid_distribution_option
+// over the halo-4 box, rho1 = (1 + phi)/2 and rho2 = (1 - phi)/2, and the equilibrium populations of both
+// colours over the halo-1 box into the dense staging arrays (then gathered like an upload).
+__global__ void k_init_dense(Dev d, double* __restrict__ dense, long ps, int option, double z0, double phi_inlet,
+                             int open_z) {
+    const Layout L = d.L;
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long nb = (long)(L.nx + 8) * (L.ny + 8) * (L.nz + 8);
+    if (t >= nb) return;
+    const int i = (int)(t % (L.nx + 8)) - 3, j = (int)((t / (L.nx + 8)) % (L.ny + 8)) - 3,
+              k = (int)(t / ((long)(L.nx + 8) * (L.ny + 8))) - 3;
+    const double x = i, y = j, z = k;
+    double v;
+    if (option == 1) {
+        v = -1.0;
+        if (z <= z0) v = 1.0;
+    } else if (option == 2) {
+        v = 1.0;
+        if (z <= z0) v = -1.0;
+    } else if (option == 3 || option == 4) {
+        const double dx = x - (L.nx + 1) * 0.0, dz = z - (L.nz + 1) * 0.5, dy = y - (L.ny + 1) * 0.5;
+        const bool in = dx * dx + dz * dz + dy * dy <= z0 * z0;
+        v = option == 3 ? (in ? 1.0 : -1.0) : (in ? -1.0 : 1.0);
+    } else {
+        const double dx = x - (L.nx + 1) * 0.5, dz = z - (L.nz + 1) * 0.5, dy = y - (L.ny + 1) * 0.5;
+        v = dx * dx + dz * dz + dy * dy <= z0 * z0 ? 1.0 : -1.0;
+    }
+    if (open_z && z <= 0) v = phi_inlet;
+    const int o = L.at(i, j, k);
+    d.phi[o] = v;
+    if (i < 0 || i > L.nx + 1 || j < 0 || j > L.ny + 1 || k < 0 || k > L.nz + 1) return;
+    const double rho = 1.0, usq = 0.0, eu = 0.0, uu = 0.0;
+    const double r1 = rho * (1.0 + v) * 0.5, r2 = rho * (1.0 - v) * 0.5;
+    for (int q = 0; q < 19; ++q) {
+        const double wq = lat::W[q];
+        double a, b;
+        if (q == 0) {
+            a = r1 * wq + r1 * wq * (-1.5 * usq);
+            b = r2 * wq + r2 * wq * (-1.5 * usq);
+        } else {
+            a = r1 * wq + r1 * wq * (3.0 * eu + 4.5 * uu * uu - 1.5 * usq);
+            b = r2 * wq + r2 * wq * (3.0 * eu + 4.5 * uu * uu - 1.5 * usq);
+        }
+        dense[q * ps + o] = a;
+        dense[(19 + q) * ps + o] = b;
+    }
+}
+// convective-outlet history of a new run: the populations and phi of the outlet layer z = nz
+__global__ void k_init_convec(Dev d, const double* __restrict__ dense, long ps) {
+    const Layout L = d.L;
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= (L.nx + 2) * (L.ny + 2)) return;
+    const int i = t % (L.nx + 2), j = t / (L.nx + 2);
+    const long pl = (long)(L.nx + 2) * (L.ny + 2);
+    const int o = L.at(i, j, L.nz);
+    for (int q = 0; q < 19; ++q) {
+        d.cf[t + q * pl] = dense[q * ps + o];
+        d.cg[t + q * pl] = dense[(19 + q) * ps + o];
+    }
+    d.cphi[t] = d.phi[o];
+}
+
 __global__ void k_copy_phi_old(Dev d, double* phi_old) {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= d.L.total) return;
@@ -925,7 +1124,8 @@ struct mflbm_ctx {
     double* phi_old = nullptr;
     double* red = nullptr;      // monitor layer sums and partials
     // geometry-dependent device buffers
-    double* compact = nullptr;  // 38 population slices (compact slots + link slots)
+    void* compact = nullptr;    // 38 population slices (compact slots + link slots), FP64 or FP32
+    bool fp32 = false;          // precision 1 or 2: FP32 storage (s, a) of collide.h
     double* force = nullptr;    // 4 compact force arrays
     int* map = nullptr;
     int* ibuf = nullptr;        // cpos, normal list, solid lists, fluid list positions
@@ -962,9 +1162,11 @@ int launch_gradient(mflbm_ctx* c) {
     return MFLBM_OK;
 }
 
+template <class T>
+int launch_faces(mflbm_ctx* c, bool odd);
+
 int launch_step(mflbm_ctx* c, int ntime) {
     const mflbm_params& p = c->p;
-    const Layout& L = c->L;
     const CollideParams P = collide_params(p);
     const bool odd = (ntime % 2) != 0;
     const bool t = c->timing;
@@ -982,44 +1184,78 @@ int launch_step(mflbm_ctx* c, int ntime) {
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
     }
     const int per = xcd_per(c->d.nf);
-    if (odd) LAUNCH(c, k_collide<true>, NXCD * per, c->d, P, per);
-    else LAUNCH(c, k_collide<false>, NXCD * per, c->d, P, per);
+    if (c->fp32 && c->p.precision == 2) {
+        const CollideParamsDev<double> PD{p.gamma, p.beta, p.force_z, p.la_nui1, p.la_nui2};
+        if (odd) LAUNCH(c, (k_collide32<true, double>), NXCD * per, c->d, PD, p.gamma, per);
+        else LAUNCH(c, (k_collide32<false, double>), NXCD * per, c->d, PD, p.gamma, per);
+    } else if (c->fp32) {
+        const CollideParamsDev<float> P32{(float)p.gamma, (float)p.beta, (float)p.force_z, (float)p.la_nui1, (float)p.la_nui2};
+        if (odd) LAUNCH(c, (k_collide32<true, float>), NXCD * per, c->d, P32, p.gamma, per);
+        else LAUNCH(c, (k_collide32<false, float>), NXCD * per, c->d, P32, p.gamma, per);
+    } else {
+        if (odd) LAUNCH(c, k_collide<true>, NXCD * per, c->d, P, per);
+        else LAUNCH(c, k_collide<false>, NXCD * per, c->d, P, per);
+    }
     if (t) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    const int rc = c->fp32 ? launch_faces<float>(c, odd) : launch_faces<double>(c, odd);
+    if (rc) return rc;
+    if (t) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    const int rg = launch_gradient(c);
+    if (rg) return rg;
+    if (t) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    return MFLBM_OK;
+}
+
+// periodic wrap and boundary conditions of one step, after the lattice update
+template <class T>
+int launch_faces(mflbm_ctx* c, bool odd) {
+    const mflbm_params& p = c->p;
+    const Layout& L = c->L;
     // periodic wrap (the reference's MPI exchange of one rank with itself)
     if (p.kper == 1 || p.jper == 1) {
         if (!odd) {
-            if (p.kper == 1) LAUNCH(c, k_wrap_pull_z, nblocks((long)L.nx * L.ny), c->d);
-            if (p.jper == 1) LAUNCH(c, k_wrap_pull_y, nblocks((long)L.nx * L.nz), c->d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_pull_edge, nblocks(L.nx), c->d);
+            if (p.kper == 1) LAUNCH(c, k_wrap_pull_z<T>, nblocks((long)L.nx * L.ny), c->d);
+            if (p.jper == 1) LAUNCH(c, k_wrap_pull_y<T>, nblocks((long)L.nx * L.nz), c->d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_pull_edge<T>, nblocks(L.nx), c->d);
         } else {
-            if (p.kper == 1) LAUNCH(c, k_wrap_push_z, nblocks((long)L.nx * L.ny), c->d);
-            if (p.jper == 1) LAUNCH(c, k_wrap_push_y, nblocks((long)L.nx * L.nz), c->d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_push_edge, nblocks(L.nx), c->d);
+            if (p.kper == 1) LAUNCH(c, k_wrap_push_z<T>, nblocks((long)L.nx * L.ny), c->d);
+            if (p.jper == 1) LAUNCH(c, k_wrap_push_y<T>, nblocks((long)L.nx * L.nz), c->d);
+            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_push_edge<T>, nblocks(L.nx), c->d);
         }
         LAUNCH(c, k_wrap_phi, nblocks((long)L.nx * (L.ny + 8) * (L.nz + 8)), c->d, p.jper, p.kper);
     }
     const unsigned gface = nblocks((long)L.nx * L.ny);
     if (open_z(p)) {
         if (!odd) {
-            if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_before, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
-            else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_before, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);
-            if (p.outlet_bc == 1) LAUNCH(c, k_outlet_convective_before, gface, c->d, p.uin_avg);
-            else if (p.outlet_bc == 2) LAUNCH(c, k_outlet_pressure_before, gface, c->d, p.rho_out);
+            if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_before<T>, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_before<T>, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_outlet_convective_before<T>, gface, c->d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_outlet_pressure_before<T>, gface, c->d, p.rho_out);
         } else {
-            if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_after, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
-            else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_after, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);
-            if (p.outlet_bc == 1) LAUNCH(c, k_outlet_convective_after, gface, c->d, p.uin_avg);
-            else if (p.outlet_bc == 2) LAUNCH(c, k_outlet_pressure_after, gface, c->d, p.rho_out);
+            if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_after<T>, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
+            else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_after<T>, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);
+            if (p.outlet_bc == 1) LAUNCH(c, k_outlet_convective_after<T>, gface, c->d, p.uin_avg);
+            else if (p.outlet_bc == 2) LAUNCH(c, k_outlet_pressure_after<T>, gface, c->d, p.rho_out);
         }
     }
     if (p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2) {
         const int zp = p.z_porous_plate;
-        if (zp >= 1 && zp <= L.nz) LAUNCH(c, k_porous_plate, gface, c->d, zp, p.porous_plate_cmd - 1, odd ? 1 : 0);
+        if (zp >= 1 && zp <= L.nz) LAUNCH(c, k_porous_plate<T>, gface, c->d, zp, p.porous_plate_cmd - 1, odd ? 1 : 0);
     }
-    if (t) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-    int rc = launch_gradient(c);
-    if (rc) return rc;
-    if (t) HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    return MFLBM_OK;
+}
+
+// dense staging -> compact storage (upload, device initialisation) and back (download)
+template <class T>
+int gather_state(mflbm_ctx* c) {
+    LAUNCH(c, k_gather<T>, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
+    LAUNCH(c, (k_links<false, T>), nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
+    return MFLBM_OK;
+}
+template <class T>
+int scatter_state(mflbm_ctx* c) {
+    LAUNCH(c, k_scatter<T>, nblocks(c->d.nc), c->d, c->dense_pdf, c->ps);
+    LAUNCH(c, (k_links<true, T>), nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
     return MFLBM_OK;
 }
 
@@ -1052,11 +1288,15 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return fail(nullptr, MFLBM_ERR_ARG, "lattice must be at least 3^3");
     if (prm->wall_x_min != 1 || prm->wall_x_max != 1)
         return fail(nullptr, MFLBM_ERR_ARG, "x needs solid walls at both ends (the reference rejects anything else)");
+    if (prm->precision < 0 || prm->precision > 2)
+        return fail(nullptr, MFLBM_ERR_ARG, "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 storage, FP64 collision), got %d",
+                    prm->precision);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MFLBM_ERR_NODEV, "no HIP device visible");
     mflbm_ctx* c = new (std::nothrow) mflbm_ctx();
     if (!c) return fail(nullptr, MFLBM_ERR_NOMEM, "context allocation failed");
     c->p = *prm;
+    c->fp32 = prm->precision != 0;
     Layout& L = c->L;
     L = make_layout(prm->nx, prm->ny, prm->nz);
     if (L.total > (1L << 31) - 1) {
@@ -1226,19 +1466,20 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     // device buffers
     free_geometry(c);
     // slice q: nc compact slots, then the link slots of direction opp(q); each slice 256-byte aligned
+    const long esz = c->fp32 ? (long)sizeof(float) : (long)sizeof(double);
     long soff[39];
     soff[0] = 0;
     for (int q = 0; q < 38; ++q) {
         const int qq = q % 19;
-        const long len = (((long)nc + (qq ? nlink[lat::OPP[qq]] : 0) + 31) / 32) * 32;
-        if (len * 8 >= (1L << 32))
+        const long len = (((long)nc + (qq ? nlink[lat::OPP[qq]] : 0) + 63) / 64) * 64;
+        if (len * esz >= (1L << 32))
             return fail(c, MFLBM_ERR_ARG, "population slice of %ld slots exceeds 32-bit byte offsets", len);
         soff[q + 1] = soff[q] + len;
     }
     const long fs = (((long)nfl + 63) / 64) * 64;
     const size_t ni = (size_t)nc + nrm.size() + ns + s1pos.size() + nf + 1;
     const size_t nd = (size_t)ns + s1w.size() + fd.size() + 1;
-    HIPCHK(c, hipMalloc(&c->compact, sizeof(double) * soff[38]));
+    HIPCHK(c, hipMalloc(&c->compact, esz * soff[38]));
     HIPCHK(c, hipMalloc(&c->force, sizeof(double) * fs * 4));
     HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
     HIPCHK(c, hipMalloc(&c->ibuf, sizeof(int) * ni));
@@ -1250,7 +1491,7 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMalloc(&c->wbase, sizeof(int) * wbase.size()));
     HIPCHK(c, hipMemcpy(c->wmask, links.wmask.data(), sizeof(unsigned long long) * links.wmask.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->wbase, wbase.data(), sizeof(int) * wbase.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->compact, 0, sizeof(double) * soff[38]));
+    HIPCHK(c, hipMemset(c->compact, 0, esz * soff[38]));
     HIPCHK(c, hipMemset(c->force, 0, sizeof(double) * fs * 4));
     // normals at solid nodes are only ever written by the extrapolation; start them from zero so no value
     // of a previous geometry survives (cn_x, cn_y, cn_z, c_norm, curv, u, v, w, rho)
@@ -1277,7 +1518,8 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMemcpy(c->walls, wp.data(), (size_t)L.total, hipMemcpyHostToDevice));
     if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
     Dev& d = c->d;
-    for (int q = 0; q < 38; ++q) d.S.p[q] = c->compact + soff[q];
+    // FP32 storage keeps float slices behind the same pointers (the kernels templated on the storage type cast)
+    for (int q = 0; q < 38; ++q) d.S.p[q] = (double*)((char*)c->compact + esz * soff[q]);
     d.map = c->map;
     d.cpos = c->ibuf;
     d.nf = nfl;
@@ -1321,8 +1563,33 @@ extern "C" int mflbm_upload_state(mflbm_ctx* c, const double* const* f, const do
     if (g_convec) HIPCHK(c, hipMemcpy(c->d.cg, g_convec, sizeof(double) * plane * 19, hipMemcpyHostToDevice));
     if (phi_convec) HIPCHK(c, hipMemcpy(c->d.cphi, phi_convec, sizeof(double) * plane, hipMemcpyHostToDevice));
     HIPCHK(c, hipDeviceSynchronize());
-    LAUNCH(c, k_gather, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
-    LAUNCH(c, k_links<false>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
+    {
+        const int rs = c->fp32 ? gather_state<float>(c) : gather_state<double>(c);
+        if (rs) return rs;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    c->phi_old_valid = false;
+    return MFLBM_OK;
+}
+
+extern "C" int mflbm_init_state_device(mflbm_ctx* c, int32_t option, double interface_z0) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom) return fail(c, MFLBM_ERR_STATE, "upload the geometry before the state");
+    if (option < 1 || option > 5)
+        return fail(c, MFLBM_ERR_ARG, "initial_fluid_distribution_option %d not supported (1..5)", option);
+    const Layout& L = c->L;
+    const mflbm_params& p = c->p;
+    const bool open_z = p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0;
+    HIPCHK(c, hipMemsetAsync(c->d.phi, 0, sizeof(double) * L.total, c->stream));
+    const long nb = (long)(L.nx + 8) * (L.ny + 8) * (L.nz + 8);
+    LAUNCH(c, k_init_dense, nblocks(nb), c->d, c->dense_pdf, c->ps, (int)option, interface_z0, p.phi_inlet,
+           open_z ? 1 : 0);
+    if (p.outlet_bc == 1) LAUNCH(c, k_init_convec, nblocks((long)(L.nx + 2) * (L.ny + 2)), c->d, (const double*)c->dense_pdf, c->ps);
+    {
+        const int rs = c->fp32 ? gather_state<float>(c) : gather_state<double>(c);
+        if (rs) return rs;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_state = true;
     c->phi_old_valid = false;
@@ -1334,8 +1601,10 @@ extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* cons
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     const Layout& L = c->L;
-    LAUNCH(c, k_scatter, nblocks(c->d.nc), c->d, c->dense_pdf, c->ps);
-    LAUNCH(c, k_links<true>, nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
+    {
+        const int rs = c->fp32 ? scatter_state<float>(c) : scatter_state<double>(c);
+        if (rs) return rs;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
     for (int q = 0; q < 38; ++q) {
@@ -1383,7 +1652,8 @@ extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     // the force input of the macroscopic velocity comes from the current normal field
     if (MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
-    LAUNCH(c, k_macro, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
+    if (c->fp32) LAUNCH(c, k_macro<float>, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
+    else LAUNCH(c, k_macro<double>, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
     LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
     return MFLBM_OK;
 }

@@ -31,10 +31,10 @@ namespace {
 constexpr int WAVE = 64;
 
 // lattice update of every active sample; global wave w covers nodes [64 (w - wave_first[s]), +64) of sample s
-template <bool ODD, class T, int NT>
+template <bool ODD, class T, class C, int NT>
 __global__ void __launch_bounds__(NT) k_spb_collide(const SDev<T>* __restrict__ devs, const int* __restrict__ wave_sample,
                                                     const int* __restrict__ wave_first, const int* __restrict__ active,
-                                                    SPCollideParamsT<T> P, int nwaves, int per) {
+                                                    SPCollideParamsT<C> P, int nwaves, int per) {
     const int w = xcd_block(per) * (NT / WAVE) + (int)(threadIdx.x / WAVE);
     if (w >= nwaves) return;
     const int s = __builtin_amdgcn_readfirstlane(wave_sample[w]);
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(NT) k_spb_collide(const SDev<T>* __restrict__ 
     const SDev<T>& d = devs[s];
     const int m = (w - wave_first[s]) * WAVE + (int)(threadIdx.x % WAVE);
     if (m >= d.nf) return;
-    sp_update_node<ODD, T>(d, P, m);
+    sp_update_node<ODD, T, C>(d, P, m);
 }
 
 // periodic self-exchange of every active sample; KIND 0..5 = pull z, pull y, pull edge, push z, push y, push
@@ -162,7 +162,8 @@ struct mflbm_spb_ctx {
     mflbm_spb_params p{};
     SPCollideParams cp{};
     SPCollideParamsT<float> cp32{};
-    bool fp32 = false;
+    bool fp32 = false;             // precision 1 or 2: FP32 deviation storage
+    bool mixed = false;            // precision 2: collided in FP64
     int block = 256, tile = 32;
     std::vector<SampleHost> host;
     // per sample after commit
@@ -215,21 +216,26 @@ const SPCollideParamsT<double>& rates_of<double>(const mflbm_spb_ctx* c) { retur
 template <>
 const SPCollideParamsT<float>& rates_of<float>(const mflbm_spb_ctx* c) { return c->cp32; }
 
-template <class T, int NT>
-int launch_collide(mflbm_spb_ctx* c, bool odd) {
+template <class T, class C, int NT>
+int launch_collide_t(mflbm_spb_ctx* c, bool odd) {
     const long nb = ((long)c->nwaves * WAVE + NT - 1) / NT;
     const int per = (int)((nb + NXCD - 1) / NXCD);
     const unsigned grid = (unsigned)(NXCD * per);
     if (grid == 0) return MFLBM_OK;
     if (odd)
-        hipLaunchKernelGGL((k_spb_collide<true, T, NT>), dim3(grid), dim3(NT), 0, c->stream, devs_of<T>(c), c->wave_sample,
-                           c->wave_first, c->M.active, rates_of<T>(c), c->nwaves, per);
+        hipLaunchKernelGGL((k_spb_collide<true, T, C, NT>), dim3(grid), dim3(NT), 0, c->stream, devs_of<T>(c),
+                           c->wave_sample, c->wave_first, c->M.active, rates_of<C>(c), c->nwaves, per);
     else
-        hipLaunchKernelGGL((k_spb_collide<false, T, NT>), dim3(grid), dim3(NT), 0, c->stream, devs_of<T>(c), c->wave_sample,
-                           c->wave_first, c->M.active, rates_of<T>(c), c->nwaves, per);
+        hipLaunchKernelGGL((k_spb_collide<false, T, C, NT>), dim3(grid), dim3(NT), 0, c->stream, devs_of<T>(c),
+                           c->wave_sample, c->wave_first, c->M.active, rates_of<C>(c), c->nwaves, per);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, MFLBM_ERR_HIP, "launch k_spb_collide: %s", hipGetErrorString(e));
     return MFLBM_OK;
+}
+
+template <class T, int NT>
+int launch_collide(mflbm_spb_ctx* c, bool odd) {
+    return c->mixed ? launch_collide_t<T, double, NT>(c, odd) : launch_collide_t<T, T, NT>(c, odd);
 }
 
 template <class T, int KIND>
@@ -352,8 +358,9 @@ extern "C" int mflbm_spb_create(const mflbm_spb_params* prm, mflbm_spb_ctx** out
     if (!out) return fail(nullptr, MFLBM_ERR_ARG, "out is NULL");
     *out = nullptr;
     if (!prm) return fail(nullptr, MFLBM_ERR_ARG, "params is NULL");
-    if (prm->precision != 0 && prm->precision != 1)
-        return fail(nullptr, MFLBM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 deviations), got %d", prm->precision);
+    if (prm->precision < 0 || prm->precision > 2)
+        return fail(nullptr, MFLBM_ERR_ARG, "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 storage, FP64 collision), got %d",
+                    prm->precision);
     if (!(prm->la_nu > 0.0)) return fail(nullptr, MFLBM_ERR_ARG, "viscosity must be positive");
     if (prm->ntime_monitor < 2 || prm->ntime_monitor % 2)
         return fail(nullptr, MFLBM_ERR_ARG, "ntime_monitor must be even and positive (the flow rate is read after even steps)");
@@ -367,7 +374,8 @@ extern "C" int mflbm_spb_create(const mflbm_spb_params* prm, mflbm_spb_ctx** out
     c->p = *prm;
     c->cp = sp_collide_params(prm->la_nu, prm->mrt_preset, prm->force_z);
     c->cp32 = to_float(c->cp);
-    c->fp32 = prm->precision == 1;
+    c->fp32 = prm->precision != 0;
+    c->mixed = prm->precision == 2;
     c->block = prm->block ? prm->block : 256;
     c->tile = prm->tile_rows ? prm->tile_rows : tile_rows();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -275,7 +275,8 @@ struct mflbm_sp_ctx {
     mflbm_sp_params p{};
     SPCollideParams cp{};
     SPCollideParamsT<float> cp32{};
-    bool fp32 = false;           // precision == 1: FP32 deviation storage
+    bool fp32 = false;           // precision 1 or 2: FP32 deviation storage
+    bool mixed = false;          // precision 2: the FP32 deviations collided in FP64
     Layout L{};
     SDev<double> d{};            // slice pointers of the FP64 mode; the other fields are shared by both views
     SDev<float> d32{};
@@ -322,8 +323,13 @@ int sp_launch_step_t(mflbm_sp_ctx* c, int ntime, hipEvent_t* ev) {
     const SDev<T>& d = view<T>(c);
     const bool odd = (ntime % 2) != 0;
     const int per = xcd_per(d.nf);
-    if (odd) LAUNCH(c, (k_sp_collide<true, T>), NXCD * per, d, rates<T>(c), per);
-    else LAUNCH(c, (k_sp_collide<false, T>), NXCD * per, d, rates<T>(c), per);
+    if (c->mixed) {
+        if (odd) LAUNCH(c, (k_sp_collide<true, T, double>), NXCD * per, d, c->cp, per);
+        else LAUNCH(c, (k_sp_collide<false, T, double>), NXCD * per, d, c->cp, per);
+    } else {
+        if (odd) LAUNCH(c, (k_sp_collide<true, T, T>), NXCD * per, d, rates<T>(c), per);
+        else LAUNCH(c, (k_sp_collide<false, T, T>), NXCD * per, d, rates<T>(c), per);
+    }
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     if (p.kper == 1 || p.jper == 1) {
         if (!odd) {
@@ -437,8 +443,9 @@ extern "C" int mflbm_sp_create(const mflbm_sp_params* prm, mflbm_sp_ctx** out) {
     if (prm->wall_x_min != 1 || prm->wall_x_max != 1)
         return fail(nullptr, MFLBM_ERR_ARG, "x needs solid walls at both ends (the reference rejects anything else)");
     if (!(prm->la_nu > 0.0)) return fail(nullptr, MFLBM_ERR_ARG, "viscosity must be positive");
-    if (prm->precision != 0 && prm->precision != 1)
-        return fail(nullptr, MFLBM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 deviations), got %d", prm->precision);
+    if (prm->precision < 0 || prm->precision > 2)
+        return fail(nullptr, MFLBM_ERR_ARG, "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 storage, FP64 collision), got %d",
+                    prm->precision);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MFLBM_ERR_NODEV, "no HIP device visible");
     mflbm_sp_ctx* c = new (std::nothrow) mflbm_sp_ctx();
@@ -446,7 +453,8 @@ extern "C" int mflbm_sp_create(const mflbm_sp_params* prm, mflbm_sp_ctx** out) {
     c->p = *prm;
     c->cp = sp_collide_params(prm->la_nu, prm->mrt_preset, prm->force_z);
     c->cp32 = to_float(c->cp);
-    c->fp32 = prm->precision == 1;
+    c->fp32 = prm->precision != 0;
+    c->mixed = prm->precision == 2;
     Layout& L = c->L;
     L = make_layout(prm->nx, prm->ny, prm->nz);
     if (L.total > (1L << 31) - 1) {

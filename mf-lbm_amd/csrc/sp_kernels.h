@@ -4,7 +4,8 @@
 //
 // Two storage modes share every kernel through the element type T of the population slices:
 //   double: the populations themselves, the reference arithmetic bit for bit;
-//   float : the deviations d = f - REST from the model's rest equilibrium (collide_sp.h), collided in FP32.
+//   float : the deviations d = f - REST from the model's rest equilibrium (collide_sp.h), collided in FP32 or,
+//           in the mixed mode, in FP64 (rounded to FP32 once per store).
 //   Boundaries, the periodic exchange, the macroscopic fields and every reduction read and write f in FP64
 //   through SPop (f = d + REST on load, d = f - REST rounded once on store); the exchange moves stored values.
 #pragma once
@@ -85,8 +86,9 @@ __device__ __forceinline__ int plane_at(const SDev<T>& d, int i, int j) { return
 // ODD = true : kernel_odd  (Kernel.F90:5-200): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
 // The update of fluid node m (compact index, m < nf).  m & 63 must be the node's lane in its wave and all
 // lanes of a wave must belong to the same sample: the link masks are per wave of 64 consecutive nodes.
-template <bool ODD, class T>
-__device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollideParamsT<T>& P, int m) {
+// T is the storage type, C the arithmetic of the collision (C = T, or FP32 storage with an FP64 collision).
+template <bool ODD, class T, class C>
+__device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollideParamsT<C>& P, int m) {
     T a[19];
     unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
     a[0] = slot_ld(d.S[0], m);
@@ -108,7 +110,16 @@ __device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollide
 #pragma unroll
         for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[lat::OPP[q]], m);
     }
-    collide_single<T>(a, P);
+    if (sizeof(C) == sizeof(T)) {
+        collide_single<C>((C*)a, P);
+    } else {   // FP32 storage collided in FP64: each deviation is rounded once, on store
+        C w[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) w[q] = (C)a[q];
+        collide_single<C>(w, P);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) a[q] = (T)w[q];
+    }
     slot_st(d.S[0], m, a[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
@@ -117,11 +128,11 @@ __device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollide
     }
 }
 
-template <bool ODD, class T>
-__global__ void __launch_bounds__(BLOCK) k_sp_collide(SDev<T> d, SPCollideParamsT<T> P, int per) {
+template <bool ODD, class T, class C>
+__global__ void __launch_bounds__(BLOCK) k_sp_collide(SDev<T> d, SPCollideParamsT<C> P, int per) {
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
-    sp_update_node<ODD, T>(d, P, m);
+    sp_update_node<ODD, T, C>(d, P, m);
 }
 
 // dense staging (populations f, FP64) <-> compact storage

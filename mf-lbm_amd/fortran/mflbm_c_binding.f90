@@ -13,6 +13,7 @@ module mflbm_c_binding
         integer(c_int32_t) :: wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max
         integer(c_int32_t) :: inlet_bc, outlet_bc, porous_plate_cmd, z_porous_plate
         integer(c_int32_t) :: n_exclude_inlet, n_exclude_outlet
+        integer(c_int32_t) :: precision   ! 0 FP64 (reference arithmetic), 1 FP32 storage
         real(c_double) :: la_nui1, la_nui2, gamma, beta, force_z, sa_inject, phi_inlet
         real(c_double) :: rho_in, rho_out, uin_avg, relaxation
     end type mflbm_params

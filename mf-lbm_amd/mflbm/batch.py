@@ -27,7 +27,7 @@ import numpy as np
 from . import geometry, io, native
 from .native import i8ptr
 
-PRECISION = {"fp64": 0, "fp32": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "mixed": 2}
 
 
 @dataclasses.dataclass

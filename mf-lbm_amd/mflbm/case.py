@@ -61,7 +61,15 @@ class BoundaryLists:
     f_theta: np.ndarray  # (nf,)
 
 
-def make_params(rp: io.RunParams) -> native.Params:
+PRECISION = {"fp64": 0, "fp32": 1, "mixed": 2}
+
+
+def make_params(rp: io.RunParams, precision: str = "fp64") -> native.Params:
+    """precision "fp64": the reference arithmetic; "fp32": populations stored in FP32 (f + g - REST and
+    f - g) with FP32 collision and recolouring; "mixed": the same storage collided and recoloured in FP64;
+    everything else FP64 in every mode."""
+    if precision not in PRECISION:
+        raise ValueError(f"precision must be one of {sorted(PRECISION)}, got {precision!r}")
     p = native.Params()
     p.nx, p.ny, p.nz = rp.nx, rp.ny, rp.nz
     p.jper, p.kper = rp.jper, rp.kper
@@ -71,6 +79,7 @@ def make_params(rp: io.RunParams) -> native.Params:
     p.inlet_bc, p.outlet_bc = rp.inlet_bc, rp.outlet_bc
     p.porous_plate_cmd, p.z_porous_plate = rp.porous_plate_cmd, rp.z_porous_plate
     p.n_exclude_inlet, p.n_exclude_outlet = rp.n_exclude_inlet, rp.n_exclude_outlet
+    p.precision = PRECISION[precision]
     p.la_nui1 = 1.0 / rp.la_nu1
     p.la_nui2 = 1.0 / rp.la_nu2
     p.gamma, p.beta, p.force_z = rp.gamma, rp.beta, rp.force_z
@@ -91,14 +100,15 @@ class Case:
     instead (geometry_preprocess_cmd 1, Geometry_preprocessing.F90:524-685)."""
 
     def __init__(self, control: Dict[str, object], sample: np.ndarray, geometry: str = "host",
-                 boundary_file: Optional[str] = None):
+                 boundary_file: Optional[str] = None, precision: str = "fp64"):
         lib = native.load()
         self.control = dict(control)
         self.rp = io.derive_params(control)
         rp = self.rp
         nx, ny, nz = rp.nx, rp.ny, rp.nz
         self.walls_global = io.global_walls(sample, nx, ny, nz, rp.wall_x, rp.wall_y, rp.wall_z)
-        self.params = make_params(rp)
+        self.precision = precision
+        self.params = make_params(rp, precision)
         self.walls_h2 = np.zeros((nz + 4, ny + 4, nx + 4), dtype=np.int8)
         wg = np.ascontiguousarray(self.walls_global)
         _check(lib.mflbm_setup_walls(C.byref(self.params), i8ptr(wg), i8ptr(self.walls_h2)), "mflbm_setup_walls")

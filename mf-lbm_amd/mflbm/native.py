@@ -25,7 +25,7 @@ class Params(C.Structure):
     _fields_ = [(n, c_int) for n in (
         "nx", "ny", "nz", "jper", "kper", "wall_x_min", "wall_x_max", "wall_y_min", "wall_y_max",
         "wall_z_min", "wall_z_max", "inlet_bc", "outlet_bc", "porous_plate_cmd", "z_porous_plate",
-        "n_exclude_inlet", "n_exclude_outlet")] + [(n, c_dbl) for n in (
+        "n_exclude_inlet", "n_exclude_outlet", "precision")] + [(n, c_dbl) for n in (
         "la_nui1", "la_nui2", "gamma", "beta", "force_z", "sa_inject", "phi_inlet", "rho_in", "rho_out",
         "uin_avg", "relaxation")]
 
@@ -80,6 +80,7 @@ _ENTRY_POINTS = {
     "upload_geometry": (c_int, [C.c_void_p, P_i8, c_int, P_int, P_int, P_int, P_dbl, c_int, P_int, P_dbl, P_dbl, P_dbl]),
     "upload_state": (c_int, [C.c_void_p, C.POINTER(P_dbl), C.POINTER(P_dbl), P_dbl, P_dbl, P_dbl, P_dbl]),
     "download_state": (c_int, [C.c_void_p, C.POINTER(P_dbl), C.POINTER(P_dbl), P_dbl, P_dbl, P_dbl, P_dbl]),
+    "init_state_device": (c_int, [C.c_void_p, c_int, c_dbl]),
     "step": (c_int, [C.c_void_p, c_int]),
     "run": (c_int, [C.c_void_p, c_int, c_int]),
     "color_gradient": (c_int, [C.c_void_p]),

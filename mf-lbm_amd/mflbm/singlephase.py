@@ -47,12 +47,13 @@ def _check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed with code {rc}")
 
 
-PRECISION = {"fp64": 0, "fp32": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "mixed": 2}
 
 
 def make_sp_params(rp: io.SPRunParams, precision: str = "fp64") -> native.SPParams:
     """precision "fp64": the reference arithmetic; "fp32": populations stored in FP32 as deviations from the
-    rest equilibrium and collided in FP32 (boundaries, macroscopic fields and reductions stay FP64)."""
+    rest equilibrium and collided in FP32; "mixed": the same FP32 storage collided in FP64 (boundaries,
+    macroscopic fields and reductions stay FP64 in every mode)."""
     if precision not in PRECISION:
         raise ValueError(f"precision must be one of {sorted(PRECISION)}, got {precision!r}")
     p = native.SPParams()

