@@ -283,6 +283,39 @@ int mflbm_spb_results(mflbm_spb_ctx* ctx, int32_t* conv_step, double* flowrate, 
 /* restarts the monitor: every sample runs again, period count and previous flow rates cleared */
 int mflbm_spb_reset_monitor(mflbm_spb_ctx* ctx);
 int mflbm_spb_synchronize(mflbm_spb_ctx* ctx);
+/* one sample's velocity (compute_macro_vars, after an even step) in the halo-1 layout of the single-phase
+ * download_macro; the training data of the warm-start network */
+int mflbm_spb_download_velocity(mflbm_spb_ctx* ctx, int32_t sample, double* u, double* v, double* w);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart: the reference starts every run at rest): kernels of a 3-D
+ * convolutional encoder-decoder that maps a wall array to a steady velocity field, trained on fields the
+ * solver produces.  Device pointers; tensors channels-last x[n][z][y][x][c] in BF16 with c a multiple of 8;
+ * 3x3x3 convolutions with zero padding on the matrix cores (BF16 in, FP32 accumulation).  `stream` is a
+ * hipStream_t (NULL = default stream).  Weights: FP32 master w[cout][27][cin]; BF16 rows wf[cout][kpad] for the
+ * forward pass and wb[cin][kpadb] (flipped, channels swapped) for the input gradient, kpad = roundup(27 cin, 32),
+ * kpadb = roundup(27 cout, 32).
+ * --------------------------------------------------------------------------------------------------------- */
+int mflbm_nn_weights_to_bf16(const float* w, void* wf, void* wb, int32_t cin, int32_t cout, void* stream);
+int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* bias, void* y, int32_t n, int32_t d, int32_t h,
+                        int32_t w, int32_t cin, int32_t cout, int32_t relu, void* stream);
+/* accumulates the weight gradient (FP32, rows of 27 cin) and, when dbias is not NULL, the bias gradient */
+int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
+                       int32_t w, int32_t cin, int32_t cout, void* stream);
+int mflbm_nn_relu_bwd(void* g, const void* y, int64_t n, void* stream);
+/* 2x average pooling of the fine tensor (bwd = 0) or its adjoint into the fine gradient (bwd = 1); (d, h, w) fine */
+int mflbm_nn_pool2(const void* x, void* y, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c, int32_t bwd, void* stream);
+/* 2x nearest upsampling into the fine tensor (bwd = 0) or its adjoint into the coarse gradient; (d, h, w) fine */
+int mflbm_nn_up2(const void* x, void* y, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c, int32_t bwd, void* stream);
+int mflbm_nn_add(const void* a, const void* b, void* y, int64_t n, void* stream);
+/* squared error over the first `used` of c channels: g = 2 scale (y - t) (BF16), *loss += sum of squares (FP32) */
+int mflbm_nn_mse(const void* y, const void* t, void* g, float* loss, int64_t npos, int32_t c, int32_t used, float scale,
+                 void* stream);
+int mflbm_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2, float eps,
+                  int32_t step, void* stream);
+/* layout check of the matrix-core instruction: d (16 x 16, FP32) = a (16 x 32, BF16 row-major) times b given
+ * as 16 columns of 32 (BF16) */
+int mflbm_nn_mfma_probe(const void* a, const void* b, float* d, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,436 @@
+// Kernels of the learned warm start: a 3-D convolutional encoder-decoder trained on steady velocity fields
+// that the single-phase solver produces (no reference counterpart; the reference starts every run at rest).
+//
+// Tensors are channels-last, x[n][z][y][x][c] in BF16; channel counts are multiples of 8 so a lane loads 8
+// consecutive channels (16 bytes) at once.  A 3x3x3 convolution (stride 1, zero padding 1) is an implicit GEMM
+// out[p][co] = sum_k A[p][k] B[k][co] with k = off * cin + ci (off = 9 (dz+1) + 3 (dy+1) + (dx+1)), run on the
+// gfx950 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Weights are stored per output channel as
+// rows of kpad = roundup(27 cin, 32) BF16 values (zero beyond 27 cin); the FP32 master copy (rows of 27 cin)
+// lives with the optimiser.  The input gradient of a convolution is the same kernel run on the output gradient with the
+// kernel flipped and its channel roles swapped (mflbm_nn_weights_to_bf16 writes both forms).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mflbm.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NT = 256;
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+__device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
+
+struct Shape {
+    int n, d, h, w;
+    __host__ __device__ __forceinline__ long count() const { return (long)n * d * h * w; }
+};
+
+// Matrix-core fragment layout of v_mfma_f32_16x16x32_bf16 for lane l (64 lanes): A row l % 16 with k
+// 8 (l / 16) .. +7; B column l % 16 with the same k; D holds rows 4 (l / 16) + t, t = 0..3, of column l % 16.
+__device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// 16 x 16 x 32 product of row-major A (16 x 32) and column-major B (columns of 32): the layout check.
+__global__ void k_mfma_probe(const __bf16* __restrict__ a, const __bf16* __restrict__ b, float* __restrict__ d) {
+    const int l = threadIdx.x;
+    const int r = l & 15, q = l >> 4;
+    bf16x8 fa, fb;
+    for (int j = 0; j < 8; ++j) {
+        fa[j] = a[r * 32 + q * 8 + j];
+        fb[j] = b[r * 32 + q * 8 + j];
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(fa, fb, acc);
+    for (int t = 0; t < 4; ++t) d[(q * 4 + t) * 16 + r] = acc[t];
+}
+
+// One wave computes 32 positions x 32 output channels; 4 waves per block along positions.
+template <int RELU>
+__global__ void __launch_bounds__(NT) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ wt,
+                                                 const float* __restrict__ bias, __bf16* __restrict__ y, Shape S,
+                                                 int cin, int cout, int kpad) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long M = S.count();
+    const long p0 = ((long)blockIdx.x * 4 + wv) * 32;
+    const int co0 = blockIdx.y * 32;
+    if (p0 >= M) return;
+    const int r = lane & 15, q = lane >> 4;
+    int pz[2], py[2], px[2], pn[2];
+    bool pv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        long p = p0 + h * 16 + r;
+        pv[h] = p < M;
+        if (!pv[h]) p = 0;
+        px[h] = (int)(p % S.w);
+        p /= S.w;
+        py[h] = (int)(p % S.h);
+        p /= S.h;
+        pz[h] = (int)(p % S.d);
+        pn[h] = (int)(p / S.d);
+    }
+    const int K = 27 * cin;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) acc[h][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero = {};
+    for (int k0 = 0; k0 < kpad; k0 += 32) {
+        const int k = k0 + q * 8;
+        bf16x8 a[2] = {zero, zero}, b[2] = {zero, zero};
+        if (k < K) {
+            const int off = k / cin, ci = k - off * cin;
+            const int dz = off / 9 - 1, dy = (off / 3) % 3 - 1, dx = off % 3 - 1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int zz = pz[h] + dz, yy = py[h] + dy, xx = px[h] + dx;
+                if (pv[h] && zz >= 0 && zz < S.d && yy >= 0 && yy < S.h && xx >= 0 && xx < S.w)
+                    a[h] = *(const bf16x8*)(x + ((((long)pn[h] * S.d + zz) * S.h + yy) * S.w + xx) * cin + ci);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            const int co = co0 + g * 16 + r;
+            if (co < cout) b[g] = *(const bf16x8*)(wt + (long)co * kpad + k);   // zero padded beyond K
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int g = 0; g < 2; ++g) acc[h][g] = mfma(a[h], b[g], acc[h][g]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            const int co = co0 + g * 16 + r;
+            if (co >= cout) continue;
+            const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const long p = p0 + h * 16 + q * 4 + t;
+                if (p >= M) continue;
+                float v = acc[h][g][t] + bb;
+                if (RELU && v < 0.f) v = 0.f;
+                y[p * cout + co] = f2bf(v);
+            }
+        }
+}
+
+// Weight gradient: dw[co][k] += sum_p dy[p][co] * x[p + off(k)][ci(k)] (rows of 27 cin, the layout of the FP32
+// master weights), the positions split into chunks of `chunk` (a multiple of 32) over grid z; one wave computes
+// 32 output channels x 32 k; FP32 atomics.
+__global__ void __launch_bounds__(NT) k_conv_dw(const __bf16* __restrict__ x, const __bf16* __restrict__ dy,
+                                                float* __restrict__ dw, Shape S, int cin, int cout, long chunk) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int co0 = blockIdx.x * 32;
+    const int kk0 = (blockIdx.y * 4 + wv) * 32;
+    const int K = 27 * cin;
+    if (kk0 >= K) return;
+    const long M = S.count();
+    const long pbeg = (long)blockIdx.z * chunk;
+    const long pend = pbeg + chunk < M ? pbeg + chunk : M;
+    // this lane's B columns (k = kk0 + g*16 + r) -> offset and channel
+    int bdz[2], bdy[2], bdx[2], bci[2];
+    bool bv[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const int kcol = kk0 + g * 16 + r;
+        bv[g] = kcol < K;
+        const int off = bv[g] ? kcol / cin : 0;
+        bci[g] = bv[g] ? kcol - off * cin : 0;
+        bdz[g] = off / 9 - 1;
+        bdy[g] = (off / 3) % 3 - 1;
+        bdx[g] = off % 3 - 1;
+    }
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) acc[h][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long pk = pbeg; pk < pend; pk += 32) {
+        const long p = pk + q * 8;   // this lane's 8 positions p .. p+7 (same row: w is a multiple of 8)
+        bf16x8 a[2] = {}, b[2] = {};
+        if (p < pend) {
+            long t = p;
+            const int xq = (int)(t % S.w);
+            t /= S.w;
+            const int yq = (int)(t % S.h);
+            t /= S.h;
+            const int zq = (int)(t % S.d);
+            const int nq = (int)(t / S.d);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int co = co0 + h * 16 + r;
+                if (co < cout)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) a[h][j] = dy[(p + j) * cout + co];
+            }
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                if (!bv[g]) continue;
+                const int zz = zq + bdz[g], yy = yq + bdy[g];
+                if (zz < 0 || zz >= S.d || yy < 0 || yy >= S.h) continue;
+                const long rowb = (((long)nq * S.d + zz) * S.h + yy) * S.w;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int xx = xq + j + bdx[g];
+                    if (xx >= 0 && xx < S.w) b[g][j] = x[(rowb + xx) * cin + bci[g]];
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int g = 0; g < 2; ++g) acc[h][g] = mfma(a[h], b[g], acc[h][g]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            const int kcol = kk0 + g * 16 + r;
+            if (kcol >= K) continue;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int co = co0 + h * 16 + q * 4 + t;
+                if (co < cout) atomicAdd(&dw[(long)co * K + kcol], acc[h][g][t]);
+            }
+        }
+}
+
+// bias gradient: db[co] += sum over positions of dy[p][co]
+__global__ void __launch_bounds__(NT) k_bias_grad(const __bf16* __restrict__ dy, float* __restrict__ db, long M, int cout,
+                                                  long chunk) {
+    const int co = threadIdx.x % 32 + blockIdx.y * 32;
+    const int lane_p = threadIdx.x / 32;
+    if (co >= cout) return;
+    const long pbeg = (long)blockIdx.x * chunk;
+    const long pend = pbeg + chunk < M ? pbeg + chunk : M;
+    float s = 0.f;
+    for (long p = pbeg + lane_p; p < pend; p += NT / 32) s += bf2f(dy[p * cout + co]);
+    atomicAdd(&db[co], s);
+}
+
+__global__ void k_relu_bwd(__bf16* __restrict__ g, const __bf16* __restrict__ y, long n) {
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i < n && !(bf2f(y[i]) > 0.f)) g[i] = f2bf(0.f);
+}
+
+// 2x average pooling (fine shape S, coarse S/2) and its adjoint; nearest 2x upsampling and its adjoint
+__global__ void k_pool2(const __bf16* __restrict__ x, __bf16* __restrict__ y, Shape S, int c, int bwd) {
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    const int D = S.d / 2, H = S.h / 2, W = S.w / 2;
+    const long n = (long)S.n * D * H * W * c;
+    if (i >= n) return;
+    const int ch = (int)(i % c);
+    long t = i / c;
+    const int xx = (int)(t % W);
+    t /= W;
+    const int yy = (int)(t % H);
+    t /= H;
+    const int zz = (int)(t % D);
+    const int nn = (int)(t / D);
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+        const long f = ((((long)nn * S.d + 2 * zz + (o >> 2)) * S.h + 2 * yy + ((o >> 1) & 1)) * S.w + 2 * xx + (o & 1)) * c + ch;
+        if (!bwd) s += bf2f(x[f]);
+        else y[f] = f2bf(0.125f * bf2f(x[i]));   // x: coarse gradient, y: fine gradient
+    }
+    if (!bwd) y[i] = f2bf(0.125f * s);
+}
+__global__ void k_up2(const __bf16* __restrict__ x, __bf16* __restrict__ y, Shape S, int c, int bwd) {
+    // S: fine shape.  forward: y(fine) = x(coarse parent); backward: y(coarse) = sum of x(fine children)
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    const int D = S.d / 2, H = S.h / 2, W = S.w / 2;
+    const long n = (long)S.n * D * H * W * c;
+    if (i >= n) return;
+    const int ch = (int)(i % c);
+    long t = i / c;
+    const int xx = (int)(t % W);
+    t /= W;
+    const int yy = (int)(t % H);
+    t /= H;
+    const int zz = (int)(t % D);
+    const int nn = (int)(t / D);
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+        const long f = ((((long)nn * S.d + 2 * zz + (o >> 2)) * S.h + 2 * yy + ((o >> 1) & 1)) * S.w + 2 * xx + (o & 1)) * c + ch;
+        if (bwd) s += bf2f(x[f]);
+        else y[f] = x[i];
+    }
+    if (bwd) y[i] = f2bf(s);
+}
+__global__ void k_add(const __bf16* __restrict__ a, const __bf16* __restrict__ b, __bf16* __restrict__ y, long n) {
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i < n) y[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+// mean squared error over the first `used` of `c` channels; writes the gradient (BF16) and adds the sum of
+// squares to *loss (FP32)
+__global__ void __launch_bounds__(NT) k_mse(const __bf16* __restrict__ y, const __bf16* __restrict__ t,
+                                            __bf16* __restrict__ g, float* __restrict__ loss, long npos, int c, int used,
+                                            float scale) {
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    __shared__ float red[NT];
+    float s = 0.f;
+    if (i < npos * c) {
+        const int ch = (int)(i % c);
+        const float d = ch < used ? bf2f(y[i]) - bf2f(t[i]) : 0.f;
+        g[i] = f2bf(2.f * scale * d);
+        s = d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+// Adam on FP32 master weights (bias-corrected, Kingma & Ba 2015)
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+// FP32 master weights w[co][27 cin] -> BF16 forward rows wf[co][kpad] and the flipped, transposed rows of the
+// input-gradient convolution wb[ci][kpadb] (wb[ci][off' cout + co] = w[co][(26 - off') cin + ci])
+__global__ void k_weights(const float* __restrict__ w, __bf16* __restrict__ wf, __bf16* __restrict__ wb, int cin,
+                          int cout, int kpad, int kpadb) {
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    const long nf = (long)cout * kpad, nb = (long)cin * kpadb;
+    if (i < nf) {
+        const int co = (int)(i / kpad), k = (int)(i % kpad);
+        wf[i] = f2bf(k < 27 * cin ? w[(long)co * 27 * cin + k] : 0.f);
+    }
+    if (wb && i < nb) {
+        const int ci = (int)(i / kpadb), k = (int)(i % kpadb);
+        float v = 0.f;
+        if (k < 27 * cout) {
+            const int off = k / cout, co = k % cout;
+            v = w[(long)co * 27 * cin + (26 - off) * cin + ci];
+        }
+        wb[i] = f2bf(v);
+    }
+}
+
+inline unsigned blocks(long n) { return (unsigned)((n + NT - 1) / NT); }
+inline int launched(hipStream_t) {
+    return hipGetLastError() == hipSuccess ? MFLBM_OK : MFLBM_ERR_HIP;
+}
+inline int kpad_of(int cin) { return (27 * cin + 31) / 32 * 32; }
+
+}  // namespace
+
+extern "C" int mflbm_nn_mfma_probe(const void* a, const void* b, float* d, void* stream) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, (const __bf16*)a, (const __bf16*)b, d);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* bias, void* y, int32_t n, int32_t d,
+                                   int32_t h, int32_t w, int32_t cin, int32_t cout, int32_t relu, void* stream) {
+    if (!x || !wf || !y || n < 1 || d < 1 || h < 1 || w < 1 || cin < 8 || cin % 8 || cout < 1) return MFLBM_ERR_ARG;
+    const Shape S{n, d, h, w};
+    const long M = S.count();
+    const dim3 grid((unsigned)((M + 127) / 128), (unsigned)((cout + 31) / 32));
+    if (relu)
+        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (const __bf16*)wf,
+                           bias, (__bf16*)y, S, (int)cin, (int)cout, kpad_of(cin));
+    else
+        hipLaunchKernelGGL(k_conv_fwd<0>, grid, dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (const __bf16*)wf,
+                           bias, (__bf16*)y, S, (int)cin, (int)cout, kpad_of(cin));
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
+                                  int32_t w, int32_t cin, int32_t cout, void* stream) {
+    if (!x || !dy || !dw || n < 1 || d < 1 || h < 1 || w < 8 || w % 8 || cin < 8 || cin % 8 || cout < 1)
+        return MFLBM_ERR_ARG;
+    const Shape S{n, d, h, w};
+    const long M = S.count();
+    const long chunk = 4096;
+    const dim3 grid((unsigned)((cout + 31) / 32), (unsigned)((27 * cin + 127) / 128), (unsigned)((M + chunk - 1) / chunk));
+    hipLaunchKernelGGL(k_conv_dw, grid, dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (const __bf16*)dy, dw, S,
+                       (int)cin, (int)cout, chunk);
+    if (dbias) {
+        const long bchunk = 8192;
+        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((M + bchunk - 1) / bchunk), (unsigned)((cout + 31) / 32)), dim3(NT),
+                           0, (hipStream_t)stream, (const __bf16*)dy, dbias, M, (int)cout, bchunk);
+    }
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_weights_to_bf16(const float* w, void* wf, void* wb, int32_t cin, int32_t cout, void* stream) {
+    if (!w || !wf || cin < 1 || cout < 1) return MFLBM_ERR_ARG;
+    const int kp = kpad_of(cin), kpb = kpad_of(cout);
+    const long n = (long)cout * kp > (long)cin * kpb ? (long)cout * kp : (long)cin * kpb;
+    hipLaunchKernelGGL(k_weights, dim3(blocks(n)), dim3(NT), 0, (hipStream_t)stream, w, (__bf16*)wf, (__bf16*)wb, (int)cin,
+                       (int)cout, kp, kpb);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_relu_bwd(void* g, const void* y, int64_t n, void* stream) {
+    if (!g || !y || n < 0) return MFLBM_ERR_ARG;
+    if (n) hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(n)), dim3(NT), 0, (hipStream_t)stream, (__bf16*)g, (const __bf16*)y, (long)n);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_pool2(const void* x, void* y, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c, int32_t bwd,
+                              void* stream) {
+    if (!x || !y || d % 2 || h % 2 || w % 2 || c < 1) return MFLBM_ERR_ARG;
+    const Shape S{n, d, h, w};
+    const long cnt = (long)n * (d / 2) * (h / 2) * (w / 2) * c;
+    hipLaunchKernelGGL(k_pool2, dim3(blocks(cnt)), dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (__bf16*)y, S, (int)c,
+                       (int)bwd);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_up2(const void* x, void* y, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c, int32_t bwd,
+                            void* stream) {
+    if (!x || !y || d % 2 || h % 2 || w % 2 || c < 1) return MFLBM_ERR_ARG;
+    const Shape S{n, d, h, w};
+    const long cnt = (long)n * (d / 2) * (h / 2) * (w / 2) * c;
+    hipLaunchKernelGGL(k_up2, dim3(blocks(cnt)), dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (__bf16*)y, S, (int)c,
+                       (int)bwd);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_add(const void* a, const void* b, void* y, int64_t n, void* stream) {
+    if (!a || !b || !y || n < 0) return MFLBM_ERR_ARG;
+    if (n) hipLaunchKernelGGL(k_add, dim3(blocks(n)), dim3(NT), 0, (hipStream_t)stream, (const __bf16*)a, (const __bf16*)b,
+                              (__bf16*)y, (long)n);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_mse(const void* y, const void* t, void* g, float* loss, int64_t npos, int32_t c, int32_t used,
+                            float scale, void* stream) {
+    if (!y || !t || !g || !loss || npos < 0 || c < 1 || used < 0 || used > c) return MFLBM_ERR_ARG;
+    const long n = (long)npos * c;
+    if (n) hipLaunchKernelGGL(k_mse, dim3(blocks(n)), dim3(NT), 0, (hipStream_t)stream, (const __bf16*)y, (const __bf16*)t,
+                              (__bf16*)g, loss, (long)npos, (int)c, (int)used, scale);
+    return launched((hipStream_t)stream);
+}
+
+extern "C" int mflbm_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                             float eps, int32_t step, void* stream) {
+    if (!w || !g || !m || !v || n < 0 || step < 1) return MFLBM_ERR_ARG;
+    float c1 = 1.f, c2 = 1.f;
+    for (int s = 0; s < step; ++s) {
+        c1 *= b1;
+        c2 *= b2;
+    }
+    if (n) hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(NT), 0, (hipStream_t)stream, w, g, m, v, (long)n, lr, b1, b2,
+                              eps, 1.f - c1, 1.f - c2);
+    return launched((hipStream_t)stream);
+}

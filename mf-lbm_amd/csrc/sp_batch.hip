@@ -629,6 +629,35 @@ extern "C" int mflbm_spb_download_state(mflbm_spb_ctx* c, int32_t sample, double
     return MFLBM_OK;
 }
 
+// velocity of one sample (compute_macro_vars, Misc.F90:368-423; meaningful after even steps) into u, v, w with
+// the halo-1 layout of mflbm_sp_download_macro; solid nodes hold 0
+extern "C" int mflbm_spb_download_velocity(mflbm_spb_ctx* c, int32_t sample, double* u, double* v, double* w) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!sample_ok(c, sample)) return fail(c, MFLBM_ERR_ARG, "no sample %d (or not committed)", sample);
+    int rc = ensure_staging(c);
+    if (rc) return rc;
+    const Layout& L = c->L[sample];
+    const long ps = L.total;
+    double* st = c->staging;
+    HIPCHK(c, hipMemsetAsync(st, 0, sizeof(double) * 4 * ps, c->stream));
+    if (c->fp32) {
+        const SDev<float>& d = c->dev32[sample];
+        LAUNCH(c, k_sp_macro<float>, nblocks(d.nf), d, c->p.force_z, st, st + ps, st + 2 * ps, st + 3 * ps);
+    } else {
+        const SDev<double>& d = c->dev64[sample];
+        LAUNCH(c, k_sp_macro<double>, nblocks(d.nf), d, c->p.force_z, st, st + ps, st + 2 * ps, st + 3 * ps);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> buf(L.total);
+    double* dst[3] = {u, v, w};
+    for (int m = 0; m < 3; ++m) {
+        if (!dst[m]) continue;
+        HIPCHK(c, hipMemcpy(buf.data(), st + m * ps, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, 1, buf.data(), dst[m]);
+    }
+    return MFLBM_OK;
+}
+
 extern "C" int mflbm_spb_run(mflbm_spb_ctx* c, int32_t ntime0, int32_t nsteps) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->committed) return fail(c, MFLBM_ERR_STATE, "not committed");

@@ -216,24 +216,6 @@ __global__ void k_sp_outlet_pressure_after(SDev<T> d, double rho_out) {
 // macroscopic fields and monitor reductions
 // ------------------------------------------------------------------------------------------------------
 
-// compute_macro_vars (Misc.F90:368-423): rho = sum f, u = sum e f - F/2 at fluid nodes.  Solid nodes hold 0
-// (the fields are cleared when the geometry is uploaded and nothing else writes them).
-template <class T>
-__global__ void k_sp_macro(SDev<T> d, double force_z, double* u, double* v, double* w, double* rho) {
-    const int m = blockIdx.x * BLOCK + threadIdx.x;
-    if (m >= d.nf) return;
-    double f[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = Store<T>::get(d.S[q][m], q);
-    const double fx = 0.0, fy = 0.0, fz = force_z;
-    const int o = d.cpos[m];
-    rho[o] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] +
-             f[15] + f[16] + f[17] + f[18];
-    u[o] = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14] - 0.5 * fx;
-    v[o] = f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18] - 0.5 * fy;
-    w[o] = f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18] - 0.5 * fz;
-}
-
 // monitor (Monitor.F90:20-48), one block per z layer: fl(k) = sum of w, pre(k) = sum of rho over the layer,
 // and the layer's max of u^2+v^2+w^2.  The sums are tree reductions (the reference adds in x-fastest order).
 __global__ void __launch_bounds__(BLOCK) k_sp_monitor_layers(Layout L, const double* u, const double* v, const double* w,

@@ -40,7 +40,8 @@ def _run(cmd, cwd=None):
 def build_device(verbose: bool = False) -> str:
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     srcs = [os.path.join(CSRC, "mflbm_device.hip"), os.path.join(CSRC, "sp_device.hip"),
-            os.path.join(CSRC, "sp_batch.hip"), os.path.join(CSRC, "geometry_device.hip"), os.path.join(CSRC, "host_setup.cpp")]
+            os.path.join(CSRC, "sp_batch.hip"), os.path.join(CSRC, "nn_kernels.hip"),
+            os.path.join(CSRC, "geometry_device.hip"), os.path.join(CSRC, "host_setup.cpp")]
     tmp = LIB + ".tmp"
     out = _run([HIPCC, *HIP_FLAGS, f"-I{CSRC}", f"-I{INCLUDE}", *srcs, "-o", tmp])
     os.replace(tmp, LIB)

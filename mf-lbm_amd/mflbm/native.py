@@ -128,6 +128,22 @@ _SPB_ENTRY_POINTS = {
     "results": (c_int, [C.c_void_p, P_int, P_dbl, P_int]),
     "reset_monitor": (c_int, [C.c_void_p]),
     "synchronize": (c_int, [C.c_void_p]),
+    "download_velocity": (c_int, [C.c_void_p, c_int, P_dbl, P_dbl, P_dbl]),
+}
+
+# warm-start network kernels (device pointers as void*, stream as void*)
+_VP = C.c_void_p
+_NN_ENTRY_POINTS = {
+    "mflbm_nn_weights_to_bf16": (c_int, [_VP, _VP, _VP, c_int, c_int, _VP]),
+    "mflbm_nn_conv3d_fwd": (c_int, [_VP, _VP, _VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
+    "mflbm_nn_conv3d_dw": (c_int, [_VP, _VP, _VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
+    "mflbm_nn_relu_bwd": (c_int, [_VP, _VP, C.c_int64, _VP]),
+    "mflbm_nn_pool2": (c_int, [_VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
+    "mflbm_nn_up2": (c_int, [_VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
+    "mflbm_nn_add": (c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
+    "mflbm_nn_mse": (c_int, [_VP, _VP, _VP, _VP, C.c_int64, c_int, c_int, C.c_float, _VP]),
+    "mflbm_nn_adam": (c_int, [_VP, _VP, _VP, _VP, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, c_int, _VP]),
+    "mflbm_nn_mfma_probe": (c_int, [_VP, _VP, _VP, _VP]),
 }
 
 _HOST_ENTRY_POINTS = {
@@ -146,7 +162,7 @@ _HOST_ENTRY_POINTS = {
 
 # every symbol include/mflbm.h declares
 EXPORTED = (["mflbm_" + n for n in _ENTRY_POINTS] + ["mflbm_sp_" + n for n in _SP_ENTRY_POINTS] +
-            ["mflbm_spb_" + n for n in _SPB_ENTRY_POINTS] + list(_HOST_ENTRY_POINTS))
+            ["mflbm_spb_" + n for n in _SPB_ENTRY_POINTS] + list(_HOST_ENTRY_POINTS) + list(_NN_ENTRY_POINTS))
 
 
 def bind(lib: C.CDLL, prefix: str, names=None, single_phase: bool = False, batch: bool = False):
@@ -182,7 +198,7 @@ def load() -> C.CDLL:
             raise RuntimeError(f"{LIB} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
                                "or `python -m mflbm.build`")
         lib = C.CDLL(LIB)
-        for name, (res, args) in _HOST_ENTRY_POINTS.items():
+        for name, (res, args) in list(_HOST_ENTRY_POINTS.items()) + list(_NN_ENTRY_POINTS.items()):
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args

@@ -11,7 +11,7 @@ REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 def declared_symbols():
     with open(os.path.join(REPO, "include", "mflbm.h")) as fh:
         text = fh.read()
-    return sorted(set(re.findall(r"\b(mflbm_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(mflbm_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,137 @@
+"""Warm-start network kernels (mflbm_nn_*) against the CPU oracle (oracle/nn_oracle.py, pinned in
+tests/test_nn_oracle.py): the matrix-core fragment layout, the convolution and its weight gradient, and the
+whole encoder-decoder's loss and gradients; a short training run lowers the loss.
+
+Tolerances: activations are BF16 on both sides with the rounding at the same points; the device accumulates in
+a different order (matrix cores, FP32 atomics), so a value may land one BF16 step (2^-8 relative) away, and
+the differences propagate through the layers: outputs to 3 % of their scale, gradients to 5 % in norm."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
+import nn_oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _t():
+    import torch
+    return torch
+
+
+def _bf(a):
+    torch = _t()
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to("cuda", torch.bfloat16)
+
+
+def test_mfma_fragment_layout():
+    import ctypes as C
+    torch = _t()
+    from mflbm import native
+    lib = native.load()
+    rng = np.random.default_rng(0)
+    a = O.to_bf16(rng.standard_normal((16, 32)).astype(np.float32))
+    b = O.to_bf16(rng.standard_normal((16, 32)).astype(np.float32))   # 16 columns of 32
+    d = torch.zeros((16, 16), device="cuda")
+    ga, gb = _bf(a), _bf(b)
+    assert lib.mflbm_nn_mfma_probe(C.c_void_p(ga.data_ptr()), C.c_void_p(gb.data_ptr()), C.c_void_p(d.data_ptr()), None) == 0
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(d.cpu().numpy(), a @ b.T, rtol=1e-5, atol=1e-5)
+
+
+def _conv_case(cin, cout, shape, seed):
+    rng = np.random.default_rng(seed)
+    x = O.to_bf16(rng.standard_normal(shape + (cin,)).astype(np.float32))
+    w = rng.standard_normal((cout, 27, cin)).astype(np.float32) * np.float32(np.sqrt(2.0 / (27 * cin)))
+    b = rng.standard_normal(cout).astype(np.float32) * 0.1
+    return x, w, b
+
+
+@pytest.mark.parametrize("cin,cout,relu", [(8, 32, True), (32, 8, False), (16, 48, True)])
+def test_conv3d_forward_and_weight_gradient_match_the_oracle(cin, cout, relu):
+    import ctypes as C
+    torch = _t()
+    from mflbm import native
+    from mflbm.warmstart import _kpad
+    lib = native.load()
+    shape = (2, 6, 5, 16)
+    x, w, b = _conv_case(cin, cout, shape, cin + cout)
+    gx, gw_t = _bf(x), torch.from_numpy(w.reshape(cout, -1)).cuda()
+    wf = torch.zeros((cout, _kpad(cin)), dtype=torch.bfloat16, device="cuda")
+    wb = torch.zeros((cin, _kpad(cout)), dtype=torch.bfloat16, device="cuda")
+    P = lambda t: C.c_void_p(t.data_ptr())
+    assert lib.mflbm_nn_weights_to_bf16(P(gw_t), P(wf), P(wb), cin, cout, None) == 0
+    gb_t = torch.from_numpy(b).cuda()
+    y = torch.empty(shape + (cout,), dtype=torch.bfloat16, device="cuda")
+    assert lib.mflbm_nn_conv3d_fwd(P(gx), P(wf), P(gb_t), P(y), *shape, cin, cout, int(relu), None) == 0
+    ref = O.to_bf16(O.conv3d(x, O.to_bf16(w), b, relu))
+    got = y.float().cpu().numpy()
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 2 ** -7 * scale, np.abs(got - ref).max() / scale
+    # weight and bias gradients for an output gradient g, and the input gradient through the flipped weights
+    rng = np.random.default_rng(7)
+    g = O.to_bf16(rng.standard_normal(shape + (cout,)).astype(np.float32))
+    gg = _bf(g)
+    dw = torch.zeros((cout, 27 * cin), device="cuda")
+    db = torch.zeros(cout, device="cuda")
+    assert lib.mflbm_nn_conv3d_dw(P(gx), P(gg), P(dw), P(db), *shape, cin, cout, None) == 0
+    dx = torch.empty(shape + (cin,), dtype=torch.bfloat16, device="cuda")
+    if cout % 8 == 0:
+        assert lib.mflbm_nn_conv3d_fwd(P(gg), P(wb), None, P(dx), *shape, cout, cin, 0, None) == 0
+    torch.cuda.synchronize()
+    rgx, rgw, rgb = O.conv3d_grads(x, O.to_bf16(w), g)
+    np.testing.assert_allclose(dw.cpu().numpy().reshape(cout, 27, cin), rgw, rtol=2e-3, atol=2e-3 * np.abs(rgw).max())
+    np.testing.assert_allclose(db.cpu().numpy(), rgb, rtol=2e-3, atol=2e-3 * np.abs(rgb).max())
+    if cout % 8 == 0:
+        rgx = O.to_bf16(rgx)
+        assert np.abs(dx.float().cpu().numpy() - rgx).max() <= 2 ** -7 * np.abs(rgx).max()
+
+
+def test_network_loss_and_gradients_match_the_oracle():
+    torch = _t()
+    from mflbm.warmstart import WarmStartNet
+    levels = (8, 16)
+    net = WarmStartNet(levels, seed=5)
+    params = net.get_params()
+    rng = np.random.default_rng(9)
+    for n in params:   # non-zero biases so that every ReLU sees both signs
+        params[n] = (params[n][0], (rng.standard_normal(params[n][1].shape) * 0.05).astype(np.float32))
+    net.set_params(params)
+    x = np.zeros((2, 8, 8, 16, 8), np.float32)
+    x[..., 0] = rng.random((2, 8, 8, 16)) > 0.35
+    t = np.zeros((2, 8, 8, 16, 8), np.float32)
+    t[..., :3] = O.to_bf16(rng.standard_normal((2, 8, 8, 16, 3)).astype(np.float32))
+    loss = net.loss_and_backward(_bf(x), _bf(t))
+    orc = O.UNetOracle(params, list(levels), 8, 8)
+    rloss, ry, rg = orc.loss_and_grads(x, t, 3)
+    print(f"[nn gpu vs oracle] loss {loss:.6g} oracle {rloss:.6g}", flush=True)
+    assert abs(loss - rloss) <= 1e-2 * rloss
+    for name, L in net.layers.items():
+        gw = L.gw.cpu().numpy().reshape(L.cout, 27, L.cin)
+        gb = L.gb.cpu().numpy()
+        rw, rb = rg[name]
+        ew = np.linalg.norm(gw - rw) / (np.linalg.norm(rw) + 1e-30)
+        eb = np.linalg.norm(gb - rb) / (np.linalg.norm(rb) + 1e-30)
+        print(f"[nn gpu vs oracle] {name}: |dgw|/|gw| = {ew:.3e}  |dgb|/|gb| = {eb:.3e}", flush=True)
+        assert ew < 5e-2 and eb < 5e-2, name
+
+
+def test_training_lowers_the_loss():
+    torch = _t()
+    from mflbm.warmstart import Dataset, WarmStartNet, train
+    rng = np.random.default_rng(4)
+    n, s = 8, 16
+    x = np.zeros((n, s, s, s, 8), np.float32)
+    x[..., 0] = rng.random((n, s, s, s)) > 0.3
+    t = np.zeros((n, s, s, s, 8), np.float32)
+    # a target the network can represent: a smoothed copy of the indicator in channel 2
+    k = x[..., 0]
+    t[..., 2] = (k + np.roll(k, 1, 1) + np.roll(k, -1, 1) + np.roll(k, 1, 2) + np.roll(k, -1, 2)) / 5.0 * k
+    data = Dataset(_bf(x), _bf(t), np.zeros(n, np.int64), 0.0)
+    net = WarmStartNet((8, 16), seed=1)
+    hist = train(net, data, epochs=30, batch_size=4, lr=3e-3)
+    print(f"[nn training] loss {hist[0]:.4g} -> {hist[-1]:.4g}", flush=True)
+    assert hist[-1] < 0.25 * hist[0]
