@@ -292,6 +292,46 @@ def run_fp32_report(args) -> dict:
             "detail": out}
 
 
+def run_warmstart(args) -> dict:
+    """Learned warm start: steady fields of args.ws_samples seeded 64^3 packs from the batched solver, the
+    encoder-decoder trained on them for args.ws_epochs epochs with the project's kernels, then unseen packs of
+    sides args.ws_eval run to the flow-rate criterion 1e-7 from rest and from the network's field."""
+    import numpy as np
+    from mflbm import batch, warmstart
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    data = warmstart.make_dataset(args.ws_samples, side=64, seed=11)
+    log(f"dataset: {args.ws_samples} fields in {data.seconds:.1f}s, steps median {int(np.median(data.conv_steps))}")
+    net = warmstart.WarmStartNet(seed=0)
+    t1 = time.perf_counter()
+    hist = warmstart.train(net, data, epochs=args.ws_epochs, batch_size=4, lr=1e-3, log=log)
+    import torch
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    iters = args.ws_epochs * (args.ws_samples // 4)
+    results = []
+    for k, side in enumerate(int(v) for v in args.ws_eval.split(",")):
+        s = batch.rev_samples(1, seed=90001 + k, sides=(side,), porosity=(0.25, 0.25))[0]
+        r = warmstart.compare_runs(net, s)
+        log(f"{side}^3: steps cold {r.steps_cold} warm {r.steps_warm}, k rel err {r.k_rel_err:.2e}, "
+            f"time cold {r.seconds_cold:.2f}s warm {r.seconds_warm + r.seconds_predict:.2f}s")
+        results.append({"side": side, "steps_cold": r.steps_cold, "steps_warm": r.steps_warm,
+                        "time_to_converged_cold_s": round(r.seconds_cold, 3),
+                        "time_to_converged_warm_s": round(r.seconds_warm + r.seconds_predict, 3),
+                        "predict_s": round(r.seconds_predict, 3), "k_cold": r.k_cold, "k_warm": r.k_warm,
+                        "k_rel_err": r.k_rel_err})
+    return {"metric": "time-to-converged permeability with and without the learned warm start (single phase, FP64 runs)",
+            "unit": "s", "value": results[0]["time_to_converged_warm_s"] if results else None,
+            "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+            "config": {"workload": f"{args.ws_samples} training fields (64^3 sphere packs, porosity 0.15-0.35, "
+                                   f"batched solver, criterion 1e-6), {args.ws_epochs} epochs, BF16 encoder-decoder "
+                                   "(4 levels, 32 base channels); evaluation on unseen packs, criterion 1e-7"},
+            "training": {"data_seconds": round(data.seconds, 1), "train_seconds": round(t_train, 1),
+                         "iterations": iters, "loss_first_epoch": hist[0] if hist else None,
+                         "loss_last_epoch": hist[-1] if hist else None},
+            "results": results, "total_seconds": round(time.perf_counter() - t0, 1)}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,9 +340,13 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report"), default="twophase")
+    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report", "warmstart"),
+                    default="twophase")
     ap.add_argument("--batch-samples", type=int, default=96)
     ap.add_argument("--report-steps", default="5000,10000,20000")
+    ap.add_argument("--ws-samples", type=int, default=256, help="warm start: training fields (the full set is 2000)")
+    ap.add_argument("--ws-epochs", type=int, default=8)
+    ap.add_argument("--ws-eval", default="128", help="warm start: sides of the unseen evaluation packs")
     ap.add_argument("--drop-steps", type=int, default=4000)
     ap.add_argument("--batch-max-steps", type=int, default=100000)
     ap.add_argument("--precision", choices=("fp64", "fp32", "mixed"), default="fp64",
@@ -312,8 +356,8 @@ def main() -> None:
         print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
               "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    run = {"singlephase": run_singlephase, "batch": run_batch, "fp32report": run_fp32_report}.get(args.workload,
-                                                                                                 run_twophase)
+    run = {"singlephase": run_singlephase, "batch": run_batch, "fp32report": run_fp32_report,
+           "warmstart": run_warmstart}.get(args.workload, run_twophase)
     out = run(args)
     print(json.dumps(out))
 
