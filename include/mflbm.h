@@ -255,7 +255,9 @@ typedef struct mflbm_spb_params {
     int32_t ntime_monitor;     /* steps per convergence test (even: the flow rate is read after even steps) */
     int32_t tile_rows;         /* compact node order: padded y rows per tile (0: MFLBM_TILE_Y or 32) */
     int32_t block;             /* threads per block of the lattice update: 64, 128, 256 or 512 (0: 256) */
-    int32_t reserved;          /* 0 */
+    int32_t rest_state;        /* initial state at rest: 0 the lattice weights (initialization_new), 1 the MRT
+                                  collision's rest equilibrium (4/21, 13/126, 1/63), which starts without the
+                                  density waves the lattice weights launch at solid links */
     double la_nu;              /* fluid_viscosity */
     double force_z;            /* body force along z */
     double tol;                /* convergence_criteria: |Q - Q_prev| / (|Q| + 1e-14) < tol */
@@ -283,9 +285,9 @@ int mflbm_spb_results(mflbm_spb_ctx* ctx, int32_t* conv_step, double* flowrate, 
 /* restarts the monitor: every sample runs again, period count and previous flow rates cleared */
 int mflbm_spb_reset_monitor(mflbm_spb_ctx* ctx);
 int mflbm_spb_synchronize(mflbm_spb_ctx* ctx);
-/* one sample's velocity (compute_macro_vars, after an even step) in the halo-1 layout of the single-phase
- * download_macro; the training data of the warm-start network */
-int mflbm_spb_download_velocity(mflbm_spb_ctx* ctx, int32_t sample, double* u, double* v, double* w);
+/* one sample's u, v, w, rho (compute_macro_vars, after an even step; any pointer may be NULL) in the halo-1
+ * layout of mflbm_sp_download_macro; the training data of the warm-start network */
+int mflbm_spb_download_macro(mflbm_spb_ctx* ctx, int32_t sample, double* u, double* v, double* w, double* rho);
 
 /* ---------------------------------------------------------------------------------------------------------
  * Learned warm start (no reference counterpart: the reference starts every run at rest): kernels of a 3-D

@@ -326,12 +326,13 @@ void free_all(mflbm_spb_ctx* c) {
 
 template <class T>
 int init_state_t(mflbm_spb_ctx* c) {
-    // fluid at rest with density 1 (initialization_new with u = 0, Initialization.F90:207-307): f_q = w_q at
-    // every slot, link slots included
+    // fluid at rest with density 1 at every slot, link slots included: f_q = w_q as initialization_new with
+    // u = 0 (Initialization.F90:207-307), or with rest_state = 1 the collision's own rest state REST_q
     for (size_t s = 0; s < c->host.size(); ++s)
         for (int q = 0; q < 19; ++q) {
             const long n = c->soff[20 * s + q + 1] - c->soff[20 * s + q];
-            const T v = (T)(c->fp32 ? lat::W[q] - rest_weight(q) : lat::W[q]);
+            const double f0 = c->p.rest_state ? rest_weight(q) : lat::W[q];
+            const T v = (T)(c->fp32 ? f0 - rest_weight(q) : f0);
             T* p = (T*)c->pop + c->elem_off[s] + c->soff[20 * s + q];
             LAUNCH(c, k_spb_fill<T>, nblocks(n), p, n, v);
         }
@@ -367,6 +368,7 @@ extern "C" int mflbm_spb_create(const mflbm_spb_params* prm, mflbm_spb_ctx** out
     if (prm->block != 0 && prm->block != 64 && prm->block != 128 && prm->block != 256 && prm->block != 512)
         return fail(nullptr, MFLBM_ERR_ARG, "block must be 0, 64, 128, 256 or 512");
     if (prm->tile_rows < 0) return fail(nullptr, MFLBM_ERR_ARG, "tile_rows must be >= 0");
+    if (prm->rest_state != 0 && prm->rest_state != 1) return fail(nullptr, MFLBM_ERR_ARG, "rest_state must be 0 or 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, MFLBM_ERR_NODEV, "no HIP device visible");
     mflbm_spb_ctx* c = new (std::nothrow) mflbm_spb_ctx();
@@ -616,7 +618,9 @@ extern "C" int mflbm_spb_download_state(mflbm_spb_ctx* c, int32_t sample, double
     if (rc) return rc;
     const Layout& L = c->L[sample];
     // slots outside the compact storage come back as the initial state (fluid at rest)
-    for (int q = 0; q < 19; ++q) LAUNCH(c, k_spb_fill<double>, nblocks(L.total), c->staging + q * L.total, L.total, lat::W[q]);
+    for (int q = 0; q < 19; ++q)
+        LAUNCH(c, k_spb_fill<double>, nblocks(L.total), c->staging + q * L.total, L.total,
+               c->p.rest_state ? rest_weight(q) : lat::W[q]);
     rc = c->fp32 ? scatter_sample<float>(c, sample) : scatter_sample<double>(c, sample);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -629,9 +633,10 @@ extern "C" int mflbm_spb_download_state(mflbm_spb_ctx* c, int32_t sample, double
     return MFLBM_OK;
 }
 
-// velocity of one sample (compute_macro_vars, Misc.F90:368-423; meaningful after even steps) into u, v, w with
-// the halo-1 layout of mflbm_sp_download_macro; solid nodes hold 0
-extern "C" int mflbm_spb_download_velocity(mflbm_spb_ctx* c, int32_t sample, double* u, double* v, double* w) {
+// macroscopic fields of one sample (compute_macro_vars, Misc.F90:368-423; meaningful after even steps) into u,
+// v, w, rho (each may be NULL) with the halo-1 layout of mflbm_sp_download_macro; solid nodes hold 0
+extern "C" int mflbm_spb_download_macro(mflbm_spb_ctx* c, int32_t sample, double* u, double* v, double* w,
+                                        double* rho) {
     if (!c) return MFLBM_ERR_ARG;
     if (!sample_ok(c, sample)) return fail(c, MFLBM_ERR_ARG, "no sample %d (or not committed)", sample);
     int rc = ensure_staging(c);
@@ -649,8 +654,8 @@ extern "C" int mflbm_spb_download_velocity(mflbm_spb_ctx* c, int32_t sample, dou
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
-    double* dst[3] = {u, v, w};
-    for (int m = 0; m < 3; ++m) {
+    double* dst[4] = {u, v, w, rho};
+    for (int m = 0; m < 4; ++m) {
         if (!dst[m]) continue;
         HIPCHK(c, hipMemcpy(buf.data(), st + m * ps, sizeof(double) * L.total, hipMemcpyDeviceToHost));
         unpack_host(L, 1, buf.data(), dst[m]);

@@ -32,7 +32,7 @@ module mflbm_c_binding
 
     ! batched single-phase runs (no reference counterpart; many samples per job)
     type, bind(c) :: mflbm_spb_params
-        integer(c_int32_t) :: precision, mrt_preset, ntime_monitor, tile_rows, block, reserved
+        integer(c_int32_t) :: precision, mrt_preset, ntime_monitor, tile_rows, block, rest_state
         real(c_double) :: la_nu, force_z, tol
     end type mflbm_spb_params
 

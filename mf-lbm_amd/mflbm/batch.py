@@ -107,7 +107,9 @@ class REVBatch:
 
     def __init__(self, samples: Sequence[REVSample], la_nu: float = 0.1, force_z: float = 1e-6,
                  precision: str = "fp32", mrt_preset: int = 1, ntime_monitor: int = 500, tol: float = 1e-7,
-                 tile_rows: int = 0, block: int = 0):
+                 tile_rows: int = 0, block: int = 0, rest_state: int = 0):
+        """rest_state 1 starts every sample at the collision's rest equilibrium instead of the lattice weights
+        (SPCase.collision_equilibrium_state)."""
         if precision not in PRECISION:
             raise ValueError(f"precision must be one of {sorted(PRECISION)}")
         self.lib = native.load()
@@ -116,7 +118,7 @@ class REVBatch:
         self.la_nu, self.force_z = la_nu, force_z
         p = native.SPBParams()
         p.precision, p.mrt_preset, p.ntime_monitor = PRECISION[precision], mrt_preset, ntime_monitor
-        p.tile_rows, p.block, p.reserved = tile_rows, block, 0
+        p.tile_rows, p.block, p.rest_state = tile_rows, block, rest_state
         p.la_nu, p.force_z, p.tol = la_nu, force_z, tol
         self.params = p
         self.ctx = C.c_void_p()

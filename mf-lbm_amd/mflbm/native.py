@@ -41,7 +41,7 @@ class SPParams(C.Structure):
 
 class SPBParams(C.Structure):
     """mflbm_spb_params (batched single-phase runs)"""
-    _fields_ = [(n, c_int) for n in ("precision", "mrt_preset", "ntime_monitor", "tile_rows", "block", "reserved")] + \
+    _fields_ = [(n, c_int) for n in ("precision", "mrt_preset", "ntime_monitor", "tile_rows", "block", "rest_state")] + \
         [(n, c_dbl) for n in ("la_nu", "force_z", "tol")]
 
 
@@ -128,7 +128,7 @@ _SPB_ENTRY_POINTS = {
     "results": (c_int, [C.c_void_p, P_int, P_dbl, P_int]),
     "reset_monitor": (c_int, [C.c_void_p]),
     "synchronize": (c_int, [C.c_void_p]),
-    "download_velocity": (c_int, [C.c_void_p, c_int, P_dbl, P_dbl, P_dbl]),
+    "download_macro": (c_int, [C.c_void_p, c_int, P_dbl, P_dbl, P_dbl, P_dbl]),
 }
 
 # warm-start network kernels (device pointers as void*, stream as void*)
@@ -197,7 +197,10 @@ def load() -> C.CDLL:
         if not os.path.exists(LIB):
             raise RuntimeError(f"{LIB} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
                                "or `python -m mflbm.build`")
-        lib = C.CDLL(LIB)
+        # RTLD_GLOBAL: the library's HIP and HSA runtimes (ROCm's) join the global symbol scope, so a torch
+        # imported later binds its own HIP runtime to the same HSA runtime instead of starting a second one,
+        # which could not take the GPU's virtual address space this process already holds
+        lib = C.CDLL(LIB, mode=C.RTLD_GLOBAL)
         for name, (res, args) in list(_HOST_ENTRY_POINTS.items()) + list(_NN_ENTRY_POINTS.items()):
             fn = getattr(lib, name)
             fn.restype = res

@@ -49,6 +49,16 @@ def _check(rc: int, what: str) -> None:
 
 PRECISION = {"fp64": 0, "fp32": 1, "mixed": 2}
 
+# lattice weights of the reference's equilibrium (singlephase_3D/0.src/Module.F90:93-98) and the rest state of
+# its MRT collision: density 1, energy moment -11, energy-square moment 0, every other moment 0, mapped back to
+# populations by the transform coefficients c1 = 1/19, c2 = 1/2394, c3 = 1/252 (Module.F90:116-125) with the
+# same double operations the device evaluates, so the values are bit-identical to the device's (4/21, 13/126,
+# 1/63 to rounding)
+LATTICE_W = (1.0 / 3.0,) + (1.0 / 18.0,) * 6 + (1.0 / 36.0,) * 12
+_C1, _C2, _C3, _E2C1 = 1.0 / 19.0, 1.0 / 2394.0, 1.0 / 252.0, 0.0
+REST = ((_C1 + 330.0 * _C2 + 12.0 * _C3 * _E2C1,) + (_C1 + 121.0 * _C2 - 4.0 * _C3 * _E2C1,) * 6 +
+        (_C1 - 88.0 * _C2 + _C3 * _E2C1,) * 12)
+
 
 def make_sp_params(rp: io.SPRunParams, precision: str = "fp64") -> native.SPParams:
     """precision "fp64": the reference arithmetic; "fp32": populations stored in FP32 as deviations from the
@@ -122,6 +132,25 @@ class SPCase:
         _check(lib.mflbm_sp_init_fields(C.byref(self.params), args[0], args[1], args[2], args[3],
                                         native.ptr_array(list(st.f)), dptr(fc) if fc is not None else None),
                "mflbm_sp_init_fields")
+        return st
+
+    def collision_equilibrium_state(self, rho: Optional[np.ndarray] = None,
+                                    velocity: Optional[tuple] = None) -> SPState:
+        """Initial populations at the MRT collision's own equilibrium instead of the lattice-weight one.
+
+        The reference's new-run state (``initial_state``, Initialization.F90:244-276) is the second-order
+        equilibrium with the lattice weights (1/3, 1/18, 1/36), whose energy-square moment is 3 rho, while the
+        collision relaxes that moment to 0 (e2 equilibrium without a density term): the rest state of the
+        collision is REST = (4/21, 13/126, 1/63).  The first steps of a reference run therefore move mass
+        between directions; at solid links this launches density waves of order 0.1, and their decay, not the
+        flow's development, sets the time a run needs to meet the flow-rate criterion.  This state adds
+        (REST_q - w_q) rho to every population (as (f_q - w_q rho) + REST_q rho, which is REST_q exactly at
+        rest), so the rest state is a fixed point of the collision and a prescribed (rho, u) starts without
+        that transient."""
+        st = self.initial_state(rho=rho, velocity=velocity)
+        r = np.ones_like(st.f[0]) if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        for q in range(19):
+            st.f[q] = (st.f[q] - LATTICE_W[q] * r) + REST[q] * r
         return st
 
 

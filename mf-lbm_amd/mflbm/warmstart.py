@@ -1,9 +1,9 @@
 """Learned warm start for single-phase permeability runs (no reference counterpart: the reference starts every
 run from rest, singlephase_3D/0.src/Initialization.F90:207-307).
 
-A 3-D convolutional encoder-decoder maps a wall array to the steady velocity field of the body-force-driven
-flow; the field seeds the equilibrium populations of a new run (``SPCase.initial_state(velocity=...)``), and the
-unchanged solver then converges to the reference's criterion.  The network is trained on the GPU by the
+A 3-D convolutional encoder-decoder maps a wall array to the steady velocity and density fields of the
+body-force-driven flow; the fields seed the populations of a new run at the collision's own equilibrium
+(``SPCase.collision_equilibrium_state``), and the unchanged solver then converges to the reference's criterion.  The network is trained on the GPU by the
 project's own kernels (``mflbm_nn_*`` of include/mflbm.h: BF16 matrix-core convolutions with FP32 accumulation,
 FP32 Adam master weights) on fields the batched solver produces (``mflbm/batch.py``).
 
@@ -11,7 +11,9 @@ Network (``layer_specs``): ``levels`` = (32, 64, 128, 256) channels; each encode
 convolutions then 2x average pooling; each decoder level nearest 2x upsampling, a ReLU convolution, the skip
 connection added, a second ReLU convolution; a linear 3x3x3 head.  Input channel 0 is the fluid indicator
 (1 fluid, 0 solid); output channels 0..2 are (u, v, w) * nu / F (a length squared, O(1) in pores of a few
-lattice units); channels are padded to multiples of 8.  torch provides device memory and the stream only.
+lattice units) and channel 3 is (rho - mean rho) / (P_SCALE F) (pressure differences across a pore of size d are
+about 3 F d in lattice units); channels are padded to multiples of 8.  torch provides device memory and the
+stream only.
 """
 from __future__ import annotations
 
@@ -28,7 +30,8 @@ from . import batch, native
 LEVELS = (32, 64, 128, 256)
 CIN = 8
 COUT = 8
-USED = 3
+USED = 4
+P_SCALE = 30.0
 
 
 def layer_specs(levels: Sequence[int], cin: int = CIN, cout: int = COUT) -> List[Tuple[str, int, int, bool]]:
@@ -165,7 +168,7 @@ class WarmStartNet:
         return gx
 
     def loss_and_backward(self, x, t) -> float:
-        """Forward, mean squared error over the first 3 output channels (per position), backward into the
+        """Forward, mean squared error over the first USED output channels (per position), backward into the
         FP32 weight gradients; returns the loss."""
         torch = self.torch
         y = self.forward(x)
@@ -242,7 +245,7 @@ class WarmStartNet:
 
     # -- use ------------------------------------------------------------------------------------------------
     def predict(self, fluid: np.ndarray) -> np.ndarray:
-        """fluid indicator (nz, ny, nx) -> normalized velocity (3, nz, ny, nx) (FP32 numpy), zero at solids."""
+        """fluid indicator (nz, ny, nx) -> normalized (u, v, w, p) (4, nz, ny, nx) (FP32 numpy), zero at solids."""
         torch = self.torch
         nz, ny, nx = fluid.shape
         x = torch.zeros((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
@@ -264,40 +267,50 @@ class Dataset:
     t: object          # (n, s, s, s, 8) BF16 on the device
     conv_steps: np.ndarray
     seconds: float
+    target_rms: Optional[np.ndarray] = None   # per output channel, over fluid positions
 
 
 def make_dataset(n: int, side: int = 64, seed: int = 11, la_nu: float = 0.1, force: float = 1e-6,
                  tol: float = 1e-6, max_steps: int = 60000, per_batch: int = 256,
                  porosity: Tuple[float, float] = (0.15, 0.35)) -> Dataset:
     """n steady fields of seeded sphere packs (side^3, z-periodic, x/y walls), solved by REVBatch in the mixed
-    precision mode to the flow-rate criterion ``tol``; targets (u, v, w) * nu / F."""
+    precision mode to the flow-rate criterion ``tol``; targets (u, v, w) * nu / F and (rho - mean) / (P_SCALE F)."""
     torch = _torch()
     x = torch.zeros((n, side, side, side, CIN), dtype=torch.bfloat16, device="cuda")
     t = torch.zeros((n, side, side, side, COUT), dtype=torch.bfloat16, device="cuda")
     steps = np.zeros(n, np.int64)
+    sq = np.zeros(USED)
+    npos = 0
     t0 = time.perf_counter()
     done = 0
     k = 0
     while done < n:
         m = min(per_batch, n - done)
         samples = batch.rev_samples(m, seed=seed + 7919 * k, sides=(side,), porosity=porosity)
-        b = batch.REVBatch(samples, la_nu, force, "mixed", ntime_monitor=200, tol=tol)
+        # started at the collision's rest state: no start-up density waves, so every pore, connected or not,
+        # keeps density 1 on average and the density target is the flow's own pressure field
+        b = batch.REVBatch(samples, la_nu, force, "mixed", ntime_monitor=200, tol=tol, rest_state=1)
         r = b.run_steady(max_steps)
         for i, s in enumerate(samples):
             nz, ny, nx = s.walls.shape
-            u, v, w = (np.zeros((nz + 2, ny + 2, nx + 2)) for _ in range(3))
-            b._call("download_velocity", i, native.dptr(u), native.dptr(v), native.dptr(w))
+            u, v, w, rho = (np.zeros((nz + 2, ny + 2, nx + 2)) for _ in range(4))
+            b._call("download_macro", i, native.dptr(u), native.dptr(v), native.dptr(w), native.dptr(rho))
             fl = (batch.global_walls(s) == 0).astype(np.float32)
             j = done + i
             x[j, ..., 0] = torch.from_numpy(fl).to("cuda", torch.bfloat16)
-            vel = np.stack([a[1:-1, 1:-1, 1:-1] for a in (u, v, w)], axis=-1) * (la_nu / force)
-            t[j, ..., :USED] = torch.from_numpy(vel.astype(np.float32)).to("cuda", torch.bfloat16)
+            tgt = np.stack([a[1:-1, 1:-1, 1:-1] for a in (u, v, w, rho)], axis=-1)
+            tgt[..., :3] *= la_nu / force
+            dens = tgt[..., 3]
+            tgt[..., 3] = np.where(fl > 0, (dens - dens[fl > 0].mean()) / (P_SCALE * force), 0.0)
+            t[j, ..., :USED] = torch.from_numpy(tgt.astype(np.float32)).to("cuda", torch.bfloat16)
+            sq += (tgt[fl > 0] ** 2).sum(axis=0)
+            npos += int((fl > 0).sum())
             steps[j] = r.conv_step[i]
         b.close()
         done += m
         k += 1
     torch.cuda.synchronize()
-    return Dataset(x, t, steps, time.perf_counter() - t0)
+    return Dataset(x, t, steps, time.perf_counter() - t0, np.sqrt(sq / max(npos, 1)))
 
 
 def train(net: WarmStartNet, data: Dataset, epochs: int, batch_size: int = 4, lr: float = 1e-3,
@@ -323,52 +336,76 @@ def train(net: WarmStartNet, data: Dataset, epochs: int, batch_size: int = 4, lr
     return hist
 
 
+def warm_state(net: WarmStartNet, case, la_nu: float, force: float):
+    """Initial state of ``case`` (an SPCase) from the network: u = prediction * F / nu, rho = 1 + P_SCALE F p
+    with p shifted to zero mean over the fluid, populations at the collision's equilibrium."""
+    fl = (case.walls_global == 0)
+    y = net.predict(fl.astype(np.float32)).astype(np.float64)
+    vel = [np.pad(y[m] * (force / la_nu), 1) for m in range(3)]
+    p = y[3] - y[3][fl].mean()
+    rho = np.pad(np.where(fl, 1.0 + P_SCALE * force * p, 1.0), 1, constant_values=1.0)
+    return case.collision_equilibrium_state(rho=rho, velocity=vel)
+
+
+@dataclasses.dataclass
+class RunRecord:
+    steps: int
+    seconds: float
+    permeability: float
+
+
 @dataclasses.dataclass
 class WarmStartResult:
+    """One unseen sample run three ways to the same criterion: the plain path (the reference's initial state,
+    ``plain_precision``), from rest at the collision's equilibrium and from the network's fields (both in
+    ``precision``)."""
     side: int
-    steps_cold: int
-    steps_warm: int
-    k_cold: float
-    k_warm: float
-    seconds_cold: float
-    seconds_warm: float
+    plain: RunRecord
+    cold: RunRecord
+    warm: RunRecord
     seconds_predict: float
 
     @property
     def k_rel_err(self) -> float:
-        return abs(self.k_warm / self.k_cold - 1.0)
+        """permeability of the warm-started run against the plain path's"""
+        return abs(self.warm.permeability / self.plain.permeability - 1.0)
+
+    @property
+    def time_ratio(self) -> float:
+        """wall time of the warm-started run (prediction included) over the plain path's"""
+        return (self.warm.seconds + self.seconds_predict) / self.plain.seconds
 
 
 def compare_runs(net: WarmStartNet, sample: batch.REVSample, la_nu: float = 0.1, force: float = 1e-6,
-                 tol: float = 1e-7, precision: str = "fp64", max_steps: int = 400000,
-                 ntime_monitor: int = 500) -> WarmStartResult:
-    """The same sample from rest and from the network's field, each run to the reference's criterion."""
+                 tol: float = 1e-7, precision: str = "mixed", plain_precision: str = "fp64",
+                 max_steps: int = 2000000, ntime_monitor: int = 500) -> WarmStartResult:
+    """The same sample run to the reference's criterion from the reference's initial state, from rest at the
+    collision's equilibrium and from the network's fields."""
     from .singlephase import SPCase, SPSimulation, permeability
     nx, ny, nz = sample.shape
     ctl = {"lattice_dimensions": (nx, ny, nz), "periodic_indicator": (0, sample.jper, 1),
            "domain_wall_status_y": (0, 0) if sample.jper else (1, 1), "domain_wall_status_z": (0, 0),
            "MPI_async_layers_num": (0, 4 if sample.jper else 0, 4), "fluid_viscosity": la_nu, "body_force_0": force,
            "MRT_collision_parameter_preset": 1, "steady_state_option": 1, "convergence_criteria": tol}
-    case = SPCase(ctl, sample.walls, precision)
-    res = {}
-    for mode in ("cold", "warm"):
-        tp = 0.0
+    runs = {}
+    tp = 0.0
+    for mode in ("plain", "cold", "warm"):
+        case = SPCase(ctl, sample.walls, plain_precision if mode == "plain" else precision)
         if mode == "warm":
             t0 = time.perf_counter()
-            fl = (case.walls_global == 0).astype(np.float32)
-            u = net.predict(fl) * (force / la_nu)
+            st = warm_state(net, case, la_nu, force)
             tp = time.perf_counter() - t0
-            vel = [np.pad(u[m].astype(np.float64), 1) for m in range(3)]
-            st = case.initial_state(velocity=vel)
+        elif mode == "cold":
+            st = case.collision_equilibrium_state()
         else:
             st = case.initial_state()
         sim = SPSimulation(case, st)
+        sim.synchronize()
         t0 = time.perf_counter()
         n, status, hist = sim.run_steady(1, max_steps, ntime_monitor)
         dt = time.perf_counter() - t0
         sim.close()
         if status != "converged":
             raise RuntimeError(f"{mode} run did not converge in {max_steps} steps ({status})")
-        res[mode] = (n, permeability(case, hist[-1][1]), dt, tp)
-    return WarmStartResult(nx, res["cold"][0], res["warm"][0], res["cold"][1], res["warm"][1], res["cold"][2],
-                           res["warm"][2], res["warm"][3])
+        runs[mode] = RunRecord(n, dt, permeability(case, hist[-1][1]))
+    return WarmStartResult(nx, runs["plain"], runs["cold"], runs["warm"], tp)

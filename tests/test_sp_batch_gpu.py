@@ -54,6 +54,26 @@ def test_batch_matches_single_sample_runs_bit_for_bit(precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_rest_state_start_matches_the_collision_equilibrium_state(precision):
+    """rest_state = 1: every sample starts at the collision's rest equilibrium, as a single-sample run started
+    from SPCase.collision_equilibrium_state()."""
+    samples = _samples()[:2]
+    print(f"[sp batch rest state] precision={precision}", flush=True)
+    b = REVBatch(samples, NU, FORCE, precision, rest_state=1)
+    b.run(1, 23)
+    for i, s in enumerate(samples):
+        case = _case(s, precision)
+        sim = SPSimulation(case, case.collision_equilibrium_state())
+        sim.run(1, 23)
+        fs, fb = sim.state().f, b.state(i)
+        fl = global_walls(s) == 0
+        for q in range(19):
+            assert np.array_equal(fb[q, 1:-1, 1:-1, 1:-1][fl], fs[q, 1:-1, 1:-1, 1:-1][fl]), (i, q)
+        sim.close()
+    b.close()
+
+
 def test_batch_steady_runs_stop_where_the_single_sample_runs_stop():
     samples = _samples()
     tol, nm = 1e-6, 100

@@ -70,3 +70,39 @@ def test_time_step_commutes_with_the_rest_shift(preset):
     c = SPOracleSimulation(case, SPState(W[:, None, None, None] + d, np.zeros((19, rp.ny + 2, rp.nx + 2))))
     c.run(1, 6)
     assert np.abs(c.state().f - W[:, None, None, None] - fb).max() > 1e-3
+
+
+def test_collision_equilibrium_state_starts_a_walled_pack_without_a_transient():
+    """SPCase.collision_equilibrium_state: a fluid at rest among solid walls stays at rest (the reference's
+    lattice-weight start launches density waves at every wall link), and a prescribed (rho, u) is carried by
+    the populations' moments."""
+    from mflbm import singlephase
+    assert np.allclose(singlephase.REST, REST, rtol=0, atol=1e-16)
+    assert np.allclose(singlephase.LATTICE_W, W, rtol=0, atol=1e-16)
+    n = 12
+    walls = (np.random.default_rng(8).random((n, n, n)) < 0.3).astype(np.int8)   # rough pore walls
+    ctl = {"lattice_dimensions": (n, n, n), "periodic_indicator": (0, 0, 1), "domain_wall_status_y": (1, 1),
+           "domain_wall_status_z": (0, 0), "MPI_async_layers_num": (0, 0, 4), "fluid_viscosity": 0.1,
+           "body_force_0": 0.0, "MRT_collision_parameter_preset": 1}
+    case = SPCase(ctl, walls)
+    umax = {}
+    for name, st in (("reference", case.initial_state()), ("equilibrium", case.collision_equilibrium_state())):
+        sim = SPOracleSimulation(case, st)
+        sim.run(1, 40)
+        m = sim.macro()
+        umax[name] = max(np.abs(m[c]).max() for c in ("u", "v", "w"))
+        sim.close()
+    assert umax["reference"] > 1e-3
+    assert umax["equilibrium"] < 1e-14
+    # moments of a prescribed state: density and momentum as given (incompressible form, u = sum e f)
+    rng = np.random.default_rng(3)
+    sh = (n + 2, n + 2, n + 2)
+    rho = 1.0 + 1e-4 * rng.standard_normal(sh)
+    vel = [1e-4 * rng.standard_normal(sh) for _ in range(3)]
+    f = np.array(case.collision_equilibrium_state(rho=rho, velocity=vel).f)
+    ex = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
+    ey = np.array([0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1])
+    ez = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1])
+    np.testing.assert_allclose(f.sum(axis=0), rho, rtol=0, atol=1e-14)
+    for e, v in zip((ex, ey, ez), vel):
+        np.testing.assert_allclose(np.tensordot(e, f, 1), rho * v, rtol=0, atol=1e-14)
