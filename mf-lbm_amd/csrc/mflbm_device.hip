@@ -797,22 +797,16 @@ __device__ __forceinline__ double dz4(const double* A, int n, int px, int pxy) {
                           A[n - px + pxy] - A[n + px - pxy]);
 }
 
-// phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30).
-// All 18 neighbour values are loaded before the masked sum (every list node lies within two layers of the
-// interior, so its neighbours are inside the padded array): the loads are then in flight together instead of
-// one per masked branch; the sum keeps the reference's order over q.
+// phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30)
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= s.n) return;
     const int n = s.pos[t];
     const unsigned mk = s.mask[t];
-    double v[19];
-#pragma unroll
-    for (int q = 1; q < 19; ++q) v[q] = d.phi[n + d.L.off(q)];
     double acc = 0.0;
 #pragma unroll
     for (int q = 1; q < 19; ++q)
-        if (mk & (1u << q)) acc = acc + v[q] * lat::W[q];
+        if (mk & (1u << q)) acc = acc + d.phi[n + d.L.off(q)] * lat::W[q];
     d.phi[n] = acc / s.w[t];
 }
 // wetting: turn the normal at a fluid node next to the wall to the contact angle (Akai et al. 2018;
@@ -879,21 +873,14 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     if (t >= s.n) return;
     const int n = s.pos[t];
     const unsigned mk = s.mask[t];
-    double vx[19], vy[19], vz[19];   // loaded together, summed under the mask (as k_phi_extrapolate)
-#pragma unroll
-    for (int q = 1; q < 19; ++q) {
-        const int o = n + d.L.off(q);
-        vx[q] = d.cnx[o];
-        vy[q] = d.cny[o];
-        vz[q] = d.cnz[o];
-    }
     double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
     for (int q = 1; q < 19; ++q)
         if (mk & (1u << q)) {
-            ax = ax + vx[q] * lat::W[q];
-            ay = ay + vy[q] * lat::W[q];
-            az = az + vz[q] * lat::W[q];
+            const int o = n + d.L.off(q);
+            ax = ax + d.cnx[o] * lat::W[q];
+            ay = ay + d.cny[o] * lat::W[q];
+            az = az + d.cnz[o] * lat::W[q];
         }
     d.cnx[n] = ax / s.w[t];
     d.cny[n] = ay / s.w[t];

@@ -60,6 +60,31 @@ REST = ((_C1 + 330.0 * _C2 + 12.0 * _C3 * _E2C1,) + (_C1 + 121.0 * _C2 - 4.0 * _
         (_C1 - 88.0 * _C2 + _C3 * _E2C1,) * 12)
 
 
+_EX = (0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0)
+_EY = (0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1)
+_EZ = (0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1)
+_OPP = (0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15)
+
+
+def _bounce_back_solid_slots(f: np.ndarray, walls_global: np.ndarray) -> None:
+    """Solid slot (y, q) facing a fluid node x = y + e_q takes x's population opp(q), in place.
+
+    A fluid node pulls its population q from that slot at the first (odd) step; under bounce-back it is the
+    population x sent toward the wall, f_opp(q)(x).  A started flow whose wall slots hold the rest state instead
+    gets a momentum defect of the order of the flow itself at every wall link, and the pressure waves that
+    defect launches decay as slowly as a start from rest.  f: (19, nz+2, ny+2, nx+2), interior at [1:-1]."""
+    fl = np.zeros(f.shape[1:], dtype=bool)
+    fl[1:-1, 1:-1, 1:-1] = walls_global == 0
+    nz, ny, nx = fl.shape
+    for q in range(1, 19):
+        ez, ey, ex = _EZ[q], _EY[q], _EX[q]
+        # y ranges over the nodes whose neighbour y + e_q lies inside the array
+        ys = tuple(slice(max(0, -e), n - max(0, e)) for e, n in ((ez, nz), (ey, ny), (ex, nx)))
+        xs = tuple(slice(max(0, e), n - max(0, -e)) for e, n in ((ez, nz), (ey, ny), (ex, nx)))
+        m = ~fl[ys] & fl[xs]
+        f[q][ys][m] = f[_OPP[q]][xs][m]
+
+
 def make_sp_params(rp: io.SPRunParams, precision: str = "fp64") -> native.SPParams:
     """precision "fp64": the reference arithmetic; "fp32": populations stored in FP32 as deviations from the
     rest equilibrium and collided in FP32; "mixed": the same FP32 storage collided in FP64 (boundaries,
@@ -151,6 +176,8 @@ class SPCase:
         r = np.ones_like(st.f[0]) if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
         for q in range(19):
             st.f[q] = (st.f[q] - LATTICE_W[q] * r) + REST[q] * r
+        if velocity is not None or rho is not None:
+            _bounce_back_solid_slots(st.f, self.walls_global)
         return st
 
 

@@ -9,8 +9,9 @@ FP32 Adam master weights) on fields the batched solver produces (``mflbm/batch.p
 
 Network (``layer_specs``): ``levels`` = (32, 64, 128, 256) channels; each encoder level two 3x3x3 ReLU
 convolutions then 2x average pooling; each decoder level nearest 2x upsampling, a ReLU convolution, the skip
-connection added, a second ReLU convolution; a linear 3x3x3 head.  Input channel 0 is the fluid indicator
-(1 fluid, 0 solid); output channels 0..2 are (u, v, w) * nu / F (a length squared, O(1) in pores of a few
+connection added, a second ReLU convolution; a linear 3x3x3 head.  Input channels (``features``): the fluid
+indicator (1 fluid, 0 solid), the distance to the nearest solid node d / 8 and (d / 8)^2 (the local duct-flow
+scale of the velocity), d capped at 16 and measured through the periodic faces; output channels 0..2 are (u, v, w) * nu / F (a length squared, O(1) in pores of a few
 lattice units) and channel 3 is (rho - mean rho) / (P_SCALE F) (pressure differences across a pore of size d are
 about 3 F d in lattice units); channels are padded to multiples of 8.  torch provides device memory and the
 stream only.
@@ -32,6 +33,22 @@ CIN = 8
 COUT = 8
 USED = 4
 P_SCALE = 30.0
+
+
+NFEAT = 3
+DCAP = 16
+
+
+def features(fluid: np.ndarray, periodic: Tuple[bool, bool, bool] = (True, False, False)) -> np.ndarray:
+    """(nz, ny, nx) bool -> (nz, ny, nx, NFEAT) FP32: indicator, d / 8, (d / 8)^2 with d the Euclidean distance
+    to the nearest solid node (0 at solids), capped at DCAP; periodic axes are wrapped by DCAP layers first."""
+    from scipy import ndimage
+    pad = [(DCAP, DCAP) if p else (0, 0) for p in periodic]
+    fp = np.pad(fluid, pad, mode="wrap") if any(periodic) else fluid
+    d = ndimage.distance_transform_edt(fp)
+    d = d[tuple(slice(DCAP, -DCAP) if p else slice(None) for p in periodic)]
+    d = np.minimum(d, DCAP) / 8.0
+    return np.stack([fluid.astype(np.float32), d.astype(np.float32), (d * d).astype(np.float32)], axis=-1)
 
 
 def layer_specs(levels: Sequence[int], cin: int = CIN, cout: int = COUT) -> List[Tuple[str, int, int, bool]]:
@@ -244,12 +261,13 @@ class WarmStartNet:
         self.set_params({n: (z[f"{n}_w"], z[f"{n}_b"]) for n in self.layers})
 
     # -- use ------------------------------------------------------------------------------------------------
-    def predict(self, fluid: np.ndarray) -> np.ndarray:
-        """fluid indicator (nz, ny, nx) -> normalized (u, v, w, p) (4, nz, ny, nx) (FP32 numpy), zero at solids."""
+    def predict(self, fluid: np.ndarray, periodic: Tuple[bool, bool, bool] = (True, False, False)) -> np.ndarray:
+        """fluid indicator (nz, ny, nx) -> normalized (u, v, w, p) (4, nz, ny, nx) (FP32 numpy), zero at solids;
+        ``periodic`` = (z, y, x) periodicity of the sample for the distance features."""
         torch = self.torch
         nz, ny, nx = fluid.shape
         x = torch.zeros((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
-        x[..., 0] = torch.from_numpy(np.ascontiguousarray(fluid, np.float32)).to("cuda", torch.bfloat16)
+        x[0, ..., :NFEAT] = torch.from_numpy(features(fluid > 0, periodic)).to("cuda", torch.bfloat16)
         with torch.no_grad():
             y = self.forward(x)
         self._acts = None
@@ -297,7 +315,7 @@ def make_dataset(n: int, side: int = 64, seed: int = 11, la_nu: float = 0.1, for
             b._call("download_macro", i, native.dptr(u), native.dptr(v), native.dptr(w), native.dptr(rho))
             fl = (batch.global_walls(s) == 0).astype(np.float32)
             j = done + i
-            x[j, ..., 0] = torch.from_numpy(fl).to("cuda", torch.bfloat16)
+            x[j, ..., :NFEAT] = torch.from_numpy(features(fl > 0, (True, bool(s.jper), False))).to("cuda", torch.bfloat16)
             tgt = np.stack([a[1:-1, 1:-1, 1:-1] for a in (u, v, w, rho)], axis=-1)
             tgt[..., :3] *= la_nu / force
             dens = tgt[..., 3]
@@ -313,21 +331,40 @@ def make_dataset(n: int, side: int = 64, seed: int = 11, la_nu: float = 0.1, for
     return Dataset(x, t, steps, time.perf_counter() - t0, np.sqrt(sq / max(npos, 1)))
 
 
+def _mirror(x, t, axis: int, channel: int):
+    """A sample mirrored across x (axis 3, channel 0 = u) or y (axis 2, channel 1 = v): the walls stay walls,
+    the mirrored steady flow is the steady flow of the mirrored pack with that velocity component negated."""
+    x, t = x.flip(axis), t.flip(axis)
+    t[..., channel] = -t[..., channel]
+    return x, t
+
+
 def train(net: WarmStartNet, data: Dataset, epochs: int, batch_size: int = 4, lr: float = 1e-3,
-          seed: int = 0, log=None) -> List[float]:
-    """Adam over shuffled mini-batches; returns the mean loss of each epoch."""
+          seed: int = 0, log=None, lr_final: Optional[float] = None, mirror: bool = True) -> List[float]:
+    """Adam over shuffled mini-batches, the step size decaying from lr to lr_final (default lr / 10) on a
+    cosine over the run; with ``mirror`` each mini-batch is mirrored across x and/or y at random; returns the
+    mean loss of each epoch."""
     torch = net.torch
     n = data.x.shape[0]
     rng = np.random.default_rng(seed)
+    lr_final = lr / 10.0 if lr_final is None else lr_final
+    total = max(1, epochs * (n // batch_size))
+    it = 0
     hist = []
     for e in range(epochs):
         order = rng.permutation(n)
         tot, cnt = 0.0, 0
         for s in range(0, n - batch_size + 1, batch_size):
             idx = torch.from_numpy(order[s:s + batch_size]).to("cuda")
-            loss = net.loss_and_backward(data.x.index_select(0, idx).contiguous(),
-                                         data.t.index_select(0, idx).contiguous())
-            net.adam(lr)
+            xb, tb = data.x.index_select(0, idx), data.t.index_select(0, idx)
+            if mirror:
+                if rng.random() < 0.5:
+                    xb, tb = _mirror(xb, tb, 3, 0)
+                if rng.random() < 0.5:
+                    xb, tb = _mirror(xb, tb, 2, 1)
+            loss = net.loss_and_backward(xb.contiguous(), tb.contiguous())
+            net.adam(lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / total)))
+            it += 1
             tot += loss
             cnt += 1
         hist.append(tot / max(cnt, 1))
@@ -340,7 +377,7 @@ def warm_state(net: WarmStartNet, case, la_nu: float, force: float):
     """Initial state of ``case`` (an SPCase) from the network: u = prediction * F / nu, rho = 1 + P_SCALE F p
     with p shifted to zero mean over the fluid, populations at the collision's equilibrium."""
     fl = (case.walls_global == 0)
-    y = net.predict(fl.astype(np.float32)).astype(np.float64)
+    y = net.predict(fl.astype(np.float32), (bool(case.rp.kper), bool(case.rp.jper), False)).astype(np.float64)
     vel = [np.pad(y[m] * (force / la_nu), 1) for m in range(3)]
     p = y[3] - y[3][fl].mean()
     rho = np.pad(np.where(fl, 1.0 + P_SCALE * force * p, 1.0), 1, constant_values=1.0)
