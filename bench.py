@@ -306,7 +306,7 @@ def run_warmstart(args) -> dict:
     from mflbm import batch, warmstart
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    data = warmstart.make_dataset(args.ws_samples, side=64, seed=11)
+    data = warmstart.make_dataset(args.ws_samples, side=64, seed=11, log=log)
     log(f"dataset: {args.ws_samples} fields in {data.seconds:.1f}s, steps median {int(np.median(data.conv_steps))}, "
         f"target rms (u, v, w, p) {np.round(data.target_rms, 4).tolist()}")
     net = warmstart.WarmStartNet(seed=0)

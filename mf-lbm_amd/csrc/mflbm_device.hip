@@ -456,9 +456,8 @@ __device__ __forceinline__ void set_inlet_phi(const Dev& d, int i, int j, double
 
 // inlet_bounce_back_velocity_BC_before_odd (Boundary_multiphase_inlet.F90:6-53)
 template <class T>
-__global__ void k_inlet_velocity_before(Dev d, double phi_inlet, double sa_inject, double relaxation) {
+__device__ __forceinline__ void bc_inlet_velocity_before(const Dev& d, int t, double phi_inlet, double sa_inject, double relaxation) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     const int wi = d.walls[L.at(i, j, 1)];
@@ -481,9 +480,8 @@ __global__ void k_inlet_velocity_before(Dev d, double phi_inlet, double sa_injec
 }
 // inlet_bounce_back_velocity_BC_after_odd (Boundary_multiphase_inlet.F90:56-102)
 template <class T>
-__global__ void k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject, double relaxation) {
+__device__ __forceinline__ void bc_inlet_velocity_after(const Dev& d, int t, double phi_inlet, double sa_inject, double relaxation) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     const int wi = d.walls[L.at(i, j, 1)];
@@ -506,9 +504,8 @@ __global__ void k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject
 }
 // inlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_inlet.F90:115-209)
 template <class T>
-__global__ void k_inlet_pressure_before(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
+__device__ __forceinline__ void bc_inlet_pressure_before(const Dev& d, int t, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     const int wi = d.walls[L.at(i, j, 1)];
@@ -539,9 +536,8 @@ __global__ void k_inlet_pressure_before(Dev d, double phi_inlet, double sa_injec
 }
 // inlet_Zou_He_pressure_BC_after_odd (Boundary_multiphase_inlet.F90:212-294)
 template <class T>
-__global__ void k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
+__device__ __forceinline__ void bc_inlet_pressure_after(const Dev& d, int t, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
     const int wi = d.walls[L.at(i, j, 1)];
@@ -568,9 +564,8 @@ __global__ void k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject
 
 // outlet_convective_BC_before_odd (Boundary_multiphase_outlet.F90:7-65)
 template <class T>
-__global__ void k_outlet_convective_before(Dev d, double u_convec) {
+__device__ __forceinline__ void bc_outlet_convective_before(const Dev& d, int t, double u_convec) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const double temp = 1.0 / (1.0 + u_convec);
@@ -605,9 +600,8 @@ __global__ void k_outlet_convective_before(Dev d, double u_convec) {
 }
 // outlet_convective_BC_after_odd (Boundary_multiphase_outlet.F90:68-127)
 template <class T>
-__global__ void k_outlet_convective_after(Dev d, double u_convec) {
+__device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, double u_convec) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const double temp = 1.0 / (1.0 + u_convec);
@@ -640,9 +634,8 @@ __global__ void k_outlet_convective_after(Dev d, double u_convec) {
 }
 // outlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_outlet.F90:139-228)
 template <class T>
-__global__ void k_outlet_pressure_before(Dev d, double rho_out) {
+__device__ __forceinline__ void bc_outlet_pressure_before(const Dev& d, int t, double rho_out) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
@@ -680,9 +673,8 @@ __global__ void k_outlet_pressure_before(Dev d, double rho_out) {
 }
 // outlet_Zou_He_pressure_BC_after_odd (Boundary_multiphase_outlet.F90:231-311)
 template <class T>
-__global__ void k_outlet_pressure_after(Dev d, double rho_out) {
+__device__ __forceinline__ void bc_outlet_pressure_after(const Dev& d, int t, double rho_out) {
     const Layout L = d.L;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
@@ -711,6 +703,71 @@ __global__ void k_outlet_pressure_after(Dev d, double rho_out) {
         A(12, o) = (A(13, o) - lat::ONE6 * s - tnx) * om + A(12, o) * wd;
         A(15, o) = (A(18, o) - lat::ONE6 * s + tny) * om + A(15, o) * wd;
         A(16, o) = (A(17, o) - lat::ONE6 * s - tny) * om + A(16, o) * wd;
+    }
+}
+
+// one launch per boundary condition (the porous-plate and short-domain cases)
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_inlet_velocity_before(Dev d, double phi_inlet, double sa_inject, double relaxation) {
+    bc_inlet_velocity_before<T>(d, blockIdx.x * BLOCK + threadIdx.x, phi_inlet, sa_inject, relaxation);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_inlet_velocity_after(Dev d, double phi_inlet, double sa_inject, double relaxation) {
+    bc_inlet_velocity_after<T>(d, blockIdx.x * BLOCK + threadIdx.x, phi_inlet, sa_inject, relaxation);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_inlet_pressure_before(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
+    bc_inlet_pressure_before<T>(d, blockIdx.x * BLOCK + threadIdx.x, phi_inlet, sa_inject, rho_in, relaxation);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_inlet_pressure_after(Dev d, double phi_inlet, double sa_inject, double rho_in, double relaxation) {
+    bc_inlet_pressure_after<T>(d, blockIdx.x * BLOCK + threadIdx.x, phi_inlet, sa_inject, rho_in, relaxation);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_outlet_convective_before(Dev d, double u_convec) {
+    bc_outlet_convective_before<T>(d, blockIdx.x * BLOCK + threadIdx.x, u_convec);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_outlet_convective_after(Dev d, double u_convec) {
+    bc_outlet_convective_after<T>(d, blockIdx.x * BLOCK + threadIdx.x, u_convec);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_outlet_pressure_before(Dev d, double rho_out) {
+    bc_outlet_pressure_before<T>(d, blockIdx.x * BLOCK + threadIdx.x, rho_out);
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_outlet_pressure_after(Dev d, double rho_out) {
+    bc_outlet_pressure_after<T>(d, blockIdx.x * BLOCK + threadIdx.x, rho_out);
+}
+
+// Inlet and outlet conditions of one step in one launch: blocks [0, gface) run the inlet condition, blocks
+// [gface, 2 gface) the outlet condition.  They touch disjoint planes (k <= 2 and k >= nz - 1; the launcher
+// requires nz >= 8), so running them together equals the reference's inlet-then-outlet order; both are
+// latency-bound plane kernels, and one launch overlaps them.  IN / OUT: 1 velocity / convective, 2 pressure.
+struct FaceArgs {
+    double phi_inlet, sa_inject, relaxation, rho_in, u_convec, rho_out;
+};
+template <class T, int IN, int OUT, bool AFTER>
+__global__ void __launch_bounds__(BLOCK) k_open_faces(Dev d, FaceArgs a, int gface) {
+    const int b = (int)blockIdx.x;
+    if (b < gface) {
+        const int t = b * BLOCK + (int)threadIdx.x;
+        if (IN == 1) {
+            if (AFTER) bc_inlet_velocity_after<T>(d, t, a.phi_inlet, a.sa_inject, a.relaxation);
+            else bc_inlet_velocity_before<T>(d, t, a.phi_inlet, a.sa_inject, a.relaxation);
+        } else {
+            if (AFTER) bc_inlet_pressure_after<T>(d, t, a.phi_inlet, a.sa_inject, a.rho_in, a.relaxation);
+            else bc_inlet_pressure_before<T>(d, t, a.phi_inlet, a.sa_inject, a.rho_in, a.relaxation);
+        }
+    } else {
+        const int t = (b - gface) * BLOCK + (int)threadIdx.x;
+        if (OUT == 1) {
+            if (AFTER) bc_outlet_convective_after<T>(d, t, a.u_convec);
+            else bc_outlet_convective_before<T>(d, t, a.u_convec);
+        } else {
+            if (AFTER) bc_outlet_pressure_after<T>(d, t, a.rho_out);
+            else bc_outlet_pressure_before<T>(d, t, a.rho_out);
+        }
     }
 }
 
@@ -1225,7 +1282,19 @@ int launch_faces(mflbm_ctx* c, bool odd) {
         LAUNCH(c, k_wrap_phi, nblocks((long)L.nx * (L.ny + 8) * (L.nz + 8)), c->d, p.jper, p.kper);
     }
     const unsigned gface = nblocks((long)L.nx * L.ny);
-    if (open_z(p)) {
+    const bool both = (p.inlet_bc == 1 || p.inlet_bc == 2) && (p.outlet_bc == 1 || p.outlet_bc == 2);
+    if (open_z(p) && both && L.nz >= 8) {
+        const FaceArgs a{p.phi_inlet, p.sa_inject, p.relaxation, p.rho_in, p.uin_avg, p.rho_out};
+        const int g = (int)gface;
+#define MFLBM_FACES(IN, OUT)                                                                                    \
+    if (odd) LAUNCH(c, (k_open_faces<T, IN, OUT, true>), 2 * gface, c->d, a, g);                                \
+    else LAUNCH(c, (k_open_faces<T, IN, OUT, false>), 2 * gface, c->d, a, g);
+        if (p.inlet_bc == 1 && p.outlet_bc == 1) { MFLBM_FACES(1, 1) }
+        else if (p.inlet_bc == 1) { MFLBM_FACES(1, 2) }
+        else if (p.outlet_bc == 1) { MFLBM_FACES(2, 1) }
+        else { MFLBM_FACES(2, 2) }
+#undef MFLBM_FACES
+    } else if (open_z(p)) {
         if (!odd) {
             if (p.inlet_bc == 1) LAUNCH(c, k_inlet_velocity_before<T>, gface, c->d, p.phi_inlet, p.sa_inject, p.relaxation);
             else if (p.inlet_bc == 2) LAUNCH(c, k_inlet_pressure_before<T>, gface, c->d, p.phi_inlet, p.sa_inject, p.rho_in, p.relaxation);

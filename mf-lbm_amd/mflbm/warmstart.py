@@ -290,7 +290,7 @@ class Dataset:
 
 def make_dataset(n: int, side: int = 64, seed: int = 11, la_nu: float = 0.1, force: float = 1e-6,
                  tol: float = 1e-6, max_steps: int = 60000, per_batch: int = 256,
-                 porosity: Tuple[float, float] = (0.15, 0.35)) -> Dataset:
+                 porosity: Tuple[float, float] = (0.15, 0.35), log=None) -> Dataset:
     """n steady fields of seeded sphere packs (side^3, z-periodic, x/y walls), solved by REVBatch in the mixed
     precision mode to the flow-rate criterion ``tol``; targets (u, v, w) * nu / F and (rho - mean) / (P_SCALE F)."""
     torch = _torch()
@@ -327,6 +327,8 @@ def make_dataset(n: int, side: int = 64, seed: int = 11, la_nu: float = 0.1, for
         b.close()
         done += m
         k += 1
+        if log:
+            log(f"dataset: {done}/{n} fields, {time.perf_counter() - t0:.1f}s")
     torch.cuda.synchronize()
     return Dataset(x, t, steps, time.perf_counter() - t0, np.sqrt(sq / max(npos, 1)))
 
