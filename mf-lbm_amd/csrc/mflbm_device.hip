@@ -56,6 +56,11 @@ struct Dev {
     double* cny;
     double* cnz;
     double* cnorm;
+    // FP32 storage modes: the normal field in FP32 (cn_x, cn_y, cn_z, c_norm), the same dense layout
+    float* cnx32;
+    float* cny32;
+    float* cnz32;
+    float* cnorm32;
     double* curv;     // dense curvature, filled only for diagnostics
     const int8_t* walls;
     double* w_in;     // (nx+2)(ny+2), reference layout
@@ -165,6 +170,7 @@ __device__ __forceinline__ void st_slot(double* slice, unsigned m, double v) {
 #ifndef MFLBM_FUSED_CURVATURE
 #define MFLBM_FUSED_CURVATURE 1
 #endif
+template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n);
 
 // ODD = false: kernel_even_color (Kernel_multiphase.F90:371-725): read swapped own slots, write own slots.
@@ -181,7 +187,7 @@ k_collide(Dev d, CollideParams P, int per) {
 #if MFLBM_FUSED_CURVATURE
     const int nn = d.cpos[m];
     const double cnx = d.cnx[nn], cny = d.cny[nn], cnz = d.cnz[nn];
-    const double tf = 0.5 * P.gamma * curvature_at(d, nn) * d.cnorm[nn];
+    const double tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
     // keep the stencil phase and the population phase apart so their registers are not live together
     __builtin_amdgcn_sched_barrier(0);
 #else
@@ -237,8 +243,8 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
-    const C cnx = (C)d.cnx[nn], cny = (C)d.cny[nn], cnz = (C)d.cnz[nn];
-    const C tf = (C)(0.5 * gamma * curvature_at(d, nn) * d.cnorm[nn]);
+    const C cnx = (C)d.cnx32[nn], cny = (C)d.cny32[nn], cnz = (C)d.cnz32[nn];
+    const C tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
     __builtin_amdgcn_sched_barrier(0);
     C s[19], a[19];
     unsigned nb[19];
@@ -838,21 +844,41 @@ struct NodeList {
 };
 
 // ISO4 first derivatives with the reference's term order (Phase_gradient.F90:40-62)
-__device__ __forceinline__ double dx4(const double* A, int n, int px, int pxy) {
+template <class T>
+__device__ __forceinline__ double dx4(const T* A, int n, int px, int pxy) {
     return lat::ISO4_1 * (A[n + 1] - A[n - 1]) +
            lat::ISO4_2 * (A[n + 1 + px] - A[n - 1 - px] + A[n + 1 - px] - A[n - 1 + px] + A[n + 1 + pxy] - A[n - 1 - pxy] +
                           A[n + 1 - pxy] - A[n - 1 + pxy]);
 }
-__device__ __forceinline__ double dy4(const double* A, int n, int px, int pxy) {
+template <class T>
+__device__ __forceinline__ double dy4(const T* A, int n, int px, int pxy) {
     return lat::ISO4_1 * (A[n + px] - A[n - px]) +
            lat::ISO4_2 * (A[n + 1 + px] - A[n - 1 - px] + A[n - 1 + px] - A[n + 1 - px] + A[n + px + pxy] - A[n - px - pxy] +
                           A[n + px - pxy] - A[n - px + pxy]);
 }
-__device__ __forceinline__ double dz4(const double* A, int n, int px, int pxy) {
+template <class T>
+__device__ __forceinline__ double dz4(const T* A, int n, int px, int pxy) {
     return lat::ISO4_1 * (A[n + pxy] - A[n - pxy]) +
            lat::ISO4_2 * (A[n + 1 + pxy] - A[n - 1 - pxy] + A[n - 1 + pxy] - A[n + 1 - pxy] + A[n + px + pxy] - A[n - px - pxy] +
                           A[n - px + pxy] - A[n + px - pxy]);
 }
+
+// The normal field: FP64 (the reference arithmetic) or, in the FP32 storage modes, FP32 (the gradient and
+// curvature are still evaluated in FP64 from the stored values; the FP32 field halves the bytes the chain and
+// the lattice update's curvature stencil move).
+template <class NT>
+struct Normals {
+    NT* x;
+    NT* y;
+    NT* z;
+    NT* n;
+};
+template <class NT>
+__device__ __forceinline__ Normals<NT> normals(const Dev& d);
+template <>
+__device__ __forceinline__ Normals<double> normals<double>(const Dev& d) { return {d.cnx, d.cny, d.cnz, d.cnorm}; }
+template <>
+__device__ __forceinline__ Normals<float> normals<float>(const Dev& d) { return {d.cnx32, d.cny32, d.cnz32, d.cnorm32}; }
 
 // phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30)
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per) {
@@ -898,6 +924,7 @@ __device__ __forceinline__ void wet(double& cx, double& cy, double& cz, double n
 // the node's own normal, so doing it here equals the reference's separate pass).  wb[t] indexes the fluid
 // boundary list or is -1.  Solid nodes keep 0 or their extrapolated value, as after the reference's
 // zero-then-extrapolate sequence.
+template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl, int per) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= nl.n) return;
@@ -919,15 +946,18 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
         const int b = wb[t];
         if (b >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * b], fl.nw[3 * b + 1], fl.nw[3 * b + 2], fl.cos_t[b], fl.sin_t[b]);
     }
-    d.cnx[n] = cx;
-    d.cny[n] = cy;
-    d.cnz[n] = cz;
-    d.cnorm[n] = cn;
+    const Normals<NT> N = normals<NT>(d);
+    N.x[n] = (NT)cx;
+    N.y[n] = (NT)cy;
+    N.z[n] = (NT)cz;
+    N.n[n] = (NT)cn;
 }
 // normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110)
+template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, int per) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= s.n) return;
+    const Normals<NT> N = normals<NT>(d);
     const int n = s.pos[t];
     const unsigned mk = s.mask[t];
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -935,42 +965,47 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     for (int q = 1; q < 19; ++q)
         if (mk & (1u << q)) {
             const int o = n + d.L.off(q);
-            ax = ax + d.cnx[o] * lat::W[q];
-            ay = ay + d.cny[o] * lat::W[q];
-            az = az + d.cnz[o] * lat::W[q];
+            ax = ax + (double)N.x[o] * lat::W[q];
+            ay = ay + (double)N.y[o] * lat::W[q];
+            az = az + (double)N.z[o] * lat::W[q];
         }
-    d.cnx[n] = ax / s.w[t];
-    d.cny[n] = ay / s.w[t];
-    d.cnz[n] = az / s.w[t];
+    N.x[n] = (NT)(ax / s.w[t]);
+    N.y[n] = (NT)(ay / s.w[t]);
+    N.z[n] = (NT)(az / s.w[t]);
 }
+template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
+    const Normals<NT> N = normals<NT>(d);
     const int px = d.L.px, pxy = d.L.pxy;
-    const double kxx = dx4(d.cnx, n, px, pxy);
-    const double kyy = dy4(d.cny, n, px, pxy);
-    const double kzz = dz4(d.cnz, n, px, pxy);
-    const double kxy = dy4(d.cnx, n, px, pxy);
-    const double kxz = dz4(d.cnx, n, px, pxy);
-    const double kyx = dx4(d.cny, n, px, pxy);
-    const double kyz = dz4(d.cny, n, px, pxy);
-    const double kzx = dx4(d.cnz, n, px, pxy);
-    const double kzy = dy4(d.cnz, n, px, pxy);
-    const double cx = d.cnx[n], cy = d.cny[n], cz = d.cnz[n];
+    const double kxx = dx4(N.x, n, px, pxy);
+    const double kyy = dy4(N.y, n, px, pxy);
+    const double kzz = dz4(N.z, n, px, pxy);
+    const double kxy = dy4(N.x, n, px, pxy);
+    const double kxz = dz4(N.x, n, px, pxy);
+    const double kyx = dx4(N.y, n, px, pxy);
+    const double kyz = dz4(N.y, n, px, pxy);
+    const double kzx = dx4(N.z, n, px, pxy);
+    const double kzy = dy4(N.z, n, px, pxy);
+    const double cx = (double)N.x[n], cy = (double)N.y[n], cz = (double)N.z[n];
     return (cx * cx - 1.0) * kxx + (cy * cy - 1.0) * kyy + (cz * cz - 1.0) * kzz + cx * cy * (kxy + kyx) +
            cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
 }
 // curvature at the fluid nodes (Phase_gradient.F90:112-201), emitted with the normal as the per-node force
 // input of the next lattice update: (cn_x, cn_y, cn_z, 0.5*gamma*curv*c_norm), Kernel_multiphase.F90:112-116
+template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_curvature(Dev d, double gamma, int per) {
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
+    const Normals<NT> N = normals<NT>(d);
     const int n = d.cpos[m];
-    const double curv = curvature_at(d, n);
-    d.force[m] = d.cnx[n];
-    d.force[d.fs + m] = d.cny[n];
-    d.force[2 * d.fs + m] = d.cnz[n];
-    d.force[3 * d.fs + m] = 0.5 * gamma * curv * d.cnorm[n];
+    const double curv = curvature_at<NT>(d, n);
+    d.force[m] = (double)N.x[n];
+    d.force[d.fs + m] = (double)N.y[n];
+    d.force[2 * d.fs + m] = (double)N.z[n];
+    d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)N.n[n];
 }
 // dense curvature over the whole interior, for downloads (the reference computes it everywhere)
+template <class NT>
 __global__ void k_curvature_dense(Dev d) {
     const Layout L = d.L;
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
@@ -981,7 +1016,7 @@ __global__ void k_curvature_dense(Dev d) {
     const int j = r / L.nx + 1;
     const int i = r - (j - 1) * L.nx + 1;
     const int n = L.at(i, j, k);
-    d.curv[n] = curvature_at(d, n);
+    d.curv[n] = curvature_at<NT>(d, n);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -1172,6 +1207,7 @@ struct mflbm_ctx {
     long ps = 0;                // stride of the dense arrays
     double* dense_pdf = nullptr;   // 38 dense arrays: upload/download staging, keeps never-touched slots
     double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
+    float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
     double* u = nullptr;
@@ -1211,13 +1247,15 @@ CollideParams collide_params(const mflbm_params& p) {
 
 bool open_z(const mflbm_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
 
-int launch_gradient(mflbm_ctx* c) {
+template <class NT>
+int launch_gradient_t(mflbm_ctx* c) {
     LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
-    LAUNCH(c, k_normals, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
-    LAUNCH(c, k_normal_extrapolate, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
-    if (!MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
+    LAUNCH(c, k_normals<NT>, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
+    LAUNCH(c, k_normal_extrapolate<NT>, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
+    if (!MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature<NT>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     return MFLBM_OK;
 }
+int launch_gradient(mflbm_ctx* c) { return c->fp32 ? launch_gradient_t<float>(c) : launch_gradient_t<double>(c); }
 
 template <class T>
 int launch_faces(mflbm_ctx* c, bool odd);
@@ -1394,7 +1432,16 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
         hipMemset(c->fields, 0, sizeof(double) * ps * 11) != hipSuccess ||
         hipMemset(c->walls, 1, (size_t)ps) != hipSuccess || hipMemset(c->planes, 0, sizeof(double) * plane * 40) != hipSuccess)
         return bail(fail(c, MFLBM_ERR_HIP, "hipMemset failed"));
+    if (c->fp32 && (hipMalloc(&c->fields32, sizeof(float) * ps * 4) != hipSuccess ||
+                    hipMemset(c->fields32, 0, sizeof(float) * ps * 4) != hipSuccess))
+        return bail(fail(c, MFLBM_ERR_NOMEM, "device allocation of the FP32 normal field failed"));
     Dev& d = c->d;
+    if (c->fp32) {
+        d.cnx32 = c->fields32;
+        d.cny32 = c->fields32 + ps;
+        d.cnz32 = c->fields32 + 2 * ps;
+        d.cnorm32 = c->fields32 + 3 * ps;
+    }
     d.L = L;
     d.phi = c->fields;
     d.cnx = c->fields + ps;
@@ -1423,7 +1470,7 @@ extern "C" void mflbm_destroy(mflbm_ctx* c) {
     for (auto e : c->ev)
         if (e) (void)hipEventDestroy(e);
     free_geometry(c);
-    void* bufs[] = {c->dense_pdf, c->fields, c->walls, c->planes, c->red};
+    void* bufs[] = {c->dense_pdf, c->fields, c->fields32, c->walls, c->planes, c->red};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1565,6 +1612,7 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     // normals at solid nodes are only ever written by the extrapolation; start them from zero so no value
     // of a previous geometry survives (cn_x, cn_y, cn_z, c_norm, curv, u, v, w, rho)
     HIPCHK(c, hipMemset(c->d.cnx, 0, sizeof(double) * c->ps * 9));
+    if (c->fields32) HIPCHK(c, hipMemset(c->fields32, 0, sizeof(float) * c->ps * 4));
     std::vector<int> hi;
     hi.reserve(ni);
     hi.insert(hi.end(), cpos.begin(), cpos.end());
@@ -1720,7 +1768,10 @@ extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     // the force input of the macroscopic velocity comes from the current normal field
-    if (MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
+    if (MFLBM_FUSED_CURVATURE) {
+        if (c->fp32) LAUNCH(c, k_curvature<float>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
+        else LAUNCH(c, k_curvature<double>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
+    }
     if (c->fp32) LAUNCH(c, k_macro<float>, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
     else LAUNCH(c, k_macro<double>, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
     LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
@@ -1798,14 +1849,24 @@ extern "C" int mflbm_download_gradient(mflbm_ctx* c, double* cn_x, double* cn_y,
     const Layout& L = c->L;
     // the step keeps curvature only at fluid nodes (folded into the force input); the dense field the
     // reference holds is evaluated from the current normals on request
-    if (curv) LAUNCH(c, k_curvature_dense, nblocks((long)L.nx * L.ny * L.nz), c->d);
+    if (curv) {
+        if (c->fp32) LAUNCH(c, k_curvature_dense<float>, nblocks((long)L.nx * L.ny * L.nz), c->d);
+        else LAUNCH(c, k_curvature_dense<double>, nblocks((long)L.nx * L.ny * L.nz), c->d);
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
+    std::vector<float> buf32(c->fp32 ? L.total : 0);
     double* src[5] = {c->d.cnx, c->d.cny, c->d.cnz, c->d.cnorm, c->d.curv};
+    float* src32[4] = {c->d.cnx32, c->d.cny32, c->d.cnz32, c->d.cnorm32};
     double* dst[5] = {cn_x, cn_y, cn_z, c_norm, curv};
     for (int m = 0; m < 5; ++m) {
         if (!dst[m]) continue;
-        HIPCHK(c, hipMemcpy(buf.data(), src[m], sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        if (c->fp32 && m < 4) {   // the FP32 normal field, widened
+            HIPCHK(c, hipMemcpy(buf32.data(), src32[m], sizeof(float) * L.total, hipMemcpyDeviceToHost));
+            for (long k = 0; k < L.total; ++k) buf[k] = (double)buf32[k];
+        } else {
+            HIPCHK(c, hipMemcpy(buf.data(), src[m], sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        }
         unpack_host(L, m == 4 ? 1 : 2, buf.data(), dst[m]);
     }
     return MFLBM_OK;
