@@ -203,10 +203,10 @@ def run_batch(args) -> dict:
     t0 = time.time()
     samples = batch.rev_samples(args.batch_samples, seed=2024)
     print(f"[bench] {len(samples)} samples generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    tune = batch.autotune(samples, args.precision if args.precision != "fp64" else "mixed")
+    p32 = args.precision if args.precision != "fp64" else "fp32"
+    tune = batch.autotune(samples, p32)
     print(f"[bench] autotune: tile_rows {tune['tile_rows']} block {tune['block']}", file=sys.stderr, flush=True)
     kw = dict(la_nu=0.1, force_z=1e-6, ntime_monitor=500, tol=1e-7, tile_rows=tune["tile_rows"], block=tune["block"])
-    p32 = args.precision if args.precision != "fp64" else "mixed"
     b = batch.REVBatch(samples, precision=p32, **kw)
     b.run(1, args.warmup)
     b.synchronize()
@@ -220,7 +220,7 @@ def run_batch(args) -> dict:
     conv = {}
     for name, prec, rest in (("fp64", "fp64", 0), (p32, p32, 0), (p32 + "_rest_state", p32, 1)):
         bb = batch.REVBatch(samples, precision=prec, rest_state=rest, **kw)
-        r = bb.run_steady(args.batch_max_steps)
+        r = bb.run_steady(args.batch_max_steps, log=lambda m: print(f"[bench] {name} {m}", file=sys.stderr, flush=True))
         bb.close()
         conv[name] = r
         print(f"[bench] {name}: steady in {r.seconds:.2f}s, {int((r.conv_step > 0).sum())}/{len(samples)} converged, "

@@ -183,12 +183,24 @@ class REVBatch:
         k = self.la_nu * q / (self.area * self.force_z)
         return BatchResult(cs, q, k, int(per[0]), seconds)
 
-    def run_steady(self, max_steps: int) -> BatchResult:
-        """Monitor periods until every sample has converged (or failed) or max_steps steps have run."""
+    def run_steady(self, max_steps: int, log=None, chunk_steps: int = 10000) -> BatchResult:
+        """Monitor periods until every sample has converged (or failed) or max_steps steps have run; the
+        periods go to the device in chunks of about chunk_steps steps, after each of which ``log`` (if given)
+        receives a progress line."""
         nm = self.params.ntime_monitor
         left = C.c_int32()
         t0 = time.perf_counter()
-        self._call("run_steady", max(1, -(-max_steps // nm)), C.byref(left))
+        total = max(1, -(-max_steps // nm))
+        chunk = max(1, chunk_steps // nm)
+        done = 0
+        while done < total:
+            k = min(chunk, total - done)
+            self._call("run_steady", k, C.byref(left))
+            done += k
+            if log:
+                log(f"batch: {done * nm} steps, {left.value} samples running, {time.perf_counter() - t0:.1f}s")
+            if left.value == 0:
+                break
         return self.results(time.perf_counter() - t0)
 
     def reset_monitor(self) -> None:
