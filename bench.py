@@ -307,7 +307,7 @@ def run_warmstart(args) -> dict:
     encoder-decoder trained on them for args.ws_epochs epochs with the project's kernels, then unseen packs of
     sides args.ws_eval run to the flow-rate criterion 1e-7 three ways: the plain path (FP64, the reference's
     initial state), from rest at the collision's equilibrium and from the network's fields (both with FP32
-    storage and FP64 collision).  ``value`` is the warm-started wall time over the plain path's."""
+    storage and FP32 collision).  ``value`` is the warm-started wall time over the plain path's."""
     import numpy as np
     from mflbm import batch, warmstart
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
@@ -341,7 +341,7 @@ def run_warmstart(args) -> dict:
             "config": {"workload": f"{args.ws_samples} training fields (64^3 sphere packs, porosity 0.15-0.35, "
                                    f"batched solver, criterion 1e-6), {args.ws_epochs} epochs, BF16 encoder-decoder "
                                    "(4 levels, 32 base channels); unseen packs (porosity 0.25) to criterion 1e-7: "
-                                   "plain = FP64 from the reference's initial state, cold = FP32 storage + FP64 "
+                                   "plain = FP64 from the reference's initial state, cold = FP32 storage + FP32 "
                                    "collision from rest at the collision's equilibrium, warm = the same from the "
                                    "network's (u, v, w, rho)"},
             "training": {"data_seconds": round(data.seconds, 1), "train_seconds": round(t_train, 1),

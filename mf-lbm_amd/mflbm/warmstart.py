@@ -416,7 +416,7 @@ class WarmStartResult:
 
 
 def compare_runs(net: WarmStartNet, sample: batch.REVSample, la_nu: float = 0.1, force: float = 1e-6,
-                 tol: float = 1e-7, precision: str = "mixed", plain_precision: str = "fp64",
+                 tol: float = 1e-7, precision: str = "fp32", plain_precision: str = "fp64",
                  max_steps: int = 2000000, ntime_monitor: int = 500) -> WarmStartResult:
     """The same sample run to the reference's criterion from the reference's initial state, from rest at the
     collision's equilibrium and from the network's fields."""
