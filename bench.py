@@ -364,7 +364,7 @@ def main() -> None:
     ap.add_argument("--batch-samples", type=int, default=96)
     ap.add_argument("--report-steps", default="5000,10000,20000")
     ap.add_argument("--ws-samples", type=int, default=256, help="warm start: training fields (the full set is 2000)")
-    ap.add_argument("--ws-epochs", type=int, default=8)
+    ap.add_argument("--ws-epochs", type=int, default=40)
     ap.add_argument("--ws-eval", default="128", help="warm start: sides of the unseen evaluation packs")
     ap.add_argument("--drop-steps", type=int, default=4000)
     ap.add_argument("--batch-max-steps", type=int, default=100000)

@@ -175,24 +175,31 @@ __device__ __forceinline__ double curvature_at(const Dev& d, int n);
 
 // ODD = false: kernel_even_color (Kernel_multiphase.F90:371-725): read swapped own slots, write own slots.
 // ODD = true : kernel_odd_color  (Kernel_multiphase.F90:6-362): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
-// The CSF force input is evaluated here from the normal field the colour-gradient chain left (curvature,
-// Phase_gradient.F90:112-201, and 0.5*gamma*curv*c_norm, Kernel_multiphase.F90:116).
-template <bool ODD>
+// The CSF force input (curvature, Phase_gradient.F90:112-201, and 0.5*gamma*curv*c_norm,
+// Kernel_multiphase.F90:116) is either evaluated here from the dense normal field the list kernels of the chain
+// left (STENCIL), or read from the compact force arrays the fused chain (k_chain_force) wrote.
+template <bool ODD, bool STENCIL>
 __global__ void __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(ODD ? MFLBM_COLLIDE_WAVES : MFLBM_COLLIDE_WAVES_EVEN, 8)))
 k_collide(Dev d, CollideParams P, int per) {
     const SlotTable& S = d.S;
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
-#if MFLBM_FUSED_CURVATURE
-    const int nn = d.cpos[m];
-    const double cnx = d.cnx[nn], cny = d.cny[nn], cnz = d.cnz[nn];
-    const double tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
-    // keep the stencil phase and the population phase apart so their registers are not live together
-    __builtin_amdgcn_sched_barrier(0);
-#else
-    const double cnx = d.force[m], cny = d.force[d.fs + m], cnz = d.force[2 * d.fs + m], tf = d.force[3 * d.fs + m];
-#endif
+    double cnx, cny, cnz, tf;
+    if (STENCIL) {
+        const int nn = d.cpos[m];
+        cnx = d.cnx[nn];
+        cny = d.cny[nn];
+        cnz = d.cnz[nn];
+        tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
+        // keep the stencil phase and the population phase apart so their registers are not live together
+        __builtin_amdgcn_sched_barrier(0);
+    } else {
+        cnx = d.force[m];
+        cny = d.force[d.fs + m];
+        cnz = d.force[2 * d.fs + m];
+        tf = d.force[3 * d.fs + m];
+    }
     double a[19], b[19];
     unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
     a[0] = ld_slot(S.p[0], m);
@@ -237,15 +244,25 @@ k_collide(Dev d, CollideParams P, int per) {
 // FP32 storage (s = f + g - REST, a = f - g, collide.h): the same update, the collision and recolouring in C
 // (float, or double for the mixed mode).  The normal, curvature and phi stay FP64 (the colour-gradient chain is
 // shared with the FP64 path).
-template <bool ODD, class C>
+template <bool ODD, class C, bool STENCIL>
 __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> P, double gamma, int per) {
     const SlotTable& S = d.S;
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
-    const C cnx = (C)d.cnx32[nn], cny = (C)d.cny32[nn], cnz = (C)d.cnz32[nn];
-    const C tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
-    __builtin_amdgcn_sched_barrier(0);
+    C cnx, cny, cnz, tf;
+    if (STENCIL) {
+        cnx = (C)d.cnx32[nn];
+        cny = (C)d.cny32[nn];
+        cnz = (C)d.cnz32[nn];
+        tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
+        __builtin_amdgcn_sched_barrier(0);
+    } else {   // the force arrays hold the FP32 normal widened to FP64
+        cnx = (C)d.force[m];
+        cny = (C)d.force[d.fs + m];
+        cnz = (C)d.force[2 * d.fs + m];
+        tf = (C)d.force[3 * d.fs + m];
+    }
     C s[19], a[19];
     unsigned nb[19];
     s[0] = slot_ld((const float*)S.p[0], (unsigned)m);
@@ -843,24 +860,68 @@ struct NodeList {
     const int* pos;
 };
 
-// ISO4 first derivatives with the reference's term order (Phase_gradient.F90:40-62)
-template <class T>
-__device__ __forceinline__ double dx4(const T* A, int n, int px, int pxy) {
-    return lat::ISO4_1 * (A[n + 1] - A[n - 1]) +
-           lat::ISO4_2 * (A[n + 1 + px] - A[n - 1 - px] + A[n + 1 - px] - A[n - 1 + px] + A[n + 1 + pxy] - A[n - 1 - pxy] +
-                          A[n + 1 - pxy] - A[n - 1 + pxy]);
+// ISO4 first derivatives with the reference's term order (Phase_gradient.F90:40-62).  A(dx, dy, dz) is the
+// field at the offset (as a double); the same expressions serve the dense arrays and the LDS tiles of the fused
+// chain, so both evaluate identical operations in identical order.
+template <class F>
+__device__ __forceinline__ double dx4f(const F& A) {
+    return lat::ISO4_1 * (A(1, 0, 0) - A(-1, 0, 0)) +
+           lat::ISO4_2 * (A(1, 1, 0) - A(-1, -1, 0) + A(1, -1, 0) - A(-1, 1, 0) + A(1, 0, 1) - A(-1, 0, -1) +
+                          A(1, 0, -1) - A(-1, 0, 1));
+}
+template <class F>
+__device__ __forceinline__ double dy4f(const F& A) {
+    return lat::ISO4_1 * (A(0, 1, 0) - A(0, -1, 0)) +
+           lat::ISO4_2 * (A(1, 1, 0) - A(-1, -1, 0) + A(-1, 1, 0) - A(1, -1, 0) + A(0, 1, 1) - A(0, -1, -1) +
+                          A(0, 1, -1) - A(0, -1, 1));
+}
+template <class F>
+__device__ __forceinline__ double dz4f(const F& A) {
+    return lat::ISO4_1 * (A(0, 0, 1) - A(0, 0, -1)) +
+           lat::ISO4_2 * (A(1, 0, 1) - A(-1, 0, -1) + A(-1, 0, 1) - A(1, 0, -1) + A(0, 1, 1) - A(0, -1, -1) +
+                          A(0, -1, 1) - A(0, 1, -1));
 }
 template <class T>
-__device__ __forceinline__ double dy4(const T* A, int n, int px, int pxy) {
-    return lat::ISO4_1 * (A[n + px] - A[n - px]) +
-           lat::ISO4_2 * (A[n + 1 + px] - A[n - 1 - px] + A[n - 1 + px] - A[n + 1 - px] + A[n + px + pxy] - A[n - px - pxy] +
-                          A[n + px - pxy] - A[n - px + pxy]);
-}
+struct DenseAt {
+    const T* A;
+    int n, px, pxy;
+    __device__ __forceinline__ double operator()(int a, int b, int c) const { return (double)A[n + a + b * px + c * pxy]; }
+};
 template <class T>
-__device__ __forceinline__ double dz4(const T* A, int n, int px, int pxy) {
-    return lat::ISO4_1 * (A[n + pxy] - A[n - pxy]) +
-           lat::ISO4_2 * (A[n + 1 + pxy] - A[n - 1 - pxy] + A[n - 1 + pxy] - A[n + 1 - pxy] + A[n + px + pxy] - A[n - px - pxy] +
-                          A[n - px + pxy] - A[n + px - pxy]);
+__device__ __forceinline__ double dx4(const T* A, int n, int px, int pxy) { return dx4f(DenseAt<T>{A, n, px, pxy}); }
+template <class T>
+__device__ __forceinline__ double dy4(const T* A, int n, int px, int pxy) { return dy4f(DenseAt<T>{A, n, px, pxy}); }
+template <class T>
+__device__ __forceinline__ double dz4(const T* A, int n, int px, int pxy) { return dz4f(DenseAt<T>{A, n, px, pxy}); }
+// curvature from the normal components X, Y, Z as accessors (Phase_gradient.F90:112-201)
+template <class FX, class FY, class FZ>
+__device__ __forceinline__ double curvature_f(const FX& X, const FY& Y, const FZ& Z) {
+    const double kxx = dx4f(X);
+    const double kyy = dy4f(Y);
+    const double kzz = dz4f(Z);
+    const double kxy = dy4f(X);
+    const double kxz = dz4f(X);
+    const double kyx = dx4f(Y);
+    const double kyz = dz4f(Y);
+    const double kzx = dx4f(Z);
+    const double kzy = dy4f(Z);
+    const double cx = X(0, 0, 0), cy = Y(0, 0, 0), cz = Z(0, 0, 0);
+    return (cx * cx - 1.0) * kxx + (cy * cy - 1.0) * kyy + (cz * cz - 1.0) * kzz + cx * cy * (kxy + kyx) +
+           cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+}
+// interface normal and |grad phi| from the gradient (Phase_gradient.F90:33-79): below 1e-6 everything is 0
+__device__ __forceinline__ void normalise(double& cx, double& cy, double& cz, double& cn) {
+    cn = sqrt(cx * cx + cy * cy + cz * cz);
+    if (cn < 1e-6) {
+        cx = 0.0;
+        cy = 0.0;
+        cz = 0.0;
+        cn = 0.0;
+    } else {
+        cx = cx / cn;
+        cy = cy / cn;
+        cz = cz / cn;
+    }
 }
 
 // The normal field: FP64 (the reference arithmetic) or, in the FP32 storage modes, FP32 (the gradient and
@@ -933,19 +994,10 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     double cx = dx4(d.phi, n, L.px, L.pxy);
     double cy = dy4(d.phi, n, L.px, L.pxy);
     double cz = dz4(d.phi, n, L.px, L.pxy);
-    double cn = sqrt(cx * cx + cy * cy + cz * cz);
-    if (cn < 1e-6) {
-        cx = 0.0;
-        cy = 0.0;
-        cz = 0.0;
-        cn = 0.0;
-    } else {
-        cx = cx / cn;
-        cy = cy / cn;
-        cz = cz / cn;
-        const int b = wb[t];
-        if (b >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * b], fl.nw[3 * b + 1], fl.nw[3 * b + 2], fl.cos_t[b], fl.sin_t[b]);
-    }
+    double cn;
+    normalise(cx, cy, cz, cn);
+    const int b = wb[t];
+    if (b >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * b], fl.nw[3 * b + 1], fl.nw[3 * b + 2], fl.cos_t[b], fl.sin_t[b]);
     const Normals<NT> N = normals<NT>(d);
     N.x[n] = (NT)cx;
     N.y[n] = (NT)cy;
@@ -977,18 +1029,7 @@ template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
     const Normals<NT> N = normals<NT>(d);
     const int px = d.L.px, pxy = d.L.pxy;
-    const double kxx = dx4(N.x, n, px, pxy);
-    const double kyy = dy4(N.y, n, px, pxy);
-    const double kzz = dz4(N.z, n, px, pxy);
-    const double kxy = dy4(N.x, n, px, pxy);
-    const double kxz = dz4(N.x, n, px, pxy);
-    const double kyx = dx4(N.y, n, px, pxy);
-    const double kyz = dz4(N.y, n, px, pxy);
-    const double kzx = dx4(N.z, n, px, pxy);
-    const double kzy = dy4(N.z, n, px, pxy);
-    const double cx = (double)N.x[n], cy = (double)N.y[n], cz = (double)N.z[n];
-    return (cx * cx - 1.0) * kxx + (cy * cy - 1.0) * kyy + (cz * cz - 1.0) * kzz + cx * cy * (kxy + kyx) +
-           cx * cz * (kxz + kzx) + cy * cz * (kzy + kyz);
+    return curvature_f(DenseAt<NT>{N.x, n, px, pxy}, DenseAt<NT>{N.y, n, px, pxy}, DenseAt<NT>{N.z, n, px, pxy});
 }
 // curvature at the fluid nodes (Phase_gradient.F90:112-201), emitted with the normal as the per-node force
 // input of the next lattice update: (cn_x, cn_y, cn_z, 0.5*gamma*curv*c_norm), Kernel_multiphase.F90:112-116
@@ -1004,6 +1045,138 @@ __global__ void __launch_bounds__(BLOCK) k_curvature(Dev d, double gamma, int pe
     d.force[2 * d.fs + m] = (double)N.z[n];
     d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)N.n[n];
 }
+// The tail of the colour-gradient chain in one pass, for the lattice update's force input: interface normals
+// with the wetting rotation, their extrapolation into the solid boundary nodes and the curvature force
+// (cn_x, cn_y, cn_z, 0.5 gamma curv c_norm) of every interior fluid node, written to the compact force arrays.
+// A work-group owns a CT x CT column of interior nodes and a segment of CSEG planes, and marches along z with
+// rolling LDS planes: at step i it loads phi plane i (tile + halo 3), computes the normals of plane i - 1
+// (halo 2), extrapolates plane i - 2 (halo 1) and evaluates the curvature of plane i - 3.  Every value is the
+// one k_normals, k_normal_extrapolate and k_curvature compute from the same phi (same expressions, same order),
+// so the force input is bit-identical to the list kernels'; the dense normal field is not written.
+// role[o]: >= 0 fluid boundary index (wetting), <= -2 solid boundary node of the extrapolation list -role - 2.
+constexpr int CT = 16;           // output columns per tile edge
+constexpr int CSEG = 32;         // planes per work-group
+constexpr int CPW = CT + 6;      // phi tile edge (halo 3)
+constexpr int CNW = CT + 4;      // normal tile edge (halo 2)
+constexpr int CBLOCK = 256;
+__device__ __forceinline__ int mod_pos(int a, int m) { return ((a % m) + m) % m; }
+template <class NT>
+__global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, SolidList s1, const int* __restrict__ role,
+                                                        double gamma, int ntx, int nty, int per) {
+    __shared__ double sphi[3][CPW * CPW];
+    __shared__ NT snrm[4][4][CNW * CNW];   // [plane slot][x, y, z, |grad phi|][node]
+    const Layout L = d.L;
+    const int b = xcd_block(per);
+    if (b >= ntx * nty * ((L.nz + CSEG - 1) / CSEG)) return;
+    const int tx = b % ntx, ty = (b / ntx) % nty, sg = b / (ntx * nty);
+    const int x0 = 1 + tx * CT, y0 = 1 + ty * CT;
+    const int k0 = 1 + sg * CSEG, k1 = min(L.nz, k0 + CSEG - 1);
+    const int tid = (int)threadIdx.x;
+    for (int i = k0 - 3; i <= k1 + 3; ++i) {
+        // phi plane i over [x0 - 3, x0 + CT + 2] x [y0 - 3, y0 + CT + 2]; outside the padded array: 0 (unused)
+        {
+            double* P = sphi[mod_pos(i, 3)];
+            for (int e = tid; e < CPW * CPW; e += CBLOCK) {
+                const int xx = x0 - 3 + e % CPW, yy = y0 - 3 + e / CPW;
+                const bool in = xx >= -3 && xx <= L.nx + 4 && yy >= -3 && yy <= L.ny + 4 && i >= -3 && i <= L.nz + 4;
+                P[e] = in ? d.phi[L.at(xx, yy, i)] : 0.0;
+            }
+        }
+        __syncthreads();
+        // normals of plane j = i - 1 at the non-solid nodes of [-1, n+2]^3 (k_normals), 0 elsewhere
+        const int j = i - 1;
+        if (j >= k0 - 2) {
+            const double* P0 = sphi[mod_pos(j - 1, 3)];
+            const double* P1 = sphi[mod_pos(j, 3)];
+            const double* P2 = sphi[mod_pos(j + 1, 3)];
+            NT(*S)[CNW * CNW] = snrm[mod_pos(j, 4)];
+            for (int e = tid; e < CNW * CNW; e += CBLOCK) {
+                const int ex = e % CNW, ey = e / CNW;
+                const int xx = x0 - 2 + ex, yy = y0 - 2 + ey;
+                double cx = 0.0, cy = 0.0, cz = 0.0, cn = 0.0;
+                if (xx >= -1 && xx <= L.nx + 2 && yy >= -1 && yy <= L.ny + 2 && j >= -1 && j <= L.nz + 2) {
+                    const int o = L.at(xx, yy, j);
+                    if (d.walls[o] == 0) {
+                        const int c0 = (ey + 1) * CPW + (ex + 1);
+                        auto A = [&](int a, int bb, int c) {
+                            const double* Pz = c < 0 ? P0 : (c > 0 ? P2 : P1);
+                            return Pz[c0 + a + bb * CPW];
+                        };
+                        cx = dx4f(A);
+                        cy = dy4f(A);
+                        cz = dz4f(A);
+                        normalise(cx, cy, cz, cn);
+                        const int r = role[o];
+                        if (r >= 0 && cn > 1e-6)
+                            wet(cx, cy, cz, fl.nw[3 * r], fl.nw[3 * r + 1], fl.nw[3 * r + 2], fl.cos_t[r], fl.sin_t[r]);
+                    }
+                }
+                S[0][e] = (NT)cx;
+                S[1][e] = (NT)cy;
+                S[2][e] = (NT)cz;
+                S[3][e] = (NT)cn;
+            }
+        }
+        __syncthreads();
+        // extrapolation into the solid boundary nodes of plane j2 = i - 2 (k_normal_extrapolate)
+        const int j2 = i - 2;
+        if (j2 >= k0 - 1) {
+            NT(*S)[CNW * CNW] = snrm[mod_pos(j2, 4)];
+            for (int e = tid; e < (CT + 2) * (CT + 2); e += CBLOCK) {
+                const int ex = e % (CT + 2) + 1, ey = e / (CT + 2) + 1;   // in the normal tile
+                const int xx = x0 - 2 + ex, yy = y0 - 2 + ey;
+                if (xx < 0 || xx > L.nx + 1 || yy < 0 || yy > L.ny + 1 || j2 < 0 || j2 > L.nz + 1) continue;
+                const int r = role[L.at(xx, yy, j2)];
+                if (r > -2) continue;
+                const int sidx = -r - 2;
+                const unsigned mk = s1.mask[sidx];
+                double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+                for (int q = 1; q < 19; ++q)
+                    if (mk & (1u << q)) {
+                        const NT(*Q)[CNW * CNW] = snrm[mod_pos(j2 + lat::EZ[q], 4)];
+                        const int en = (ey + lat::EY[q]) * CNW + (ex + lat::EX[q]);
+                        ax = ax + (double)Q[0][en] * lat::W[q];
+                        ay = ay + (double)Q[1][en] * lat::W[q];
+                        az = az + (double)Q[2][en] * lat::W[q];
+                    }
+                const int es = ey * CNW + ex;
+                S[0][es] = (NT)(ax / s1.w[sidx]);
+                S[1][es] = (NT)(ay / s1.w[sidx]);
+                S[2][es] = (NT)(az / s1.w[sidx]);
+            }
+        }
+        __syncthreads();
+        // curvature force of the interior fluid nodes of plane k = i - 3 (k_curvature)
+        const int k = i - 3;
+        if (k >= k0 && k <= k1) {
+            const int ex = tid % CT + 2, ey = tid / CT + 2;
+            const int xx = x0 - 2 + ex, yy = y0 - 2 + ey;
+            if (xx <= L.nx && yy <= L.ny) {
+                const int o = L.at(xx, yy, k);
+                const int m = d.map[o];
+                if (d.walls[o] == 0 && m >= 0 && m < d.nf) {
+                    const NT(*Sm)[CNW * CNW] = snrm[mod_pos(k - 1, 4)];
+                    const NT(*Sc)[CNW * CNW] = snrm[mod_pos(k, 4)];
+                    const NT(*Sp)[CNW * CNW] = snrm[mod_pos(k + 1, 4)];
+                    const int c0 = ey * CNW + ex;
+                    auto comp = [&](int cmp) {
+                        return [=](int a, int bb, int c) {
+                            const NT(*Sz)[CNW * CNW] = c < 0 ? Sm : (c > 0 ? Sp : Sc);
+                            return (double)Sz[cmp][c0 + a + bb * CNW];
+                        };
+                    };
+                    const double curv = curvature_f(comp(0), comp(1), comp(2));
+                    d.force[m] = (double)Sc[0][c0];
+                    d.force[d.fs + m] = (double)Sc[1][c0];
+                    d.force[2 * d.fs + m] = (double)Sc[2][c0];
+                    d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)Sc[3][c0];
+                }
+            }
+        }
+    }
+}
+
 // dense curvature over the whole interior, for downloads (the reference computes it everywhere)
 template <class NT>
 __global__ void k_curvature_dense(Dev d) {
@@ -1208,6 +1381,8 @@ struct mflbm_ctx {
     double* dense_pdf = nullptr;   // 38 dense arrays: upload/download staging, keeps never-touched slots
     double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
     float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
+    int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
+    bool chain_fused = false;   // one fused pass for normals, extrapolation and curvature (MFLBM_CHAIN=fused)
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
     double* u = nullptr;
@@ -1247,11 +1422,26 @@ CollideParams collide_params(const mflbm_params& p) {
 
 bool open_z(const mflbm_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
 
+// dense normal field by the list kernels (the chain of the reference, one pass per stage)
+template <class NT>
+int launch_normals_dense(mflbm_ctx* c) {
+    LAUNCH(c, k_normals<NT>, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
+    LAUNCH(c, k_normal_extrapolate<NT>, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
+    return MFLBM_OK;
+}
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c) {
     LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
-    LAUNCH(c, k_normals<NT>, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
-    LAUNCH(c, k_normal_extrapolate<NT>, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
+    if (c->chain_fused) {
+        const Layout& L = c->L;
+        const int ntx = (L.nx + CT - 1) / CT, nty = (L.ny + CT - 1) / CT, nseg = (L.nz + CSEG - 1) / CSEG;
+        const long nblk = (long)ntx * nty * nseg;
+        const int per = (int)((nblk + NXCD - 1) / NXCD);
+        LAUNCH(c, k_chain_force<NT>, NXCD * per, c->d, c->fl, c->sl1, (const int*)c->role, c->p.gamma, ntx, nty, per);
+        return MFLBM_OK;
+    }
+    const int rc = launch_normals_dense<NT>(c);
+    if (rc) return rc;
     if (!MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature<NT>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     return MFLBM_OK;
 }
@@ -1279,18 +1469,28 @@ int launch_step(mflbm_ctx* c, int ntime) {
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
     }
     const int per = xcd_per(c->d.nf);
+#define MFLBM_COLLIDE32(C_, PP)                                                                                  \
+    if (c->chain_fused) {                                                                                        \
+        if (odd) LAUNCH(c, (k_collide32<true, C_, false>), NXCD * per, c->d, PP, p.gamma, per);                   \
+        else LAUNCH(c, (k_collide32<false, C_, false>), NXCD * per, c->d, PP, p.gamma, per);                      \
+    } else {                                                                                                     \
+        if (odd) LAUNCH(c, (k_collide32<true, C_, true>), NXCD * per, c->d, PP, p.gamma, per);                    \
+        else LAUNCH(c, (k_collide32<false, C_, true>), NXCD * per, c->d, PP, p.gamma, per);                       \
+    }
     if (c->fp32 && c->p.precision == 2) {
         const CollideParamsDev<double> PD{p.gamma, p.beta, p.force_z, p.la_nui1, p.la_nui2};
-        if (odd) LAUNCH(c, (k_collide32<true, double>), NXCD * per, c->d, PD, p.gamma, per);
-        else LAUNCH(c, (k_collide32<false, double>), NXCD * per, c->d, PD, p.gamma, per);
+        MFLBM_COLLIDE32(double, PD)
     } else if (c->fp32) {
         const CollideParamsDev<float> P32{(float)p.gamma, (float)p.beta, (float)p.force_z, (float)p.la_nui1, (float)p.la_nui2};
-        if (odd) LAUNCH(c, (k_collide32<true, float>), NXCD * per, c->d, P32, p.gamma, per);
-        else LAUNCH(c, (k_collide32<false, float>), NXCD * per, c->d, P32, p.gamma, per);
+        MFLBM_COLLIDE32(float, P32)
+    } else if (c->chain_fused) {
+        if (odd) LAUNCH(c, (k_collide<true, false>), NXCD * per, c->d, P, per);
+        else LAUNCH(c, (k_collide<false, false>), NXCD * per, c->d, P, per);
     } else {
-        if (odd) LAUNCH(c, k_collide<true>, NXCD * per, c->d, P, per);
-        else LAUNCH(c, k_collide<false>, NXCD * per, c->d, P, per);
+        if (odd) LAUNCH(c, (k_collide<true, true>), NXCD * per, c->d, P, per);
+        else LAUNCH(c, (k_collide<false, true>), NXCD * per, c->d, P, per);
     }
+#undef MFLBM_COLLIDE32
     if (t) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     const int rc = c->fp32 ? launch_faces<float>(c, odd) : launch_faces<double>(c, odd);
     if (rc) return rc;
@@ -1367,7 +1567,7 @@ int scatter_state(mflbm_ctx* c) {
 }
 
 void free_geometry(mflbm_ctx* c) {
-    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase};
+    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
@@ -1379,6 +1579,7 @@ void free_geometry(mflbm_ctx* c) {
     c->dbuf = nullptr;
     c->wmask = nullptr;
     c->wbase = nullptr;
+    c->role = nullptr;
     c->have_geom = false;
     c->have_state = false;
 }
@@ -1404,6 +1605,12 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     if (!c) return fail(nullptr, MFLBM_ERR_NOMEM, "context allocation failed");
     c->p = *prm;
     c->fp32 = prm->precision != 0;
+    {
+        // the fused chain is bit-identical but slower on the 256^3 benchmark (1.23 vs 0.42 ms; the lattice
+        // update it feeds is 8 % faster): the list kernels stay the default
+        const char* e = std::getenv("MFLBM_CHAIN");
+        c->chain_fused = e && std::strcmp(e, "fused") == 0;
+    }
     Layout& L = c->L;
     L = make_layout(prm->nx, prm->ny, prm->nz);
     if (L.total > (1L << 31) - 1) {
@@ -1579,6 +1786,12 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
         for (int t = 0; t < nf; ++t) at[fpos[t]] = t;
         for (size_t t = 0; t < nrm.size(); ++t) wb[t] = at[nrm[t]];
     }
+    // the same per node for the fused chain: the fluid boundary index of a non-solid node (as wb), or the
+    // extrapolation-list index of a solid boundary node in [0, n+1]^3 as -index - 2
+    std::vector<int> role(L.total, -1);
+    for (int t = 0; t < nf; ++t)
+        if (wp[fpos[t]] == 0) role[fpos[t]] = t;
+    for (size_t u = 0; u < s1pos.size(); ++u) role[s1pos[u]] = -(int)u - 2;
     // device buffers
     free_geometry(c);
     // slice q: nc compact slots, then the link slots of direction opp(q); each slice 256-byte aligned
@@ -1629,6 +1842,8 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     hd.insert(hd.end(), fd.begin(), fd.end());
     hd.push_back(0.0);
     HIPCHK(c, hipMemcpy(c->map, cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->role, sizeof(int) * L.total));
+    HIPCHK(c, hipMemcpy(c->role, role.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->mbuf, hm.data(), sizeof(unsigned) * hm.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dbuf, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
@@ -1768,7 +1983,8 @@ extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     // the force input of the macroscopic velocity comes from the current normal field
-    if (MFLBM_FUSED_CURVATURE) {
+    // (the fused chain keeps the force arrays current; the list chain leaves only the dense normals)
+    if (MFLBM_FUSED_CURVATURE && !c->chain_fused) {
         if (c->fp32) LAUNCH(c, k_curvature<float>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
         else LAUNCH(c, k_curvature<double>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     }
@@ -1849,6 +2065,10 @@ extern "C" int mflbm_download_gradient(mflbm_ctx* c, double* cn_x, double* cn_y,
     const Layout& L = c->L;
     // the step keeps curvature only at fluid nodes (folded into the force input); the dense field the
     // reference holds is evaluated from the current normals on request
+    if (c->chain_fused) {   // the fused chain does not keep the dense normal field: evaluate it now
+        const int rc = c->fp32 ? launch_normals_dense<float>(c) : launch_normals_dense<double>(c);
+        if (rc) return rc;
+    }
     if (curv) {
         if (c->fp32) LAUNCH(c, k_curvature_dense<float>, nblocks((long)L.nx * L.ny * L.nz), c->d);
         else LAUNCH(c, k_curvature_dense<double>, nblocks((long)L.nx * L.ny * L.nz), c->d);
