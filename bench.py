@@ -330,7 +330,8 @@ def run_warmstart(args) -> dict:
                "time_ratio_vs_plain": round(r.time_ratio, 4)}
         for name in ("plain", "cold", "warm"):
             rec = getattr(r, name)
-            row[name] = {"steps": rec.steps, "seconds": round(rec.seconds, 3), "permeability": rec.permeability}
+            row[name] = {"steps": rec.steps, "seconds": round(rec.seconds, 3), "permeability": rec.permeability,
+                         "status": rec.status}
         log(f"{side}^3: steps plain {r.plain.steps} cold {r.cold.steps} warm {r.warm.steps}; time plain "
             f"{r.plain.seconds:.2f}s warm {r.warm.seconds + r.seconds_predict:.2f}s (ratio {r.time_ratio:.3f}); "
             f"k rel err {r.k_rel_err:.2e}")

@@ -391,6 +391,7 @@ class RunRecord:
     steps: int
     seconds: float
     permeability: float
+    status: str = "converged"   # or the run_steady status it stopped with ("max_steps", "failed")
 
 
 @dataclasses.dataclass
@@ -417,7 +418,7 @@ class WarmStartResult:
 
 def compare_runs(net: WarmStartNet, sample: batch.REVSample, la_nu: float = 0.1, force: float = 1e-6,
                  tol: float = 1e-7, precision: str = "fp32", plain_precision: str = "fp64",
-                 max_steps: int = 2000000, ntime_monitor: int = 500) -> WarmStartResult:
+                 max_steps: int = 600000, ntime_monitor: int = 500) -> WarmStartResult:
     """The same sample run to the reference's criterion from the reference's initial state, from rest at the
     collision's equilibrium and from the network's fields."""
     from .singlephase import SPCase, SPSimulation, permeability
@@ -444,7 +445,5 @@ def compare_runs(net: WarmStartNet, sample: batch.REVSample, la_nu: float = 0.1,
         n, status, hist = sim.run_steady(1, max_steps, ntime_monitor)
         dt = time.perf_counter() - t0
         sim.close()
-        if status != "converged":
-            raise RuntimeError(f"{mode} run did not converge in {max_steps} steps ({status})")
-        runs[mode] = RunRecord(n, dt, permeability(case, hist[-1][1]))
+        runs[mode] = RunRecord(n, dt, permeability(case, hist[-1][1]), status)
     return WarmStartResult(nx, runs["plain"], runs["cold"], runs["warm"], tp)
