@@ -1058,13 +1058,19 @@ constexpr int CT = 16;           // output columns per tile edge
 constexpr int CSEG = 32;         // planes per work-group
 constexpr int CPW = CT + 6;      // phi tile edge (halo 3)
 constexpr int CNW = CT + 4;      // normal tile edge (halo 2)
-constexpr int CBLOCK = 256;
+constexpr int CXW = CT + 2;      // extrapolation tile edge (halo 1)
+constexpr int CBLOCK = 512;
 __device__ __forceinline__ int mod_pos(int a, int m) { return ((a % m) + m) % m; }
 template <class NT>
 __global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, SolidList s1, const int* __restrict__ role,
                                                         double gamma, int ntx, int nty, int per) {
     __shared__ double sphi[3][CPW * CPW];
-    __shared__ NT snrm[4][4][CNW * CNW];   // [plane slot][x, y, z, |grad phi|][node]
+    __shared__ NT snrm[4][3][CNW * CNW];   // [plane slot][x, y, z][node]
+    __shared__ NT scn[4][CT * CT];         // |grad phi| of the output columns
+    __shared__ int8_t sbw[CNW * CNW];      // normals stage: wall flag (1: no normal) of plane i - 1
+    __shared__ int sbr[CNW * CNW];         //                role of plane i - 1
+    __shared__ int scr[CXW * CXW];         // extrapolation stage: role of plane i - 2 (-1 outside [0, n+1])
+    __shared__ int sdm[CT * CT];           // curvature stage: compact index of plane i - 3 (-1: no output)
     const Layout L = d.L;
     const int b = xcd_block(per);
     if (b >= ntx * nty * ((L.nz + CSEG - 1) / CSEG)) return;
@@ -1073,13 +1079,46 @@ __global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, Sol
     const int k0 = 1 + sg * CSEG, k1 = min(L.nz, k0 + CSEG - 1);
     const int tid = (int)threadIdx.x;
     for (int i = k0 - 3; i <= k1 + 3; ++i) {
-        // phi plane i over [x0 - 3, x0 + CT + 2] x [y0 - 3, y0 + CT + 2]; outside the padded array: 0 (unused)
+        // every global read of the step in one stage: phi of plane i (tile + halo 3; 0 outside the padded
+        // array, never used), walls and roles of planes i - 1 and i - 2, compact indices of plane i - 3
         {
             double* P = sphi[mod_pos(i, 3)];
             for (int e = tid; e < CPW * CPW; e += CBLOCK) {
                 const int xx = x0 - 3 + e % CPW, yy = y0 - 3 + e / CPW;
                 const bool in = xx >= -3 && xx <= L.nx + 4 && yy >= -3 && yy <= L.ny + 4 && i >= -3 && i <= L.nz + 4;
                 P[e] = in ? d.phi[L.at(xx, yy, i)] : 0.0;
+            }
+            const int j = i - 1;
+            for (int e = tid; e < CNW * CNW; e += CBLOCK) {
+                const int xx = x0 - 2 + e % CNW, yy = y0 - 2 + e / CNW;
+                int8_t w = 1;
+                int r = -1;
+                if (xx >= -1 && xx <= L.nx + 2 && yy >= -1 && yy <= L.ny + 2 && j >= -1 && j <= L.nz + 2) {
+                    const int o = L.at(xx, yy, j);
+                    w = d.walls[o] == 0 ? 0 : 1;
+                    r = role[o];
+                }
+                sbw[e] = w;
+                sbr[e] = r;
+            }
+            const int j2 = i - 2;
+            for (int e = tid; e < CXW * CXW; e += CBLOCK) {
+                const int xx = x0 - 1 + e % CXW, yy = y0 - 1 + e / CXW;
+                int r = -1;
+                if (xx >= 0 && xx <= L.nx + 1 && yy >= 0 && yy <= L.ny + 1 && j2 >= 0 && j2 <= L.nz + 1)
+                    r = role[L.at(xx, yy, j2)];
+                scr[e] = r;
+            }
+            const int k = i - 3;
+            if (tid < CT * CT) {
+                const int xx = x0 + tid % CT, yy = y0 + tid / CT;
+                int m = -1;
+                if (xx <= L.nx && yy <= L.ny && k >= 1 && k <= L.nz) {
+                    const int o = L.at(xx, yy, k);
+                    const int mm = d.map[o];
+                    if (d.walls[o] == 0 && mm >= 0 && mm < d.nf) m = mm;
+                }
+                sdm[tid] = m;
             }
         }
         __syncthreads();
@@ -1090,31 +1129,28 @@ __global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, Sol
             const double* P1 = sphi[mod_pos(j, 3)];
             const double* P2 = sphi[mod_pos(j + 1, 3)];
             NT(*S)[CNW * CNW] = snrm[mod_pos(j, 4)];
+            NT* SC = scn[mod_pos(j, 4)];
             for (int e = tid; e < CNW * CNW; e += CBLOCK) {
                 const int ex = e % CNW, ey = e / CNW;
-                const int xx = x0 - 2 + ex, yy = y0 - 2 + ey;
                 double cx = 0.0, cy = 0.0, cz = 0.0, cn = 0.0;
-                if (xx >= -1 && xx <= L.nx + 2 && yy >= -1 && yy <= L.ny + 2 && j >= -1 && j <= L.nz + 2) {
-                    const int o = L.at(xx, yy, j);
-                    if (d.walls[o] == 0) {
-                        const int c0 = (ey + 1) * CPW + (ex + 1);
-                        auto A = [&](int a, int bb, int c) {
-                            const double* Pz = c < 0 ? P0 : (c > 0 ? P2 : P1);
-                            return Pz[c0 + a + bb * CPW];
-                        };
-                        cx = dx4f(A);
-                        cy = dy4f(A);
-                        cz = dz4f(A);
-                        normalise(cx, cy, cz, cn);
-                        const int r = role[o];
-                        if (r >= 0 && cn > 1e-6)
-                            wet(cx, cy, cz, fl.nw[3 * r], fl.nw[3 * r + 1], fl.nw[3 * r + 2], fl.cos_t[r], fl.sin_t[r]);
-                    }
+                if (sbw[e] == 0) {
+                    const int c0 = (ey + 1) * CPW + (ex + 1);
+                    auto A = [&](int a, int bb, int c) {
+                        const double* Pz = c < 0 ? P0 : (c > 0 ? P2 : P1);
+                        return Pz[c0 + a + bb * CPW];
+                    };
+                    cx = dx4f(A);
+                    cy = dy4f(A);
+                    cz = dz4f(A);
+                    normalise(cx, cy, cz, cn);
+                    const int r = sbr[e];
+                    if (r >= 0 && cn > 1e-6)
+                        wet(cx, cy, cz, fl.nw[3 * r], fl.nw[3 * r + 1], fl.nw[3 * r + 2], fl.cos_t[r], fl.sin_t[r]);
                 }
                 S[0][e] = (NT)cx;
                 S[1][e] = (NT)cy;
                 S[2][e] = (NT)cz;
-                S[3][e] = (NT)cn;
+                if (ex >= 2 && ex < CT + 2 && ey >= 2 && ey < CT + 2) SC[(ey - 2) * CT + (ex - 2)] = (NT)cn;
             }
         }
         __syncthreads();
@@ -1122,12 +1158,10 @@ __global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, Sol
         const int j2 = i - 2;
         if (j2 >= k0 - 1) {
             NT(*S)[CNW * CNW] = snrm[mod_pos(j2, 4)];
-            for (int e = tid; e < (CT + 2) * (CT + 2); e += CBLOCK) {
-                const int ex = e % (CT + 2) + 1, ey = e / (CT + 2) + 1;   // in the normal tile
-                const int xx = x0 - 2 + ex, yy = y0 - 2 + ey;
-                if (xx < 0 || xx > L.nx + 1 || yy < 0 || yy > L.ny + 1 || j2 < 0 || j2 > L.nz + 1) continue;
-                const int r = role[L.at(xx, yy, j2)];
+            for (int e = tid; e < CXW * CXW; e += CBLOCK) {
+                const int r = scr[e];
                 if (r > -2) continue;
+                const int ex = e % CXW + 1, ey = e / CXW + 1;   // in the normal tile
                 const int sidx = -r - 2;
                 const unsigned mk = s1.mask[sidx];
                 double ax = 0.0, ay = 0.0, az = 0.0;
@@ -1149,31 +1183,28 @@ __global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, Sol
         __syncthreads();
         // curvature force of the interior fluid nodes of plane k = i - 3 (k_curvature)
         const int k = i - 3;
-        if (k >= k0 && k <= k1) {
-            const int ex = tid % CT + 2, ey = tid / CT + 2;
-            const int xx = x0 - 2 + ex, yy = y0 - 2 + ey;
-            if (xx <= L.nx && yy <= L.ny) {
-                const int o = L.at(xx, yy, k);
-                const int m = d.map[o];
-                if (d.walls[o] == 0 && m >= 0 && m < d.nf) {
-                    const NT(*Sm)[CNW * CNW] = snrm[mod_pos(k - 1, 4)];
-                    const NT(*Sc)[CNW * CNW] = snrm[mod_pos(k, 4)];
-                    const NT(*Sp)[CNW * CNW] = snrm[mod_pos(k + 1, 4)];
-                    const int c0 = ey * CNW + ex;
-                    auto comp = [&](int cmp) {
-                        return [=](int a, int bb, int c) {
-                            const NT(*Sz)[CNW * CNW] = c < 0 ? Sm : (c > 0 ? Sp : Sc);
-                            return (double)Sz[cmp][c0 + a + bb * CNW];
-                        };
+        if (k >= k0 && k <= k1 && tid < CT * CT) {
+            const int m = sdm[tid];
+            if (m >= 0) {
+                const int ex = tid % CT + 2, ey = tid / CT + 2;
+                const NT(*Sm)[CNW * CNW] = snrm[mod_pos(k - 1, 4)];
+                const NT(*Sc)[CNW * CNW] = snrm[mod_pos(k, 4)];
+                const NT(*Sp)[CNW * CNW] = snrm[mod_pos(k + 1, 4)];
+                const int c0 = ey * CNW + ex;
+                auto comp = [&](int cmp) {
+                    return [=](int a, int bb, int c) {
+                        const NT(*Sz)[CNW * CNW] = c < 0 ? Sm : (c > 0 ? Sp : Sc);
+                        return (double)Sz[cmp][c0 + a + bb * CNW];
                     };
-                    const double curv = curvature_f(comp(0), comp(1), comp(2));
-                    d.force[m] = (double)Sc[0][c0];
-                    d.force[d.fs + m] = (double)Sc[1][c0];
-                    d.force[2 * d.fs + m] = (double)Sc[2][c0];
-                    d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)Sc[3][c0];
-                }
+                };
+                const double curv = curvature_f(comp(0), comp(1), comp(2));
+                d.force[m] = (double)Sc[0][c0];
+                d.force[d.fs + m] = (double)Sc[1][c0];
+                d.force[2 * d.fs + m] = (double)Sc[2][c0];
+                d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)scn[mod_pos(k, 4)][tid];
             }
         }
+        __syncthreads();   // the next step's first stage rewrites the per-plane tables
     }
 }
 
@@ -1437,7 +1468,10 @@ int launch_gradient_t(mflbm_ctx* c) {
         const int ntx = (L.nx + CT - 1) / CT, nty = (L.ny + CT - 1) / CT, nseg = (L.nz + CSEG - 1) / CSEG;
         const long nblk = (long)ntx * nty * nseg;
         const int per = (int)((nblk + NXCD - 1) / NXCD);
-        LAUNCH(c, k_chain_force<NT>, NXCD * per, c->d, c->fl, c->sl1, (const int*)c->role, c->p.gamma, ntx, nty, per);
+        hipLaunchKernelGGL(k_chain_force<NT>, dim3(NXCD * per), dim3(CBLOCK), 0, c->stream, c->d, c->fl, c->sl1,
+                           (const int*)c->role, c->p.gamma, ntx, nty, per);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(c, MFLBM_ERR_HIP, "launch k_chain_force: %s", hipGetErrorString(e));
         return MFLBM_OK;
     }
     const int rc = launch_normals_dense<NT>(c);
