@@ -1640,7 +1640,7 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     c->p = *prm;
     c->fp32 = prm->precision != 0;
     {
-        // the fused chain is bit-identical but slower on the 256^3 benchmark (1.23 vs 0.42 ms; the lattice
+        // the fused chain is bit-identical but slower on the 256^3 benchmark (0.86 vs 0.42 ms; the lattice
         // update it feeds is 8 % faster): the list kernels stay the default
         const char* e = std::getenv("MFLBM_CHAIN");
         c->chain_fused = e && std::strcmp(e, "fused") == 0;
