@@ -1078,50 +1078,65 @@ __global__ void __launch_bounds__(CBLOCK) k_chain_force(Dev d, FluidList fl, Sol
     const int x0 = 1 + tx * CT, y0 = 1 + ty * CT;
     const int k0 = 1 + sg * CSEG, k1 = min(L.nz, k0 + CSEG - 1);
     const int tid = (int)threadIdx.x;
-    for (int i = k0 - 3; i <= k1 + 3; ++i) {
-        // every global read of the step in one stage: phi of plane i (tile + halo 3; 0 outside the padded
-        // array, never used), walls and roles of planes i - 1 and i - 2, compact indices of plane i - 3
-        {
-            double* P = sphi[mod_pos(i, 3)];
-            for (int e = tid; e < CPW * CPW; e += CBLOCK) {
+    // Every global read of step i (phi of plane i with halo 3, walls and roles of planes i - 1 and i - 2, compact
+    // indices of plane i - 3) goes to registers one step ahead: the reads of step i + 1 are in flight while the
+    // stages of step i run on LDS.  Element e of each table is handled by thread e (e < CBLOCK) and e + CBLOCK.
+    static_assert(CPW * CPW <= 2 * CBLOCK && CNW * CNW <= CBLOCK && CXW * CXW <= CBLOCK, "tile tables per thread");
+    double rphi[2];
+    int8_t rbw;
+    int rbr, rcr, rdm;
+    auto fetch = [&](int ii) {
+        for (int h = 0; h < 2; ++h) {
+            const int e = tid + h * CBLOCK;
+            double v = 0.0;
+            if (e < CPW * CPW) {
                 const int xx = x0 - 3 + e % CPW, yy = y0 - 3 + e / CPW;
-                const bool in = xx >= -3 && xx <= L.nx + 4 && yy >= -3 && yy <= L.ny + 4 && i >= -3 && i <= L.nz + 4;
-                P[e] = in ? d.phi[L.at(xx, yy, i)] : 0.0;
+                if (xx >= -3 && xx <= L.nx + 4 && yy >= -3 && yy <= L.ny + 4 && ii >= -3 && ii <= L.nz + 4)
+                    v = d.phi[L.at(xx, yy, ii)];
             }
-            const int j = i - 1;
-            for (int e = tid; e < CNW * CNW; e += CBLOCK) {
-                const int xx = x0 - 2 + e % CNW, yy = y0 - 2 + e / CNW;
-                int8_t w = 1;
-                int r = -1;
-                if (xx >= -1 && xx <= L.nx + 2 && yy >= -1 && yy <= L.ny + 2 && j >= -1 && j <= L.nz + 2) {
-                    const int o = L.at(xx, yy, j);
-                    w = d.walls[o] == 0 ? 0 : 1;
-                    r = role[o];
-                }
-                sbw[e] = w;
-                sbr[e] = r;
-            }
-            const int j2 = i - 2;
-            for (int e = tid; e < CXW * CXW; e += CBLOCK) {
-                const int xx = x0 - 1 + e % CXW, yy = y0 - 1 + e / CXW;
-                int r = -1;
-                if (xx >= 0 && xx <= L.nx + 1 && yy >= 0 && yy <= L.ny + 1 && j2 >= 0 && j2 <= L.nz + 1)
-                    r = role[L.at(xx, yy, j2)];
-                scr[e] = r;
-            }
-            const int k = i - 3;
-            if (tid < CT * CT) {
-                const int xx = x0 + tid % CT, yy = y0 + tid / CT;
-                int m = -1;
-                if (xx <= L.nx && yy <= L.ny && k >= 1 && k <= L.nz) {
-                    const int o = L.at(xx, yy, k);
-                    const int mm = d.map[o];
-                    if (d.walls[o] == 0 && mm >= 0 && mm < d.nf) m = mm;
-                }
-                sdm[tid] = m;
+            rphi[h] = v;
+        }
+        rbw = 1;
+        rbr = -1;
+        if (tid < CNW * CNW) {
+            const int xx = x0 - 2 + tid % CNW, yy = y0 - 2 + tid / CNW, jj = ii - 1;
+            if (xx >= -1 && xx <= L.nx + 2 && yy >= -1 && yy <= L.ny + 2 && jj >= -1 && jj <= L.nz + 2) {
+                const int o = L.at(xx, yy, jj);
+                rbw = d.walls[o] == 0 ? 0 : 1;
+                rbr = role[o];
             }
         }
+        rcr = -1;
+        if (tid < CXW * CXW) {
+            const int xx = x0 - 1 + tid % CXW, yy = y0 - 1 + tid / CXW, jj = ii - 2;
+            if (xx >= 0 && xx <= L.nx + 1 && yy >= 0 && yy <= L.ny + 1 && jj >= 0 && jj <= L.nz + 1)
+                rcr = role[L.at(xx, yy, jj)];
+        }
+        rdm = -1;
+        if (tid < CT * CT) {
+            const int xx = x0 + tid % CT, yy = y0 + tid / CT, kk = ii - 3;
+            if (xx <= L.nx && yy <= L.ny && kk >= 1 && kk <= L.nz) {
+                const int o = L.at(xx, yy, kk);
+                const int mm = d.map[o];
+                if (d.walls[o] == 0 && mm >= 0 && mm < d.nf) rdm = mm;
+            }
+        }
+    };
+    fetch(k0 - 3);
+    for (int i = k0 - 3; i <= k1 + 3; ++i) {
+        {
+            double* P = sphi[mod_pos(i, 3)];
+            for (int h = 0; h < 2; ++h)
+                if (tid + h * CBLOCK < CPW * CPW) P[tid + h * CBLOCK] = rphi[h];
+            if (tid < CNW * CNW) {
+                sbw[tid] = rbw;
+                sbr[tid] = rbr;
+            }
+            if (tid < CXW * CXW) scr[tid] = rcr;
+            if (tid < CT * CT) sdm[tid] = rdm;
+        }
         __syncthreads();
+        if (i < k1 + 3) fetch(i + 1);
         // normals of plane j = i - 1 at the non-solid nodes of [-1, n+2]^3 (k_normals), 0 elsewhere
         const int j = i - 1;
         if (j >= k0 - 2) {
@@ -1640,7 +1655,7 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     c->p = *prm;
     c->fp32 = prm->precision != 0;
     {
-        // the fused chain is bit-identical but slower on the 256^3 benchmark (0.86 vs 0.42 ms; the lattice
+        // the fused chain is bit-identical but slower on the 256^3 benchmark (0.78 vs 0.42 ms; the lattice
         // update it feeds is 8 % faster): the list kernels stay the default
         const char* e = std::getenv("MFLBM_CHAIN");
         c->chain_fused = e && std::strcmp(e, "fused") == 0;
