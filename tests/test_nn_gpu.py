@@ -132,7 +132,7 @@ def test_training_lowers_the_loss():
     t[..., 2] = (k + np.roll(k, 1, 1) + np.roll(k, -1, 1) + np.roll(k, 1, 2) + np.roll(k, -1, 2)) / 5.0 * k
     data = Dataset(_bf(x), _bf(t), np.zeros(n, np.int64), 0.0)
     net = WarmStartNet((8, 16), seed=1)
-    hist = train(net, data, epochs=30, batch_size=4, lr=3e-3)
+    hist = train(net, data, epochs=30, batch_size=4, lr=3e-3, lr_final=3e-3, mirror=False)
     print(f"[nn training] loss {hist[0]:.4g} -> {hist[-1]:.4g}", flush=True)
     assert hist[-1] < 0.25 * hist[0]
 
