@@ -147,7 +147,7 @@ def test_trained_network_shortens_runs_on_unseen_packs():
     _t()
     from mflbm import batch
     from mflbm.singlephase import SPCase, SPSimulation, permeability
-    from mflbm.warmstart import WarmStartNet, make_dataset, train, warm_state
+    from mflbm.warmstart import T_PROBE, WarmStartNet, make_dataset, probe_fields, train, warm_state
     la_nu, force, side = 0.1, 1e-6, 32
     data = make_dataset(192, side=side, seed=3, la_nu=la_nu, force=force, per_batch=192)
     net = WarmStartNet((32, 64, 128), seed=2)
@@ -164,11 +164,15 @@ def test_trained_network_shortens_runs_on_unseen_packs():
                "convergence_criteria": 1e-7}
         case = SPCase(ctl, s.walls, "fp64")
         res = {}
-        for mode, st in (("cold", case.collision_equilibrium_state()), ("warm", warm_state(net, case, la_nu, force))):
-            sim = SPSimulation(case, st)
-            if mode == "warm":
+        for mode in ("cold", "warm"):
+            sim = SPSimulation(case, case.collision_equilibrium_state())
+            n0 = 1
+            if mode == "warm":   # T_PROBE steps from rest, then restarted from the network's fields
+                probe = probe_fields(sim, case, la_nu, force)
+                sim.upload(warm_state(net, case, la_nu, force, probe))
                 u0 = sim.macro()
-            n, status, hist_q = sim.run_steady(1, 200000, 100)
+                n0 = T_PROBE + 1
+            n, status, hist_q = sim.run_steady(n0, 200000, 100)
             assert status == "converged", (mode, status)
             res[mode] = (n, permeability(case, hist_q[-1][1]), sim.macro())
             steps[mode] += n
