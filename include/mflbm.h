@@ -240,6 +240,12 @@ int mflbm_sp_timing_enable(mflbm_sp_ctx* ctx, int32_t enable);
  * convective-outlet history f_convec_bc(:, :, q) = f_q(:, :, nz) (may be NULL). */
 int mflbm_sp_init_fields(const mflbm_sp_params* params, const double* rho0, const double* u0, const double* v0,
                          const double* w0, double* const* f, double* f_convec);
+/* No reference counterpart (the reference starts at the lattice-weight equilibrium): the state of a context set
+ * on the device to the MRT collision's equilibrium of (rho, u, v, w) (halo-1 host arrays, NULL = rho 1 / at rest),
+ * f_q = (feq_q - w_q rho) + REST_q rho with REST the collision's rest state, and every solid slot facing a fluid
+ * node set to that node's opposite population (its first bounce-back); for starting a run from predicted fields
+ * without building and uploading 19 population arrays.  Leaves f_convec unchanged. */
+int mflbm_sp_set_equilibrium(mflbm_sp_ctx* ctx, const double* rho, const double* u, const double* v, const double* w);
 
 /* ---------------------------------------------------------------------------------------------------------
  * Batched single-phase permeability runs (no reference counterpart: the reference runs one sample per job
@@ -305,6 +311,11 @@ int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* bias, void* 
 int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
                        int32_t w, int32_t cin, int32_t cout, void* stream);
 int mflbm_nn_relu_bwd(void* g, const void* y, int64_t n, void* stream);
+/* channels 0..2 of the BF16 channels-last input x (d, h, w, cin): fluid indicator (fluid: int8 1 fluid, 0 solid),
+ * d / 8 and (d / 8)^2, d the Euclidean distance to the nearest solid node capped at 16 (wrapped along the axes
+ * flagged periodic); work: 2 d h w int32 of scratch */
+int mflbm_nn_distance_features(const int8_t* fluid, int32_t* work, void* x, int32_t d, int32_t h, int32_t w,
+                               int32_t cin, int32_t zper, int32_t yper, int32_t xper, void* stream);
 /* 2x average pooling of the fine tensor (bwd = 0) or its adjoint into the fine gradient (bwd = 1); (d, h, w) fine */
 int mflbm_nn_pool2(const void* x, void* y, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c, int32_t bwd, void* stream);
 /* 2x nearest upsampling into the fine tensor (bwd = 0) or its adjoint into the coarse gradient; (d, h, w) fine */

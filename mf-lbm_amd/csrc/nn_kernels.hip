@@ -386,6 +386,69 @@ extern "C" int mflbm_nn_relu_bwd(void* g, const void* y, int64_t n, void* stream
     return launched((hipStream_t)stream);
 }
 
+// Input features of the warm-start network: the fluid indicator, d / 8 and (d / 8)^2 with d the Euclidean
+// distance from a node to the nearest solid node capped at 16, in channels 0..2 of a channels-last BF16 tensor.
+// The squared distance is a separable minimum over offsets of at most 16 per axis (exact below the cap: the
+// nearest solid within 16 lies inside that cube), wrapped along periodic axes and cut at the others.
+constexpr int DCAP = 16;
+constexpr int DBIG = 1 << 20;
+__device__ __forceinline__ int wrap_or_cut(int a, int n, int per) { return per ? ((a % n) + n) % n : (a >= 0 && a < n ? a : -1); }
+__global__ void k_dist_x(const int8_t* __restrict__ fluid, int* __restrict__ g, int d, int h, int w, int per) {
+    const long t = (long)blockIdx.x * NT + threadIdx.x;
+    if (t >= (long)d * h * w) return;
+    const int x = (int)(t % w);
+    const long row = t - x;
+    int best = DBIG;
+    for (int a = -DCAP; a <= DCAP; ++a) {
+        const int xx = wrap_or_cut(x + a, w, per);
+        if (xx >= 0 && fluid[row + xx] == 0) best = min(best, a * a);
+    }
+    g[t] = best;
+}
+__global__ void k_dist_y(const int* __restrict__ gin, int* __restrict__ gout, int d, int h, int w, int per) {
+    const long t = (long)blockIdx.x * NT + threadIdx.x;
+    if (t >= (long)d * h * w) return;
+    const int x = (int)(t % w), y = (int)((t / w) % h);
+    const long base = t - (long)y * w;
+    int best = DBIG;
+    for (int a = -DCAP; a <= DCAP; ++a) {
+        const int yy = wrap_or_cut(y + a, h, per);
+        if (yy >= 0) best = min(best, gin[base + (long)yy * w] + a * a);
+    }
+    gout[t] = best;
+    (void)x;
+}
+__global__ void k_dist_z_features(const int8_t* __restrict__ fluid, const int* __restrict__ gin, __bf16* __restrict__ out,
+                                  int d, int h, int w, int cin, int per) {
+    const long t = (long)blockIdx.x * NT + threadIdx.x;
+    const long plane = (long)h * w;
+    if (t >= (long)d * plane) return;
+    const int z = (int)(t / plane);
+    const long base = t - (long)z * plane;
+    int best = DBIG;
+    for (int a = -DCAP; a <= DCAP; ++a) {
+        const int zz = wrap_or_cut(z + a, d, per);
+        if (zz >= 0) best = min(best, gin[base + (long)zz * plane] + a * a);
+    }
+    const double dist = fmin(sqrt((double)best), (double)DCAP) / 8.0;
+    __bf16* o = out + t * cin;
+    o[0] = f2bf((float)fluid[t]);
+    o[1] = f2bf((float)dist);
+    o[2] = f2bf((float)(dist * dist));
+}
+extern "C" int mflbm_nn_distance_features(const int8_t* fluid, int32_t* work, void* x, int32_t d, int32_t h, int32_t w,
+                                          int32_t cin, int32_t zper, int32_t yper, int32_t xper, void* stream) {
+    if (!fluid || !work || !x || d < 1 || h < 1 || w < 1 || cin < 3) return MFLBM_ERR_ARG;
+    const long cnt = (long)d * h * w;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_dist_x, dim3(blocks(cnt)), dim3(NT), 0, st, fluid, (int*)work, (int)d, (int)h, (int)w, (int)xper);
+    hipLaunchKernelGGL(k_dist_y, dim3(blocks(cnt)), dim3(NT), 0, st, (const int*)work, (int*)work + cnt, (int)d, (int)h,
+                       (int)w, (int)yper);
+    hipLaunchKernelGGL(k_dist_z_features, dim3(blocks(cnt)), dim3(NT), 0, st, fluid, (const int*)work + cnt, (__bf16*)x,
+                       (int)d, (int)h, (int)w, (int)cin, (int)zper);
+    return launched(st);
+}
+
 extern "C" int mflbm_nn_pool2(const void* x, void* y, int32_t n, int32_t d, int32_t h, int32_t w, int32_t c, int32_t bwd,
                               void* stream) {
     if (!x || !y || d % 2 || h % 2 || w % 2 || c < 1) return MFLBM_ERR_ARG;

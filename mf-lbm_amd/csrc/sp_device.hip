@@ -576,6 +576,88 @@ extern "C" int mflbm_sp_upload_state(mflbm_sp_ctx* c, const double* const* f, co
     return MFLBM_OK;
 }
 
+// Populations at the MRT collision's equilibrium of prescribed fields, into the dense staging (halo-1 box):
+// the second-order equilibrium of mflbm_sp_init_fields (same expressions, same order) shifted to the
+// collision's rest state, f_q = (feq_q - w_q rho) + REST_q rho.  fin: u, v, w, rho (dense; has = which are given).
+__global__ void k_sp_equilibrium_dense(Layout L, const double* __restrict__ fin, long ps, int has, double* dense) {
+    const long nb = (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nb) return;
+    const int i = (int)(t % (L.nx + 2)), j = (int)((t / (L.nx + 2)) % (L.ny + 2)), k = (int)(t / ((long)(L.nx + 2) * (L.ny + 2)));
+    const int o = L.at(i, j, k);
+    const double u = (has & 1) ? fin[o] : 0.0, v = (has & 2) ? fin[ps + o] : 0.0, w = (has & 4) ? fin[2 * ps + o] : 0.0;
+    const double rho1 = (has & 8) ? fin[3 * ps + o] : 1.0;
+    const double usqrt = u * u + v * v + w * w;
+    const double w0e = lat::W0, w1 = lat::W1, w2 = lat::W2;
+    auto axis = [&](double c, double cu) { return rho1 * w1 + rho1 * w1 * (3.0 * cu + 4.5 * c * c - 1.5 * usqrt); };
+    auto diag = [&](double cu) { return rho1 * w2 + rho1 * w2 * (3.0 * cu + 4.5 * cu * cu - 1.5 * usqrt); };
+    double f[19];
+    f[0] = rho1 * w0e + rho1 * w0e * (-1.5 * usqrt);
+    f[1] = axis(u, u);
+    f[2] = axis(u, -u);
+    f[3] = axis(v, v);
+    f[4] = axis(v, -v);
+    f[5] = axis(w, w);
+    f[6] = axis(w, -w);
+    f[7] = diag(u + v);
+    f[8] = diag(-u + v);
+    f[9] = diag(u - v);
+    f[10] = diag(-u - v);
+    f[11] = diag(u + w);
+    f[12] = diag(-u + w);
+    f[13] = diag(u - w);
+    f[14] = diag(-u - w);
+    f[15] = diag(v + w);
+    f[16] = diag(-v + w);
+    f[17] = diag(v - w);
+    f[18] = diag(-v - w);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) dense[q * ps + o] = (f[q] - lat::W[q] * rho1) + rest_weight(q) * rho1;
+}
+// A slot (y, q) of a node that is not an interior fluid node, facing an interior fluid node x = y + e_q, takes
+// x's population opp(q): what x pulls from it at the first (odd) step is the population x sent to the wall.
+__global__ void k_sp_bounce_dense(Layout L, const int8_t* __restrict__ walls, long ps, double* dense) {
+    const long nb = (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nb) return;
+    const int i = (int)(t % (L.nx + 2)), j = (int)((t / (L.nx + 2)) % (L.ny + 2)), k = (int)(t / ((long)(L.nx + 2) * (L.ny + 2)));
+    auto fluid = [&](int a, int b, int cz) {
+        return a >= 1 && a <= L.nx && b >= 1 && b <= L.ny && cz >= 1 && cz <= L.nz && walls[L.at(a, b, cz)] == 0;
+    };
+    if (fluid(i, j, k)) return;
+    const int o = L.at(i, j, k);
+    for (int q = 1; q < 19; ++q) {
+        const int a = i + lat::EX[q], b = j + lat::EY[q], cz = k + lat::EZ[q];
+        if (fluid(a, b, cz)) dense[q * ps + o] = dense[lat::OPP[q] * ps + L.at(a, b, cz)];
+    }
+}
+
+extern "C" int mflbm_sp_set_equilibrium(mflbm_sp_ctx* c, const double* rho, const double* u, const double* v,
+                                        const double* w) {
+    if (!c) return MFLBM_ERR_ARG;
+    if (!c->have_geom) return fail(c, MFLBM_ERR_STATE, "upload the geometry before the state");
+    const Layout& L = c->L;
+    const long ps = c->ps;
+    // the macroscopic-field buffers are free between compute_macro calls: they take the prescribed fields
+    std::vector<double> buf(L.total, 0.0);
+    const double* src[4] = {u, v, w, rho};
+    int has = 0;
+    for (int m = 0; m < 4; ++m) {
+        if (!src[m]) continue;
+        has |= 1 << m;
+        pack_host(L, 1, src[m], buf.data());
+        HIPCHK(c, hipMemcpy(c->fields + m * ps, buf.data(), sizeof(double) * L.total, hipMemcpyHostToDevice));
+    }
+    const long nb = (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2);
+    LAUNCH(c, k_sp_equilibrium_dense, nblocks(nb), L, (const double*)c->fields, ps, has, c->dense_pdf);
+    LAUNCH(c, k_sp_bounce_dense, nblocks(nb), L, (const int8_t*)c->walls, ps, c->dense_pdf);
+    const int rc = c->fp32 ? sp_gather_t<float>(c) : sp_gather_t<double>(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return MFLBM_OK;
+}
+
 extern "C" int mflbm_sp_download_state(mflbm_sp_ctx* c, double* const* f, double* f_convec) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");

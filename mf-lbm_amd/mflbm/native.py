@@ -110,6 +110,7 @@ _SP_ENTRY_POINTS = {
     "synchronize": (c_int, [C.c_void_p]),
     "timing": (c_int, [C.c_void_p, c_int, P_dbl, C.POINTER(C.c_int64)]),
     "timing_enable": (c_int, [C.c_void_p, c_int]),
+    "set_equilibrium": (c_int, [C.c_void_p, P_dbl, P_dbl, P_dbl, P_dbl]),
 }
 
 # batched single-phase entry points, bound as prefix + "spb_" + name
@@ -144,6 +145,7 @@ _NN_ENTRY_POINTS = {
     "mflbm_nn_mse": (c_int, [_VP, _VP, _VP, _VP, C.c_int64, c_int, c_int, C.c_float, _VP]),
     "mflbm_nn_adam": (c_int, [_VP, _VP, _VP, _VP, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, c_int, _VP]),
     "mflbm_nn_mfma_probe": (c_int, [_VP, _VP, _VP, _VP]),
+    "mflbm_nn_distance_features": (c_int, [_VP, _VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
 }
 
 _HOST_ENTRY_POINTS = {

@@ -280,6 +280,20 @@ class SPSimulation:
         self._keep = st
         self._call("upload_state", native.ptr_array(list(st.f)), dptr(np.ascontiguousarray(st.f_convec)))
 
+    def set_equilibrium(self, rho: Optional[np.ndarray] = None, velocity: Optional[tuple] = None) -> None:
+        """The state set on the device to SPCase.collision_equilibrium_state(rho, velocity) (halo-1 arrays;
+        mflbm_sp_set_equilibrium): only the given fields cross PCIe."""
+        args = []
+        keep = []
+        for a in [rho] + list(velocity if velocity is not None else (None, None, None)):
+            if a is None:
+                args.append(None)
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                keep.append(a)
+                args.append(dptr(a))
+        self._call("set_equilibrium", args[0], args[1], args[2], args[3])
+
     def state(self) -> SPState:
         st = SPState.empty(self.nx, self.ny, self.nz)
         self._call("download_state", native.ptr_array(list(st.f)), dptr(st.f_convec))

@@ -44,9 +44,25 @@ T_PROBE = 200
 NIN = NFEAT + 4
 
 
+def device_features(x, fluid: np.ndarray, periodic: Tuple[bool, bool, bool] = (True, False, False)) -> None:
+    """Write the NFEAT input features of ``fluid`` ((nz, ny, nx) bool) into channels 0..2 of ``x``, a contiguous
+    (nz, ny, nx, CIN) BF16 device tensor, with mflbm_nn_distance_features (the GPU form of ``features``)."""
+    torch = _torch()
+    lib = native.load()
+    nz, ny, nx = fluid.shape
+    fl = torch.from_numpy(np.ascontiguousarray(fluid, dtype=np.int8)).to("cuda")
+    work = torch.empty(2 * fl.numel(), dtype=torch.int32, device="cuda")
+    rc = lib.mflbm_nn_distance_features(_p(fl), _p(work), _p(x), nz, ny, nx, x.shape[-1], int(periodic[0]),
+                                        int(periodic[1]), int(periodic[2]),
+                                        C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"mflbm_nn_distance_features failed with code {rc}")
+
+
 def features(fluid: np.ndarray, periodic: Tuple[bool, bool, bool] = (True, False, False)) -> np.ndarray:
     """(nz, ny, nx) bool -> (nz, ny, nx, NFEAT) FP32: indicator, d / 8, (d / 8)^2 with d the Euclidean distance
-    to the nearest solid node (0 at solids), capped at DCAP; periodic axes are wrapped by DCAP layers first."""
+    to the nearest solid node (0 at solids), capped at DCAP; periodic axes are wrapped by DCAP layers first.
+    The host form of ``device_features`` (which the network uses), kept to check it."""
     from scipy import ndimage
     pad = [(DCAP, DCAP) if p else (0, 0) for p in periodic]
     fp = np.pad(fluid, pad, mode="wrap") if any(periodic) else fluid
@@ -285,7 +301,7 @@ class WarmStartNet:
         torch = self.torch
         nz, ny, nx = fluid.shape
         x = torch.zeros((1, nz, ny, nx, CIN), dtype=torch.bfloat16, device="cuda")
-        x[0, ..., :NFEAT] = torch.from_numpy(features(fluid > 0, periodic)).to("cuda", torch.bfloat16)
+        device_features(x[0], fluid > 0, periodic)
         if probe is not None:
             x[0, ..., NFEAT:NIN] = torch.from_numpy(np.ascontiguousarray(probe, np.float32)).to("cuda", torch.bfloat16)
         with torch.no_grad():
@@ -342,7 +358,7 @@ def make_dataset(n: int, side: int = 64, seed: int = 11, la_nu: float = 0.1, for
             b._call("download_macro", i, native.dptr(u), native.dptr(v), native.dptr(w), native.dptr(rho))
             fl = (batch.global_walls(s) == 0).astype(np.float32)
             j = done + i
-            x[j, ..., :NFEAT] = torch.from_numpy(features(fl > 0, (True, bool(s.jper), False))).to("cuda", torch.bfloat16)
+            device_features(x[j], fl > 0, (True, bool(s.jper), False))
             x[j, ..., NFEAT:NIN] = torch.from_numpy(probes[i]).to("cuda", torch.bfloat16)
             tgt = normalized_fields(u, v, w, rho, fl > 0, la_nu, force)
             t[j, ..., :USED] = torch.from_numpy(tgt.astype(np.float32)).to("cuda", torch.bfloat16)
@@ -408,15 +424,22 @@ def probe_fields(sim, case, la_nu: float, force: float) -> np.ndarray:
     return normalized_fields(m["u"], m["v"], m["w"], m["rho"], case.walls_global == 0, la_nu, force)
 
 
-def warm_state(net: WarmStartNet, case, la_nu: float, force: float, probe: Optional[np.ndarray] = None):
-    """Initial state of ``case`` (an SPCase) from the network: u = prediction * F / nu, rho = 1 + P_SCALE F p
-    with p shifted to zero mean over the fluid, populations at the collision's equilibrium; ``probe`` the
-    fields after T_PROBE steps of the same run (``probe_fields``)."""
+def warm_fields(net: WarmStartNet, case, la_nu: float, force: float, probe: Optional[np.ndarray] = None):
+    """(rho, (u, v, w)) halo-1 arrays of ``case`` (an SPCase) from the network: u = prediction * F / nu,
+    rho = 1 + P_SCALE F p with p shifted to zero mean over the fluid; ``probe`` the fields after T_PROBE steps
+    of the same run (``probe_fields``)."""
     fl = (case.walls_global == 0)
     y = net.predict(fl.astype(np.float32), (bool(case.rp.kper), bool(case.rp.jper), False), probe).astype(np.float64)
     vel = [np.pad(y[m] * (force / la_nu), 1) for m in range(3)]
     p = y[3] - y[3][fl].mean()
     rho = np.pad(np.where(fl, 1.0 + P_SCALE * force * p, 1.0), 1, constant_values=1.0)
+    return rho, vel
+
+
+def warm_state(net: WarmStartNet, case, la_nu: float, force: float, probe: Optional[np.ndarray] = None):
+    """Host-built initial state from the network's fields (populations at the collision's equilibrium); a
+    running SPSimulation takes the same state on the device with ``sim.set_equilibrium(*warm_fields(...))``."""
+    rho, vel = warm_fields(net, case, la_nu, force, probe)
     return case.collision_equilibrium_state(rho=rho, velocity=vel)
 
 
@@ -473,7 +496,7 @@ def compare_runs(net: WarmStartNet, sample: batch.REVSample, la_nu: float = 0.1,
         if mode == "warm":   # T_PROBE steps from rest, the network, then the run from its fields
             t1 = time.perf_counter()
             probe = probe_fields(sim, case, la_nu, force)
-            sim.upload(warm_state(net, case, la_nu, force, probe))
+            sim.set_equilibrium(*warm_fields(net, case, la_nu, force, probe))
             n0 = T_PROBE + 1
             sim.synchronize()
             tp = time.perf_counter() - t1

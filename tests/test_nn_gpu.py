@@ -192,3 +192,20 @@ def test_trained_network_shortens_runs_on_unseen_packs():
     # the training's FP32 atomics make the network differ in its last bits from run to run; the step counts
     # (monitor every 100 steps) are compared with one period per pack of slack
     assert steps["warm"] <= steps["cold"] + 300
+
+
+@pytest.mark.parametrize("periodic", [(True, False, False), (True, True, False), (False, False, False)])
+def test_distance_features_on_the_device_equal_the_host_transform(periodic):
+    """mflbm_nn_distance_features (separable capped squared-distance minimum) against the exact Euclidean
+    distance transform of scipy on the host (features): equal after the BF16 rounding both go through."""
+    torch = _t()
+    from mflbm.warmstart import CIN, NFEAT, device_features, features
+    rng = np.random.default_rng(21)
+    fluid = rng.random((24, 40, 32)) > 0.08          # sparse solids: distances up to the cap
+    fluid[:, :, :2] = False                          # and a wall slab
+    x = torch.zeros((24, 40, 32, CIN), dtype=torch.bfloat16, device="cuda")
+    device_features(x, fluid, periodic)
+    ref = torch.from_numpy(features(fluid, periodic)).to(torch.bfloat16)
+    got = x[..., :NFEAT].cpu()
+    assert torch.equal(got, ref)
+    assert float(x[..., NFEAT:].float().abs().max()) == 0.0
