@@ -182,6 +182,12 @@ module mflbm_c_binding
             type(c_ptr), value :: ctx
             real(c_double), intent(inout) :: u(*), v(*), w(*), rho(*)
         end function
+        ! the state set on the device to the MRT collision's equilibrium of the fields (halo-1 arrays; any may be
+        ! c_null_ptr: rho 1 / at rest), solid slots by bounce-back of their fluid neighbours
+        integer(c_int) function mflbm_sp_set_equilibrium(ctx, rho, u, v, w) bind(c, name="mflbm_sp_set_equilibrium")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, rho, u, v, w
+        end function
         ! rho0/u0/v0/w0 may be c_null_ptr (rest state at rho = 1); f: 19 addresses; f_convec may be c_null_ptr
         integer(c_int) function mflbm_sp_init_fields(params, rho0, u0, v0, w0, f, f_convec) &
                 bind(c, name="mflbm_sp_init_fields")
