@@ -28,7 +28,9 @@ using namespace mflbm_dev;
 
 namespace {
 
-// base of each population slice: fluid 1 direction q at p[q], fluid 2 at p[19 + q]
+// Population slots: slice q holds, per slot, the pair (fluid 1, fluid 2) of direction q (FP32 storage: (s, a)), so
+// p[q] is the base of the fluid-1 halves and p[19 + q] = p[q] + one element the base of the fluid-2 halves; slot i
+// of either colour is element 2 i from its base.
 struct SlotTable {
     double* p[38];
 };
@@ -93,7 +95,7 @@ template <>
 struct Pop<double> {
     const Dev* d;
     int c0;   // 0 fluid 1, 19 fluid 2
-    __device__ __forceinline__ double& operator()(int q, int o) const { return d->S.p[c0 + q][slot_index(*d, q, o)]; }
+    __device__ __forceinline__ double& operator()(int q, int o) const { return d->S.p[c0 + q][2 * slot_index(*d, q, o)]; }
 };
 struct CRef {
     float* s;
@@ -119,7 +121,7 @@ struct Pop<float> {
     int c0;
     __device__ __forceinline__ CRef operator()(int q, int o) const {
         const int i = slot_index(*d, q, o);
-        return CRef{(float*)d->S.p[q] + i, (float*)d->S.p[19 + q] + i, q, c0 != 0};
+        return CRef{(float*)d->S.p[q] + 2 * i, (float*)d->S.p[19 + q] + 2 * i, q, c0 != 0};
     }
 };
 template <class T>
@@ -167,6 +169,21 @@ __device__ __forceinline__ void st_slot(double* slice, unsigned m, double v) {
 #endif
 }
 
+// The two colours of a slot are adjacent (f_q, g_q or s_q, a_q): one 16-byte (FP64) or 8-byte (FP32) access
+// moves both, which halves the scattered accesses of the odd step.
+__device__ __forceinline__ double2 ld_pair(const double* slice, unsigned m) {
+    return *(const double2*)((const char*)slice + m * 16u);
+}
+__device__ __forceinline__ void st_pair(double* slice, unsigned m, double f, double g) {
+    *(double2*)((char*)slice + m * 16u) = double2{f, g};
+}
+__device__ __forceinline__ float2 ld_pair32(const double* slice, unsigned m) {
+    return *(const float2*)((const char*)slice + m * 8u);
+}
+__device__ __forceinline__ void st_pair32(double* slice, unsigned m, float s, float a) {
+    *(float2*)((char*)slice + m * 8u) = float2{s, a};
+}
+
 #ifndef MFLBM_FUSED_CURVATURE
 #define MFLBM_FUSED_CURVATURE 1
 #endif
@@ -202,8 +219,11 @@ k_collide(Dev d, CollideParams P, int per) {
     }
     double a[19], b[19];
     unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
-    a[0] = ld_slot(S.p[0], m);
-    b[0] = ld_slot(S.p[19], m);
+    {
+        const double2 v = ld_pair(S.p[0], m);
+        a[0] = v.x;
+        b[0] = v.y;
+    }
     if (ODD) {
         const int n = d.cpos[m];
         // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
@@ -218,26 +238,26 @@ k_collide(Dev d, CollideParams P, int per) {
         }
 #pragma unroll
         for (int q = 1; q < 19; ++q) {   // slot q of the upwind node x - e_q = x + e_opp(q)
-            a[q] = ld_slot(S.p[q], nb[lat::OPP[q]]);
-            b[q] = ld_slot(S.p[19 + q], nb[lat::OPP[q]]);
+            const double2 v = ld_pair(S.p[q], nb[lat::OPP[q]]);
+            a[q] = v.x;
+            b[q] = v.y;
         }
     } else {
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
-            a[q] = ld_slot(S.p[lat::OPP[q]], m);
-            b[q] = ld_slot(S.p[19 + lat::OPP[q]], m);
+            const double2 v = ld_pair(S.p[lat::OPP[q]], m);
+            a[q] = v.x;
+            b[q] = v.y;
         }
     }
     const double phi = collide_node_tf(a, b, cnx, cny, cnz, tf, P);
     d.phi[d.cpos[m]] = phi;
-    st_slot(S.p[0], m, a[0]);
-    st_slot(S.p[19], m, b[0]);
+    st_pair(S.p[0], m, a[0], b[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {   // odd: slot opp(q) of x + e_q
         const int qs = ODD ? lat::OPP[q] : q;
         const unsigned ms = ODD ? nb[q] : (unsigned)m;
-        st_slot(S.p[qs], ms, a[q]);
-        st_slot(S.p[19 + qs], ms, b[q]);
+        st_pair(S.p[qs], ms, a[q], b[q]);
     }
 }
 
@@ -265,8 +285,11 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     }
     C s[19], a[19];
     unsigned nb[19];
-    s[0] = slot_ld((const float*)S.p[0], (unsigned)m);
-    a[0] = slot_ld((const float*)S.p[19], (unsigned)m);
+    {
+        const float2 v = ld_pair32(S.p[0], (unsigned)m);
+        s[0] = v.x;
+        a[0] = v.y;
+    }
     if (ODD) {
         const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
         const unsigned lane = (unsigned)m & 63u;
@@ -279,26 +302,26 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
         }
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
-            s[q] = slot_ld((const float*)S.p[q], nb[lat::OPP[q]]);
-            a[q] = slot_ld((const float*)S.p[19 + q], nb[lat::OPP[q]]);
+            const float2 v = ld_pair32(S.p[q], nb[lat::OPP[q]]);
+            s[q] = v.x;
+            a[q] = v.y;
         }
     } else {
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
-            s[q] = slot_ld((const float*)S.p[lat::OPP[q]], (unsigned)m);
-            a[q] = slot_ld((const float*)S.p[19 + lat::OPP[q]], (unsigned)m);
+            const float2 v = ld_pair32(S.p[lat::OPP[q]], (unsigned)m);
+            s[q] = v.x;
+            a[q] = v.y;
         }
     }
     const C phi = collide_node_dev<C>(s, a, cnx, cny, cnz, tf, P);
     d.phi[nn] = (double)phi;
-    slot_st((float*)S.p[0], (unsigned)m, (float)s[0]);
-    slot_st((float*)S.p[19], (unsigned)m, (float)a[0]);
+    st_pair32(S.p[0], (unsigned)m, (float)s[0], (float)a[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
         const int qs = ODD ? lat::OPP[q] : q;
         const unsigned ms = ODD ? nb[q] : (unsigned)m;
-        slot_st((float*)S.p[qs], ms, (float)s[q]);
-        slot_st((float*)S.p[19 + qs], ms, (float)a[q]);
+        st_pair32(S.p[qs], ms, (float)s[q], (float)a[q]);
     }
 }
 
@@ -306,21 +329,21 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
 template <class T>
 __device__ __forceinline__ void slot_from_dense(const Dev& d, int q, int i, const double* dense, long ps, long o) {
     if (sizeof(T) == 8) {
-        d.S.p[q][i] = dense[q * ps + o];
-        d.S.p[19 + q][i] = dense[(19 + q) * ps + o];
+        d.S.p[q][2 * i] = dense[q * ps + o];
+        d.S.p[19 + q][2 * i] = dense[(19 + q) * ps + o];
     } else {
         const double f = dense[q * ps + o], g = dense[(19 + q) * ps + o];
-        ((float*)d.S.p[q])[i] = (float)(f + g - rest_weight(q));
-        ((float*)d.S.p[19 + q])[i] = (float)(f - g);
+        ((float*)d.S.p[q])[2 * i] = (float)(f + g - rest_weight(q));
+        ((float*)d.S.p[19 + q])[2 * i] = (float)(f - g);
     }
 }
 template <class T>
 __device__ __forceinline__ void slot_to_dense(const Dev& d, int q, int i, double* dense, long ps, long o) {
     if (sizeof(T) == 8) {
-        dense[q * ps + o] = d.S.p[q][i];
-        dense[(19 + q) * ps + o] = d.S.p[19 + q][i];
+        dense[q * ps + o] = d.S.p[q][2 * i];
+        dense[(19 + q) * ps + o] = d.S.p[19 + q][2 * i];
     } else {
-        const double F = (double)((const float*)d.S.p[q])[i] + rest_weight(q), A = (double)((const float*)d.S.p[19 + q])[i];
+        const double F = (double)((const float*)d.S.p[q])[2 * i] + rest_weight(q), A = (double)((const float*)d.S.p[19 + q])[2 * i];
         dense[q * ps + o] = 0.5 * (F + A);
         dense[(19 + q) * ps + o] = 0.5 * (F - A);
     }
@@ -372,8 +395,8 @@ __device__ __forceinline__ void copy_both(const Dev& d, int q, int dst, int src)
     T* f = (T*)d.S.p[q];
     T* g = (T*)d.S.p[19 + q];
     const int a = slot_index(d, q, dst), b = slot_index(d, q, src);
-    f[a] = f[b];
-    g[a] = g[b];
+    f[2 * a] = f[2 * b];
+    g[2 * a] = g[2 * b];
 }
 // after an even step: outgoing boundary-layer populations into the opposite halo
 // (mpi_pdf_pull_async_pack / _sync_update, Mpi.F90:101-346)
@@ -1250,7 +1273,7 @@ __global__ void k_macro(Dev d, double force_z, double* u, double* v, double* w, 
     const int n = d.cpos[m];
     double ft[19];
     for (int q = 0; q < 19; ++q)
-        ft[q] = sizeof(T) == 8 ? d.S.p[q][m] + d.S.p[19 + q][m] : (double)((const float*)d.S.p[q])[m] + rest_weight(q);
+        ft[q] = sizeof(T) == 8 ? d.S.p[q][2 * m] + d.S.p[19 + q][2 * m] : (double)((const float*)d.S.p[q])[2 * m] + rest_weight(q);
     const double om = 1.0;
     rho[n] = (ft[0] + ft[1] + ft[2] + ft[3] + ft[4] + ft[5] + ft[6] + ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] +
               ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18]) * om;
@@ -1843,21 +1866,21 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     for (size_t u = 0; u < s1pos.size(); ++u) role[s1pos[u]] = -(int)u - 2;
     // device buffers
     free_geometry(c);
-    // slice q: nc compact slots, then the link slots of direction opp(q); each slice 256-byte aligned
+    // slice q: nc compact slots, then the link slots of direction opp(q), each slot the pair (colour 1, colour
+    // 2) of direction q; each slice 256-byte aligned
     const long esz = c->fp32 ? (long)sizeof(float) : (long)sizeof(double);
-    long soff[39];
+    long soff[20];
     soff[0] = 0;
-    for (int q = 0; q < 38; ++q) {
-        const int qq = q % 19;
-        const long len = (((long)nc + (qq ? nlink[lat::OPP[qq]] : 0) + 63) / 64) * 64;
-        if (len * esz >= (1L << 32))
+    for (int q = 0; q < 19; ++q) {
+        const long len = (((long)nc + (q ? nlink[lat::OPP[q]] : 0) + 63) / 64) * 64;
+        if (2 * len * esz >= (1L << 32))
             return fail(c, MFLBM_ERR_ARG, "population slice of %ld slots exceeds 32-bit byte offsets", len);
-        soff[q + 1] = soff[q] + len;
+        soff[q + 1] = soff[q] + 2 * len;
     }
     const long fs = (((long)nfl + 63) / 64) * 64;
     const size_t ni = (size_t)nc + nrm.size() + ns + s1pos.size() + nf + 1;
     const size_t nd = (size_t)ns + s1w.size() + fd.size() + 1;
-    HIPCHK(c, hipMalloc(&c->compact, esz * soff[38]));
+    HIPCHK(c, hipMalloc(&c->compact, esz * soff[19]));
     HIPCHK(c, hipMalloc(&c->force, sizeof(double) * fs * 4));
     HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
     HIPCHK(c, hipMalloc(&c->ibuf, sizeof(int) * ni));
@@ -1869,7 +1892,7 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMalloc(&c->wbase, sizeof(int) * wbase.size()));
     HIPCHK(c, hipMemcpy(c->wmask, links.wmask.data(), sizeof(unsigned long long) * links.wmask.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->wbase, wbase.data(), sizeof(int) * wbase.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->compact, 0, esz * soff[38]));
+    HIPCHK(c, hipMemset(c->compact, 0, esz * soff[19]));
     HIPCHK(c, hipMemset(c->force, 0, sizeof(double) * fs * 4));
     // normals at solid nodes are only ever written by the extrapolation; start them from zero so no value
     // of a previous geometry survives (cn_x, cn_y, cn_z, c_norm, curv, u, v, w, rho)
@@ -1900,7 +1923,11 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     if (w_in) HIPCHK(c, hipMemcpy(c->d.w_in, w_in, sizeof(double) * (L.nx + 2) * (L.ny + 2), hipMemcpyHostToDevice));
     Dev& d = c->d;
     // FP32 storage keeps float slices behind the same pointers (the kernels templated on the storage type cast)
-    for (int q = 0; q < 38; ++q) d.S.p[q] = (double*)((char*)c->compact + esz * soff[q]);
+    // p[q] colour 1 and p[19 + q] colour 2 of direction q: the two halves of the pairs (stride two elements)
+    for (int q = 0; q < 19; ++q) {
+        d.S.p[q] = (double*)((char*)c->compact + esz * soff[q]);
+        d.S.p[19 + q] = (double*)((char*)c->compact + esz * (soff[q] + 1));
+    }
     d.map = c->map;
     d.cpos = c->ibuf;
     d.nf = nfl;
