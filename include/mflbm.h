@@ -310,6 +310,12 @@ int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* bias, void* 
 /* accumulates the weight gradient (FP32, rows of 27 cin) and, when dbias is not NULL, the bias gradient */
 int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
                        int32_t w, int32_t cin, int32_t cout, void* stream);
+/* the same gradients summed in a fixed order, so that repeated calls give identical bits: per-chunk sums are
+ * stored to the caller's FP32 workspace (work_floats >= mflbm_nn_conv3d_dw_work(...), else MFLBM_ERR_ARG) and
+ * added to dw / dbias chunk by chunk */
+int64_t mflbm_nn_conv3d_dw_work(int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout);
+int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* dw, float* dbias, float* work, int64_t work_floats,
+                               int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout, void* stream);
 int mflbm_nn_relu_bwd(void* g, const void* y, int64_t n, void* stream);
 /* channels 0..2 of the BF16 channels-last input x (d, h, w, cin): fluid indicator (fluid: int8 1 fluid, 0 solid),
  * d / 8 and (d / 8)^2, d the Euclidean distance to the nearest solid node capped at 16 (wrapped along the axes

@@ -124,7 +124,9 @@ __global__ void __launch_bounds__(NT) k_conv_fwd(const __bf16* __restrict__ x, c
 
 // Weight gradient: dw[co][k] += sum_p dy[p][co] * x[p + off(k)][ci(k)] (rows of 27 cin, the layout of the FP32
 // master weights), the positions split into chunks of `chunk` (a multiple of 32) over grid z; one wave computes
-// 32 output channels x 32 k; FP32 atomics.
+// 32 output channels x 32 k.  The chunk sums go to dw with FP32 atomics, or (ORDERED) are stored to the
+// chunk's own slice dw + z * cout * K, every element written by exactly one lane, for a fixed-order sum after.
+template <bool ORDERED>
 __global__ void __launch_bounds__(NT) k_conv_dw(const __bf16* __restrict__ x, const __bf16* __restrict__ dy,
                                                 float* __restrict__ dw, Shape S, int cin, int cout, long chunk) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -199,22 +201,50 @@ __global__ void __launch_bounds__(NT) k_conv_dw(const __bf16* __restrict__ x, co
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int co = co0 + h * 16 + q * 4 + t;
-                if (co < cout) atomicAdd(&dw[(long)co * K + kcol], acc[h][g][t]);
+                if (co >= cout) continue;
+                if (ORDERED)
+                    dw[((long)blockIdx.z * cout + co) * K + kcol] = acc[h][g][t];
+                else
+                    atomicAdd(&dw[(long)co * K + kcol], acc[h][g][t]);
             }
         }
 }
 
-// bias gradient: db[co] += sum over positions of dy[p][co]
+// positions per chunk of the weight- and bias-gradient sums
+constexpr long DW_CHUNK = 4096, DB_CHUNK = 8192;
+
+// bias gradient: db[co] += sum over positions of dy[p][co]; ORDERED: the chunk's sum, its eight partial sums
+// added in a fixed order, is stored to db[chunk * cout + co]
+template <bool ORDERED>
 __global__ void __launch_bounds__(NT) k_bias_grad(const __bf16* __restrict__ dy, float* __restrict__ db, long M, int cout,
                                                   long chunk) {
+    __shared__ float part[NT];
     const int co = threadIdx.x % 32 + blockIdx.y * 32;
     const int lane_p = threadIdx.x / 32;
-    if (co >= cout) return;
     const long pbeg = (long)blockIdx.x * chunk;
     const long pend = pbeg + chunk < M ? pbeg + chunk : M;
     float s = 0.f;
-    for (long p = pbeg + lane_p; p < pend; p += NT / 32) s += bf2f(dy[p * cout + co]);
-    atomicAdd(&db[co], s);
+    if (co < cout)
+        for (long p = pbeg + lane_p; p < pend; p += NT / 32) s += bf2f(dy[p * cout + co]);
+    if (!ORDERED) {
+        if (co < cout) atomicAdd(&db[co], s);
+        return;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (lane_p == 0 && co < cout) {
+        float t = part[threadIdx.x];
+        for (int k = 1; k < NT / 32; ++k) t += part[threadIdx.x + 32 * k];
+        db[(long)blockIdx.x * cout + co] = t;
+    }
+}
+// out[e] += sum over z = 0 .. nz-1 of part[z * ne + e], in that order
+__global__ void __launch_bounds__(NT) k_sum_slices(const float* __restrict__ part, float* __restrict__ out, long ne, int nz) {
+    const long e = (long)blockIdx.x * NT + threadIdx.x;
+    if (e >= ne) return;
+    float s = 0.f;
+    for (int z = 0; z < nz; ++z) s += part[(long)z * ne + e];
+    out[e] += s;
 }
 
 __global__ void k_relu_bwd(__bf16* __restrict__ g, const __bf16* __restrict__ y, long n) {
@@ -359,16 +389,44 @@ extern "C" int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, floa
         return MFLBM_ERR_ARG;
     const Shape S{n, d, h, w};
     const long M = S.count();
-    const long chunk = 4096;
-    const dim3 grid((unsigned)((cout + 31) / 32), (unsigned)((27 * cin + 127) / 128), (unsigned)((M + chunk - 1) / chunk));
-    hipLaunchKernelGGL(k_conv_dw, grid, dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (const __bf16*)dy, dw, S,
-                       (int)cin, (int)cout, chunk);
-    if (dbias) {
-        const long bchunk = 8192;
-        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((M + bchunk - 1) / bchunk), (unsigned)((cout + 31) / 32)), dim3(NT),
-                           0, (hipStream_t)stream, (const __bf16*)dy, dbias, M, (int)cout, bchunk);
-    }
+    const dim3 grid((unsigned)((cout + 31) / 32), (unsigned)((27 * cin + 127) / 128), (unsigned)((M + DW_CHUNK - 1) / DW_CHUNK));
+    hipLaunchKernelGGL(k_conv_dw<false>, grid, dim3(NT), 0, (hipStream_t)stream, (const __bf16*)x, (const __bf16*)dy, dw,
+                       S, (int)cin, (int)cout, DW_CHUNK);
+    if (dbias)
+        hipLaunchKernelGGL(k_bias_grad<false>, dim3((unsigned)((M + DB_CHUNK - 1) / DB_CHUNK), (unsigned)((cout + 31) / 32)),
+                           dim3(NT), 0, (hipStream_t)stream, (const __bf16*)dy, dbias, M, (int)cout, DB_CHUNK);
     return launched((hipStream_t)stream);
+}
+
+extern "C" int64_t mflbm_nn_conv3d_dw_work(int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout) {
+    if (n < 1 || d < 1 || h < 1 || w < 1 || cin < 1 || cout < 1) return -1;
+    const long M = (long)n * d * h * w;
+    return ((M + DW_CHUNK - 1) / DW_CHUNK) * cout * 27L * cin + ((M + DB_CHUNK - 1) / DB_CHUNK) * cout;
+}
+
+extern "C" int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* dw, float* dbias, float* work,
+                                          int64_t work_floats, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
+                                          int32_t cout, void* stream) {
+    if (!x || !dy || !dw || !work || n < 1 || d < 1 || h < 1 || w < 8 || w % 8 || cin < 8 || cin % 8 || cout < 1)
+        return MFLBM_ERR_ARG;
+    if (work_floats < mflbm_nn_conv3d_dw_work(n, d, h, w, cin, cout)) return MFLBM_ERR_ARG;
+    const Shape S{n, d, h, w};
+    const long M = S.count();
+    const long K = 27L * cin, nzw = (M + DW_CHUNK - 1) / DW_CHUNK, nzb = (M + DB_CHUNK - 1) / DB_CHUNK;
+    float* bpart = work + nzw * cout * K;
+    const hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((unsigned)((cout + 31) / 32), (unsigned)((K + 127) / 128), (unsigned)nzw);
+    hipLaunchKernelGGL(k_conv_dw<true>, grid, dim3(NT), 0, st, (const __bf16*)x, (const __bf16*)dy, work, S, (int)cin,
+                       (int)cout, DW_CHUNK);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((cout * K + NT - 1) / NT)), dim3(NT), 0, st, (const float*)work, dw,
+                       cout * K, (int)nzw);
+    if (dbias) {
+        hipLaunchKernelGGL(k_bias_grad<true>, dim3((unsigned)nzb, (unsigned)((cout + 31) / 32)), dim3(NT), 0, st,
+                           (const __bf16*)dy, bpart, M, (int)cout, DB_CHUNK);
+        hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((cout + NT - 1) / NT)), dim3(NT), 0, st, (const float*)bpart, dbias,
+                           (long)cout, (int)nzb);
+    }
+    return launched(st);
 }
 
 extern "C" int mflbm_nn_weights_to_bf16(const float* w, void* wf, void* wb, int32_t cin, int32_t cout, void* stream) {

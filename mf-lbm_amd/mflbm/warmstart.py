@@ -138,6 +138,7 @@ class WarmStartNet:
         for name, ci, co, relu in layer_specs(self.levels):
             self.layers[name] = _Conv(torch, name, ci, co, relu, gen)
         self.step_count = 0
+        self._work = None   # FP32 workspace of the fixed-order gradient sums
         self.refresh()
 
     # -- plumbing -----------------------------------------------------------------------------------------
@@ -207,8 +208,13 @@ class WarmStartNet:
         n, d, h, w = shape
         if L.relu:
             self._ck(self.lib.mflbm_nn_relu_bwd(_p(g), _p(yout), g.numel(), self._s()), "mflbm_nn_relu_bwd")
-        self._ck(self.lib.mflbm_nn_conv3d_dw(_p(xin), _p(g), _p(L.gw), _p(L.gb), n, d, h, w, L.cin, L.cout, self._s()),
-                 "mflbm_nn_conv3d_dw")
+        # weight and bias gradients summed in a fixed order: a repeated training run gives the same weights
+        need = int(self.lib.mflbm_nn_conv3d_dw_work(n, d, h, w, L.cin, L.cout))
+        if self._work is None or self._work.numel() < need:
+            self._work = self.torch.empty(need, dtype=self.torch.float32, device="cuda")
+        self._ck(self.lib.mflbm_nn_conv3d_dw_ordered(_p(xin), _p(g), _p(L.gw), _p(L.gb), _p(self._work),
+                                                     self._work.numel(), n, d, h, w, L.cin, L.cout, self._s()),
+                 "mflbm_nn_conv3d_dw_ordered")
         if not need_dx:
             return None
         gx = self.torch.empty((n, d, h, w, L.cin), dtype=self.torch.bfloat16, device="cuda")

@@ -90,6 +90,65 @@ def test_conv3d_forward_and_weight_gradient_match_the_oracle(cin, cout, relu):
         assert np.abs(dx.float().cpu().numpy() - rgx).max() <= 2 ** -7 * np.abs(rgx).max()
 
 
+@pytest.mark.parametrize("cin,cout", [(8, 32), (32, 8), (16, 48)])
+def test_ordered_weight_gradient_is_repeatable_and_matches_the_oracle(cin, cout):
+    """The fixed-order gradient sums: the oracle's gradients to the same tolerance as the atomic form, identical
+    bits on a repeated call, and accumulation onto what dw / dbias already hold."""
+    import ctypes as C
+    torch = _t()
+    from mflbm import native
+    lib = native.load()
+    P = lambda t: C.c_void_p(t.data_ptr())
+    shape = (2, 6, 5, 16)
+    x, w, _ = _conv_case(cin, cout, shape, 3 * cin + cout)
+    g = O.to_bf16(np.random.default_rng(11).standard_normal(shape + (cout,)).astype(np.float32))
+    gx, gg = _bf(x), _bf(g)
+    need = lib.mflbm_nn_conv3d_dw_work(*shape, cin, cout)
+    M = int(np.prod(shape))
+    assert need == -(-M // 4096) * cout * 27 * cin + -(-M // 8192) * cout
+    work = torch.empty(need, device="cuda")
+    runs = []
+    for _ in range(2):
+        dw = torch.zeros((cout, 27 * cin), device="cuda")
+        db = torch.zeros(cout, device="cuda")
+        assert lib.mflbm_nn_conv3d_dw_ordered(P(gx), P(gg), P(dw), P(db), P(work), need, *shape, cin, cout, None) == 0
+        runs.append((dw.clone(), db.clone()))
+    # a workspace one float short is refused
+    assert lib.mflbm_nn_conv3d_dw_ordered(P(gx), P(gg), P(dw), P(db), P(work), need - 1, *shape, cin, cout, None) != 0
+    # accumulation: a second call onto the first call's result doubles it exactly (x + x is exact in FP32)
+    assert lib.mflbm_nn_conv3d_dw_ordered(P(gx), P(gg), P(dw), P(db), P(work), need, *shape, cin, cout, None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(runs[0][0], runs[1][0]) and torch.equal(runs[0][1], runs[1][1])
+    assert torch.equal(dw, 2 * runs[0][0]) and torch.equal(db, 2 * runs[0][1])
+    _, rgw, rgb = O.conv3d_grads(x, O.to_bf16(w), g)
+    got_w = runs[0][0].cpu().numpy().reshape(cout, 27, cin)
+    np.testing.assert_allclose(got_w, rgw, rtol=2e-3, atol=2e-3 * np.abs(rgw).max())
+    np.testing.assert_allclose(runs[0][1].cpu().numpy(), rgb, rtol=2e-3, atol=2e-3 * np.abs(rgb).max())
+
+
+def test_repeated_training_steps_give_identical_weights():
+    """Three loss/gradient/Adam steps from the same start, twice: the weights agree bit for bit (the weight and
+    bias gradients are summed in a fixed order)."""
+    torch = _t()
+    from mflbm.warmstart import USED, WarmStartNet
+    rng = np.random.default_rng(21)
+    x = np.zeros((2, 8, 8, 32, 8), np.float32)
+    x[..., 0] = rng.random((2, 8, 8, 32)) > 0.35
+    t = np.zeros((2, 8, 8, 32, 8), np.float32)
+    t[..., :USED] = O.to_bf16(rng.standard_normal((2, 8, 8, 32, USED)).astype(np.float32))
+    finals = []
+    for _ in range(2):
+        net = WarmStartNet((8, 16, 32), seed=3)
+        for _ in range(3):
+            net.loss_and_backward(_bf(x), _bf(t))
+            net.adam(1e-3)
+        torch.cuda.synchronize()
+        finals.append(net.get_params())
+    for name in finals[0]:
+        for a, b in zip(finals[0][name], finals[1][name]):
+            assert np.array_equal(a, b), name
+
+
 def test_network_loss_and_gradients_match_the_oracle():
     torch = _t()
     from mflbm.warmstart import USED, WarmStartNet
