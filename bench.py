@@ -352,6 +352,32 @@ def run_warmstart(args) -> dict:
             "results": results, "total_seconds": round(time.perf_counter() - t0, 1)}
 
 
+def run_copy(args) -> dict:
+    """Calibration workload for the hardware byte counters: a plain device-to-device copy of args.copy_mib MiB.
+    Under ``rocprofv3 --pmc FETCH_SIZE`` (or ``WRITE_SIZE``) the copy kernel reads (writes) exactly that many
+    bytes per dispatch; the ratio of counter to size is the factor that turns the solver kernels' counters into
+    HBM bytes.  ``value`` is the copy's read + write bandwidth, the practical ceiling of a streaming kernel."""
+    import torch
+    n = args.copy_mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    reps = 5
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    gbs = 2.0 * args.copy_mib * (1 << 20) / (ms * 1e-3) / 1e9
+    return {"metric": "device copy bandwidth (read + write)", "value": round(gbs, 1), "unit": "GB/s",
+            "n_gpus": 1, "ms_per_copy": round(ms, 4), "bytes_each_way": args.copy_mib * (1 << 20),
+            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "data": "synthetic"}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,8 +386,9 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report", "warmstart"),
+    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report", "warmstart", "copy"),
                     default="twophase")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: MiB copied per dispatch")
     ap.add_argument("--batch-samples", type=int, default=96)
     ap.add_argument("--report-steps", default="5000,10000,20000")
     ap.add_argument("--ws-samples", type=int, default=256, help="warm start: training fields (the full set is 2000)")
@@ -377,7 +404,7 @@ def main() -> None:
               "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     run = {"singlephase": run_singlephase, "batch": run_batch, "fp32report": run_fp32_report,
-           "warmstart": run_warmstart}.get(args.workload, run_twophase)
+           "warmstart": run_warmstart, "copy": run_copy}.get(args.workload, run_twophase)
     out = run(args)
     print(json.dumps(out))
 

@@ -135,7 +135,10 @@ int mflbm_color_gradient(mflbm_ctx* ctx);
  * zeroing phi inside solids as the reference does. Stays on the device. */
 int mflbm_compute_macro(mflbm_ctx* ctx);
 /* Monitor reductions on the device (Monitor.F90); calls compute_macro first.  layers: 7*nz doubles.
- * update_phi_old != 0 copies phi into phi_old afterwards (monitor_multiphase_steady_phasefield). */
+ * update_phi_old != 0 copies phi into phi_old afterwards (monitor_multiphase_steady_phasefield).  phi_old
+ * is zero after every state upload until such a call (a driver starting a new run calls it once to set
+ * phi_old = phi, Init_multiphase.F90:341-351; a continued run starts from zero, as the reference's unset
+ * phi_old does). */
 int mflbm_monitor(mflbm_ctx* ctx, double* layers, mflbm_monitor_sums* sums, int32_t update_phi_old);
 
 int mflbm_download_macro(mflbm_ctx* ctx, double* u, double* v, double* w, double* rho, double* phi);
@@ -310,9 +313,10 @@ int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* bias, void* 
 /* accumulates the weight gradient (FP32, rows of 27 cin) and, when dbias is not NULL, the bias gradient */
 int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
                        int32_t w, int32_t cin, int32_t cout, void* stream);
-/* the same gradients summed in a fixed order, so that repeated calls give identical bits: per-chunk sums are
- * stored to the caller's FP32 workspace (work_floats >= mflbm_nn_conv3d_dw_work(...), else MFLBM_ERR_ARG) and
- * added to dw / dbias chunk by chunk */
+/* the same gradients summed in a fixed order, so that repeated calls give identical bits: the positions are cut
+ * into chunks (4096 for the weights, 8192 for the bias), each chunk's sum is stored to its own slice of the
+ * caller's FP32 workspace (work_floats >= mflbm_nn_conv3d_dw_work(...), else MFLBM_ERR_ARG), then per element the
+ * chunk sums are added in chunk order and that total is added to dw / dbias once */
 int64_t mflbm_nn_conv3d_dw_work(int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout);
 int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* dw, float* dbias, float* work, int64_t work_floats,
                                int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout, void* stream);

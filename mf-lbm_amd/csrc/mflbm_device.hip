@@ -64,6 +64,20 @@ struct Dev {
     float* cnz32;
     float* cnorm32;
     double* curv;     // dense curvature, filled only for diagnostics
+    // Quiet-node flags of the list chain.  The normal field is exactly +0 (all bits clear) away from the
+    // interface, and there the curvature force of a fluid node is a constant: the lattice update reads the
+    // stencil only where a flag says a bit may be set.
+    //   act[m]:    fluid node m has a normal with a bit set at itself or an ISO4 neighbour (set by the chain,
+    //              cleared by the lattice update that consumes it)
+    //   xmark[t]:  an entry of the normal-extrapolation list has a neighbour whose normal has a bit set
+    //   xdirty[t]: the entry's last extrapolated normal had a bit set
+    //   xskip:     every neighbour the extrapolation reads is a node of the normal list (checked at upload),
+    //              so an entry without a mark extrapolates to +0
+    unsigned char* act;
+    unsigned char* xmark;
+    unsigned char* xdirty;
+    const int* role;  // padded offset -> -(normal-extrapolation list index) - 2 for its entries
+    int xskip;
     const int8_t* walls;
     double* w_in;     // (nx+2)(ny+2), reference layout
     double* cf;       // convective-outlet history f: 19 planes (nx+2)(ny+2)
@@ -135,40 +149,10 @@ __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i +
 // lattice update: collision + recolouring + AA-pattern streaming of both colours, one thread per fluid node
 // ------------------------------------------------------------------------------------------------------
 
-// Tuning knobs of the lattice-update kernel (build-time): the minimum number of waves per SIMD the register
-// allocation must allow, and whether population traffic uses non-temporal (streaming) loads and stores.
-#ifndef MFLBM_COLLIDE_WAVES
-#define MFLBM_COLLIDE_WAVES 2
-#endif
-#ifndef MFLBM_COLLIDE_WAVES_EVEN
-#define MFLBM_COLLIDE_WAVES_EVEN MFLBM_COLLIDE_WAVES
-#endif
-
-#ifndef MFLBM_COLLIDE_NT
-#define MFLBM_COLLIDE_NT 0
-#endif
-
 // Population slots.  The 38 slice base pointers travel in the kernel arguments, so each is a scalar value
 // the compiler cannot relate to the others: every access becomes one global load/store with a scalar base
 // and a 32-bit vector byte offset, instead of a 64-bit vector address per slot kept live across the collision.
 // upload_geometry guarantees that every slice is below 2^32 bytes.
-__device__ __forceinline__ double ld_slot(const double* slice, unsigned m) {
-    const double* p = (const double*)((const char*)slice + m * 8u);
-#if MFLBM_COLLIDE_NT
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
-__device__ __forceinline__ void st_slot(double* slice, unsigned m, double v) {
-    double* p = (double*)((char*)slice + m * 8u);
-#if MFLBM_COLLIDE_NT
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
 // The two colours of a slot are adjacent (f_q, g_q or s_q, a_q): one 16-byte (FP64) or 8-byte (FP32) access
 // moves both, which halves the scattered accesses of the odd step.
 __device__ __forceinline__ double2 ld_pair(const double* slice, unsigned m) {
@@ -184,11 +168,10 @@ __device__ __forceinline__ void st_pair32(double* slice, unsigned m, float s, fl
     *(float2*)((char*)slice + m * 8u) = float2{s, a};
 }
 
-#ifndef MFLBM_FUSED_CURVATURE
-#define MFLBM_FUSED_CURVATURE 1
-#endif
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n);
+// the curvature where every normal of the stencil is +0
+__device__ __forceinline__ double quiet_curvature();
 
 // ODD = false: kernel_even_color (Kernel_multiphase.F90:371-725): read swapped own slots, write own slots.
 // ODD = true : kernel_odd_color  (Kernel_multiphase.F90:6-362): pull f_q(x - e_q), push to f_opp(q)(x + e_q).
@@ -197,18 +180,26 @@ __device__ __forceinline__ double curvature_at(const Dev& d, int n);
 // left (STENCIL), or read from the compact force arrays the fused chain (k_chain_force) wrote.
 template <bool ODD, bool STENCIL>
 __global__ void __launch_bounds__(BLOCK)
-__attribute__((amdgpu_waves_per_eu(ODD ? MFLBM_COLLIDE_WAVES : MFLBM_COLLIDE_WAVES_EVEN, 8)))
+__attribute__((amdgpu_waves_per_eu(2, 8)))
 k_collide(Dev d, CollideParams P, int per) {
     const SlotTable& S = d.S;
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     double cnx, cny, cnz, tf;
     if (STENCIL) {
-        const int nn = d.cpos[m];
-        cnx = d.cnx[nn];
-        cny = d.cny[nn];
-        cnz = d.cnz[nn];
-        tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
+        if (d.act[m]) {
+            const int nn = d.cpos[m];
+            cnx = d.cnx[nn];
+            cny = d.cny[nn];
+            cnz = d.cnz[nn];
+            tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
+            d.act[m] = 0;
+        } else {   // every normal of the stencil is +0: the same expressions on those values
+            cnx = 0.0;
+            cny = 0.0;
+            cnz = 0.0;
+            tf = 0.5 * P.gamma * quiet_curvature() * 0.0;
+        }
         // keep the stencil phase and the population phase apart so their registers are not live together
         __builtin_amdgcn_sched_barrier(0);
     } else {
@@ -272,10 +263,18 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     const int nn = d.cpos[m];
     C cnx, cny, cnz, tf;
     if (STENCIL) {
-        cnx = (C)d.cnx32[nn];
-        cny = (C)d.cny32[nn];
-        cnz = (C)d.cnz32[nn];
-        tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
+        if (d.act[m]) {
+            cnx = (C)d.cnx32[nn];
+            cny = (C)d.cny32[nn];
+            cnz = (C)d.cnz32[nn];
+            tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
+            d.act[m] = 0;
+        } else {
+            cnx = (C)0.0f;
+            cny = (C)0.0f;
+            cnz = (C)0.0f;
+            tf = (C)(0.5 * gamma * quiet_curvature() * (double)0.0f);
+        }
         __builtin_amdgcn_sched_barrier(0);
     } else {   // the force arrays hold the FP32 normal widened to FP64
         cnx = (C)d.force[m];
@@ -964,6 +963,22 @@ __device__ __forceinline__ Normals<double> normals<double>(const Dev& d) { retur
 template <>
 __device__ __forceinline__ Normals<float> normals<float>(const Dev& d) { return {d.cnx32, d.cny32, d.cnz32, d.cnorm32}; }
 
+// any bit of the stored value set (-0.0 and NaN count as set)
+__device__ __forceinline__ bool bits_set(double v) { return __double_as_longlong(v) != 0; }
+__device__ __forceinline__ bool bits_set(float v) { return __float_as_uint(v) != 0u; }
+// flag the interior fluid nodes among node n (q0 = 0) and its 18 neighbours: their curvature stencil holds n
+__device__ __forceinline__ void mark_fluid(const Dev& d, int n, int q0) {
+#pragma unroll
+    for (int q = q0; q < 19; ++q) {
+        const int m = d.map[n + d.L.off(q)];
+        if ((unsigned)m < (unsigned)d.nf) d.act[m] = 1;
+    }
+}
+struct ZeroAt {
+    __device__ __forceinline__ double operator()(int, int, int) const { return 0.0; }
+};
+__device__ __forceinline__ double quiet_curvature() { return curvature_f(ZeroAt{}, ZeroAt{}, ZeroAt{}); }
+
 // phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30)
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
@@ -1022,10 +1037,21 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     const int b = wb[t];
     if (b >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * b], fl.nw[3 * b + 1], fl.nw[3 * b + 2], fl.cos_t[b], fl.sin_t[b]);
     const Normals<NT> N = normals<NT>(d);
-    N.x[n] = (NT)cx;
-    N.y[n] = (NT)cy;
-    N.z[n] = (NT)cz;
-    N.n[n] = (NT)cn;
+    const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
+    N.x[n] = sx;
+    N.y[n] = sy;
+    N.z[n] = sz;
+    N.n[n] = sn;
+    if (bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn)) {
+        // fluid nodes whose curvature stencil holds this node, and extrapolation entries that read it
+        mark_fluid(d, n, 0);
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            const int o = n + L.off(q);
+            const int r = d.role[o];
+            if (r <= -2) d.xmark[-r - 2] = 1;
+        }
+    }
 }
 // normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110)
 template <class NT>
@@ -1034,6 +1060,19 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     if (t >= s.n) return;
     const Normals<NT> N = normals<NT>(d);
     const int n = s.pos[t];
+    if (d.xskip) {
+        if (!d.xmark[t]) {
+            // every neighbour's normal is +0, so the weighted mean is +0: store it once, then nothing to do
+            if (d.xdirty[t]) {
+                N.x[n] = (NT)0.0;
+                N.y[n] = (NT)0.0;
+                N.z[n] = (NT)0.0;
+                d.xdirty[t] = 0;
+            }
+            return;
+        }
+        d.xmark[t] = 0;
+    }
     const unsigned mk = s.mask[t];
     double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
@@ -1044,9 +1083,13 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
             ay = ay + (double)N.y[o] * lat::W[q];
             az = az + (double)N.z[o] * lat::W[q];
         }
-    N.x[n] = (NT)(ax / s.w[t]);
-    N.y[n] = (NT)(ay / s.w[t]);
-    N.z[n] = (NT)(az / s.w[t]);
+    const NT sx = (NT)(ax / s.w[t]), sy = (NT)(ay / s.w[t]), sz = (NT)(az / s.w[t]);
+    N.x[n] = sx;
+    N.y[n] = sy;
+    N.z[n] = sz;
+    const bool set = bits_set(sx) | bits_set(sy) | bits_set(sz);
+    if (d.xskip) d.xdirty[t] = set ? 1 : 0;
+    if (set) mark_fluid(d, n, 1);
 }
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
@@ -1331,7 +1374,7 @@ __global__ void __launch_bounds__(BLOCK) k_monitor_layers(Dev d, const double* u
         if (wi == 0) {
             if (ph < -0.99) { s[9] += rho[n]; s[10] += 1.0; }
             if (ph > 0.99) { s[11] += rho[n]; }
-            if (phi_old) dphi = fmax(dphi, fabs(ph - phi_old[n]) * om);
+            dphi = fmax(dphi, fabs(ph - phi_old[n]) * om);
         }
     }
     __shared__ double sh[BLOCK];
@@ -1451,6 +1494,7 @@ struct mflbm_ctx {
     double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
     float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
     int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
+    unsigned char* qbuf = nullptr;   // quiet-node flags: act (per fluid node), xmark, xdirty (per extrapolation entry)
     bool chain_fused = false;   // one fused pass for normals, extrapolation and curvature (MFLBM_CHAIN=fused)
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
@@ -1474,7 +1518,7 @@ struct mflbm_ctx {
     SolidList sl{}, sl1{};
     FluidList fl{};
     NodeList nrm{};
-    bool have_geom = false, have_state = false, phi_old_valid = false;
+    bool have_geom = false, have_state = false;
     bool timing = false;
     std::vector<hipEvent_t> ev;   // pool of 4 events per timed step: start, after collide, before chain, end
     size_t ev_used = 0;
@@ -1514,7 +1558,6 @@ int launch_gradient_t(mflbm_ctx* c) {
     }
     const int rc = launch_normals_dense<NT>(c);
     if (rc) return rc;
-    if (!MFLBM_FUSED_CURVATURE) LAUNCH(c, k_curvature<NT>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     return MFLBM_OK;
 }
 int launch_gradient(mflbm_ctx* c) { return c->fp32 ? launch_gradient_t<float>(c) : launch_gradient_t<double>(c); }
@@ -1639,7 +1682,7 @@ int scatter_state(mflbm_ctx* c) {
 }
 
 void free_geometry(mflbm_ctx* c) {
-    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role};
+    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role, c->qbuf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
@@ -1652,6 +1695,7 @@ void free_geometry(mflbm_ctx* c) {
     c->wmask = nullptr;
     c->wbase = nullptr;
     c->role = nullptr;
+    c->qbuf = nullptr;
     c->have_geom = false;
     c->have_state = false;
 }
@@ -1864,6 +1908,15 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     for (int t = 0; t < nf; ++t)
         if (wp[fpos[t]] == 0) role[fpos[t]] = t;
     for (size_t u = 0; u < s1pos.size(); ++u) role[s1pos[u]] = -(int)u - 2;
+    // the extrapolation may skip entries without a mark only if every neighbour it reads is a node of the
+    // normal list (non-solid), so that the normal kernel's marks see every value it reads
+    bool xskip = true;
+    for (size_t u = 0; u < s1pos.size() && xskip; ++u)
+        for (int q = 1; q < 19; ++q)
+            if ((s1mask[u] & (1u << q)) && wp[s1pos[u] + L.off(q)] != 0) {
+                xskip = false;
+                break;
+            }
     // device buffers
     free_geometry(c);
     // slice q: nc compact slots, then the link slots of direction opp(q), each slot the pair (colour 1, colour
@@ -1916,6 +1969,10 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMemcpy(c->map, cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->role, sizeof(int) * L.total));
     HIPCHK(c, hipMemcpy(c->role, role.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
+    // quiet-node flags start set: the first chain and lattice update evaluate everything
+    const size_t nact = (size_t)fs, nx1 = s1pos.size() + 1;
+    HIPCHK(c, hipMalloc(&c->qbuf, nact + 2 * nx1));
+    HIPCHK(c, hipMemset(c->qbuf, 1, nact + 2 * nx1));
     HIPCHK(c, hipMemcpy(c->ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->mbuf, hm.data(), sizeof(unsigned) * hm.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dbuf, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
@@ -1936,6 +1993,11 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     d.fs = fs;
     d.wmask = c->wmask;
     d.wbase = c->wbase;
+    d.act = c->qbuf;
+    d.xmark = c->qbuf + nact;
+    d.xdirty = c->qbuf + nact + nx1;
+    d.role = c->role;
+    d.xskip = xskip ? 1 : 0;
     const int* ip = c->ibuf + nc;
     c->nrm = NodeList{(int)nrm.size(), ip};
     ip += nrm.size();
@@ -1976,8 +2038,10 @@ extern "C" int mflbm_upload_state(mflbm_ctx* c, const double* const* f, const do
         if (rs) return rs;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // phi_old reads as zero until a monitor call updates it (a restarted run's first phase-field monitor then
+    // sees the whole field as changed, as the reference's unset phi_old does)
+    HIPCHK(c, hipMemset(c->phi_old, 0, sizeof(double) * c->ps));
     c->have_state = true;
-    c->phi_old_valid = false;
     return MFLBM_OK;
 }
 
@@ -1999,8 +2063,10 @@ extern "C" int mflbm_init_state_device(mflbm_ctx* c, int32_t option, double inte
         if (rs) return rs;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // phi_old reads as zero until a monitor call updates it (a restarted run's first phase-field monitor then
+    // sees the whole field as changed, as the reference's unset phi_old does)
+    HIPCHK(c, hipMemset(c->phi_old, 0, sizeof(double) * c->ps));
     c->have_state = true;
-    c->phi_old_valid = false;
     return MFLBM_OK;
 }
 
@@ -2060,7 +2126,7 @@ extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     // the force input of the macroscopic velocity comes from the current normal field
     // (the fused chain keeps the force arrays current; the list chain leaves only the dense normals)
-    if (MFLBM_FUSED_CURVATURE && !c->chain_fused) {
+    if (!c->chain_fused) {
         if (c->fp32) LAUNCH(c, k_curvature<float>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
         else LAUNCH(c, k_curvature<double>, NXCD * xcd_per(c->d.nf), c->d, c->p.gamma, xcd_per(c->d.nf));
     }
@@ -2078,7 +2144,7 @@ extern "C" int mflbm_monitor(mflbm_ctx* c, double* layers, mflbm_monitor_sums* s
     const Layout& L = c->L;
     double* lay = c->red;
     double* part = c->red + 7 * (size_t)L.nz;
-    LAUNCH(c, k_monitor_layers, (unsigned)L.nz, c->d, c->u, c->v, c->w, c->rho, c->phi_old_valid ? c->phi_old : nullptr,
+    LAUNCH(c, k_monitor_layers, (unsigned)L.nz, c->d, c->u, c->v, c->w, c->rho, c->phi_old,
            lay, part);
     std::vector<double> h((size_t)L.nz * (7 + NPART + 3 - 7));
     HIPCHK(c, hipMemcpyAsync(h.data(), c->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
@@ -2116,7 +2182,6 @@ extern "C" int mflbm_monitor(mflbm_ctx* c, double* layers, mflbm_monitor_sums* s
     }
     if (update_phi_old) {
         LAUNCH(c, k_copy_phi_old, nblocks(L.total), c->d, c->phi_old);
-        c->phi_old_valid = true;
     }
     return MFLBM_OK;
 }

@@ -320,6 +320,61 @@ def write_checkpoint(path: str, ck: Checkpoint) -> None:
             fh.write(np.ascontiguousarray(ck.phi_convec, dtype="<f8").tobytes())
 
 
+# -------------------------------------------------------------------------------------------------
+# text output in Fortran edit descriptors (the monitor files, Monitor.F90:153-248; info.txt, time.dat)
+# -------------------------------------------------------------------------------------------------
+
+def fortran_e(x: float, w: int, d: int) -> str:
+    """Ew.d edit of x: ``0.<d digits>E<sign><2 digits>`` right-justified in w characters (three-digit
+    exponents drop the ``E``, as the Fortran standard prescribes; NaN and infinities as the runtime prints
+    them)."""
+    if math.isnan(x):
+        return f"{'NaN':>{w}s}"
+    if math.isinf(x):
+        return f"{('-' if x < 0 else '') + 'Infinity':>{w}s}"
+    if x == 0.0:
+        body = "0." + "0" * d + "E+00"
+        return f"{('-' if math.copysign(1.0, x) < 0 else '') + body:>{w}s}"
+    m, e = f"{abs(x):.{d - 1}e}".split("e")
+    digits = m.replace(".", "")
+    ex = int(e) + 1
+    exp = f"E{'+' if ex >= 0 else '-'}{abs(ex):02d}" if abs(ex) <= 99 else f"{'+' if ex >= 0 else '-'}{abs(ex):03d}"
+    s = ("-" if x < 0 else "") + "0." + digits + exp
+    return f"{s:>{w}s}" if len(s) <= w else "*" * w
+
+
+def fortran_f(x: float, w: int, d: int) -> str:
+    """Fw.d edit of x right-justified in w characters (asterisks when it does not fit)."""
+    if math.isnan(x):
+        return f"{'NaN':>{w}s}"
+    s = f"{x:.{d}f}"
+    if len(s) > w and s.startswith("0."):
+        s = s[1:]          # the optional leading zero is dropped first
+    elif len(s) > w and s.startswith("-0."):
+        s = "-" + s[2:]
+    return f"{s:>{w}s}" if len(s) <= w else "*" * w
+
+
+_EDIT = {"e14.7": (14, 7), "E14.6": (14, 6)}
+
+
+def fortran_fields(edit: str, *values: float) -> str:
+    """``n(1x, Ew.d)`` of the values."""
+    w, d = _EDIT[edit]
+    return "".join(" " + fortran_e(float(v), w, d) for v in values)
+
+
+def fortran_line(ntime: int, edit: str, *values: float) -> str:
+    """``(I10, n(1x, Ew.d))``: the step and the values, one monitor-file line."""
+    return f"{ntime:10d}" + fortran_fields(edit, *values)
+
+
+def profile_name(ntime: int) -> str:
+    """File name of a monitor profile: ``write(flnm,'(I10.8,".dat")')`` (Monitor.F90:229-231), i.e. the step
+    with at least 8 digits right-justified in 10 characters, then ``.dat``."""
+    return f"{ntime:10d}.dat" if ntime >= 10 ** 8 else "  " + f"{ntime:08d}.dat"
+
+
 # =================================================================================================
 # single-phase program (singlephase_3D/0.src): control file, derived constants, checkpoint
 # =================================================================================================

@@ -32,6 +32,8 @@ class MonitorResult:
     mass2_sum: float
     pressure_drop: Optional[float]
     capillary_pressure: float
+    pre_w: float                # mean density of the wetting (phi < -0.99) and non-wetting (phi > 0.99) nodes
+    pre_nw: float
     dphi_max: float
     outlet_phase1: int
     failed: bool
@@ -78,8 +80,8 @@ def post_monitor(case: Case, layers: np.ndarray, s) -> MonitorResult:
     pre_nw = s.pre_nw / (s.n_nw + EPS)
     failed = math.isnan(sat_full) or math.isnan(ca) or umax > 0.5
     return MonitorResult(saturation, sat_full, ca, umax, (0.5 * s.usq1, 0.5 * s.usq2), fl_whole, fl_avg, vol1_sum,
-                         vol2_sum, mass1_sum, mass2_sum, dp, (pre_nw - pre_w) / 3.0, s.dphi_max, int(s.outlet_phase1),
-                         failed, layers)
+                         vol2_sum, mass1_sum, mass2_sum, dp, (pre_nw - pre_w) / 3.0, pre_w, pre_nw, s.dphi_max,
+                         int(s.outlet_phase1), failed, layers)
 
 
 class Simulation:
@@ -160,7 +162,8 @@ class Simulation:
         self._call("run", ntime0, nsteps)
 
     def synchronize(self) -> None:
-        self._call("synchronize")
+        if "synchronize" in self.fn:   # a host-side binding of the same ABI has nothing to wait for
+            self._call("synchronize")
 
     # -- diagnostics -----------------------------------------------------------------------------------
     def macro(self) -> Dict[str, np.ndarray]:

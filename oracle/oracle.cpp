@@ -54,7 +54,6 @@ struct oracle_ctx {
     std::vector<double> s_w;
     std::vector<int> f_ijk;
     std::vector<double> f_nw, f_theta;
-    bool phi_old_valid = false;
 
     int8_t& W(int i, int j, int k) {
         return walls[(size_t)(i + 1) + (size_t)(j + 1) * (nx + 4) + (size_t)(k + 1) * (nx + 4) * (ny + 4)];
@@ -777,7 +776,7 @@ extern "C" int oracle_upload_state(oracle_ctx* c, const double* const* f, const 
     if (f_convec) std::memcpy(c->f_convec.data(), f_convec, c->f_convec.size() * sizeof(double));
     if (g_convec) std::memcpy(c->g_convec.data(), g_convec, c->g_convec.size() * sizeof(double));
     if (phi_convec) std::memcpy(c->phi_convec.data(), phi_convec, c->phi_convec.size() * sizeof(double));
-    c->phi_old_valid = false;
+    std::fill(c->phi_old.v.begin(), c->phi_old.v.end(), 0.0);   // phi_old reads as zero until the first update
     return 0;
 }
 
@@ -855,7 +854,7 @@ extern "C" int oracle_monitor(oracle_ctx* c, double* layers, oracle_monitor_sums
                 if (s.umax2 < temp) s.umax2 = temp;
                 if (ph > 0.999) s.usq1 = s.usq1 + temp;
                 else if (ph < -0.999) s.usq2 = s.usq2 + temp;
-                if (c->phi_old_valid) {
+                {
                     const double d = std::fabs(ph - c->phi_old(i, j, k)) * a;
                     if (s.dphi_max < d) s.dphi_max = d;
                 }
@@ -896,7 +895,6 @@ extern "C" int oracle_monitor(oracle_ctx* c, double* layers, oracle_monitor_sums
     if (sums) *sums = s;
     if (update_phi_old) {
         c->phi_old.v = c->phi.v;
-        c->phi_old_valid = true;
     }
     return 0;
 }

@@ -90,22 +90,24 @@ def test_conv3d_forward_and_weight_gradient_match_the_oracle(cin, cout, relu):
         assert np.abs(dx.float().cpu().numpy() - rgx).max() <= 2 ** -7 * np.abs(rgx).max()
 
 
-@pytest.mark.parametrize("cin,cout", [(8, 32), (32, 8), (16, 48)])
-def test_ordered_weight_gradient_is_repeatable_and_matches_the_oracle(cin, cout):
+@pytest.mark.parametrize("cin,cout,shape", [(8, 32, (2, 6, 5, 16)), (32, 8, (2, 6, 5, 16)), (16, 48, (2, 6, 5, 16)),
+                                            (16, 48, (2, 12, 16, 48))])
+def test_ordered_weight_gradient_is_repeatable_and_matches_the_oracle(cin, cout, shape):
     """The fixed-order gradient sums: the oracle's gradients to the same tolerance as the atomic form, identical
-    bits on a repeated call, and accumulation onto what dw / dbias already hold."""
+    bits on a repeated call, and accumulation onto what dw / dbias already hold.  The last case has 18432
+    positions: 5 weight chunks and 3 bias chunks, so the chunk sums are really summed."""
     import ctypes as C
     torch = _t()
     from mflbm import native
     lib = native.load()
     P = lambda t: C.c_void_p(t.data_ptr())
-    shape = (2, 6, 5, 16)
     x, w, _ = _conv_case(cin, cout, shape, 3 * cin + cout)
     g = O.to_bf16(np.random.default_rng(11).standard_normal(shape + (cout,)).astype(np.float32))
     gx, gg = _bf(x), _bf(g)
     need = lib.mflbm_nn_conv3d_dw_work(*shape, cin, cout)
     M = int(np.prod(shape))
     assert need == -(-M // 4096) * cout * 27 * cin + -(-M // 8192) * cout
+    print(f"[nn dw ordered] shape {shape}: {-(-M // 4096)} weight chunks, {-(-M // 8192)} bias chunks", flush=True)
     work = torch.empty(need, device="cuda")
     runs = []
     for _ in range(2):
@@ -124,18 +126,27 @@ def test_ordered_weight_gradient_is_repeatable_and_matches_the_oracle(cin, cout)
     got_w = runs[0][0].cpu().numpy().reshape(cout, 27, cin)
     np.testing.assert_allclose(got_w, rgw, rtol=2e-3, atol=2e-3 * np.abs(rgw).max())
     np.testing.assert_allclose(runs[0][1].cpu().numpy(), rgb, rtol=2e-3, atol=2e-3 * np.abs(rgb).max())
+    # the atomic form agrees with the ordered one to FP32 summation-order differences
+    dwa = torch.zeros((cout, 27 * cin), device="cuda")
+    dba = torch.zeros(cout, device="cuda")
+    assert lib.mflbm_nn_conv3d_dw(P(gx), P(gg), P(dwa), P(dba), *shape, cin, cout, None) == 0
+    torch.cuda.synchronize()
+    scale = float(runs[0][0].abs().max())
+    assert float((dwa - runs[0][0]).abs().max()) <= 1e-5 * scale * np.sqrt(M)
+    assert float((dba - runs[0][1]).abs().max()) <= 1e-5 * float(runs[0][1].abs().max()) * np.sqrt(M)
 
 
 def test_repeated_training_steps_give_identical_weights():
     """Three loss/gradient/Adam steps from the same start, twice: the weights agree bit for bit (the weight and
-    bias gradients are summed in a fixed order)."""
+    bias gradients are summed in a fixed order).  16384 positions at the first level: 4 weight chunks and 2
+    bias chunks, so the fixed-order sums of several chunks are what is repeated."""
     torch = _t()
     from mflbm.warmstart import USED, WarmStartNet
     rng = np.random.default_rng(21)
-    x = np.zeros((2, 8, 8, 32, 8), np.float32)
-    x[..., 0] = rng.random((2, 8, 8, 32)) > 0.35
-    t = np.zeros((2, 8, 8, 32, 8), np.float32)
-    t[..., :USED] = O.to_bf16(rng.standard_normal((2, 8, 8, 32, USED)).astype(np.float32))
+    x = np.zeros((2, 16, 16, 32, 8), np.float32)
+    x[..., 0] = rng.random((2, 16, 16, 32)) > 0.35
+    t = np.zeros((2, 16, 16, 32, 8), np.float32)
+    t[..., :USED] = O.to_bf16(rng.standard_normal((2, 16, 16, 32, USED)).astype(np.float32))
     finals = []
     for _ in range(2):
         net = WarmStartNet((8, 16, 32), seed=3)
