@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_plan.h"
 #include "device_common.h"
 #include "lattice.h"
 #include "mflbm.h"
@@ -419,45 +420,29 @@ extern "C" int mflbm_spb_commit(mflbm_spb_ctx* c) {
     const int n = (int)c->host.size();
     if (n == 0) return fail(c, MFLBM_ERR_ARG, "no samples");
     const long esz = c->fp32 ? (long)sizeof(float) : (long)sizeof(double);
-    std::vector<CompactSet> cs(n);
-    std::vector<LinkSet> ls(n);
-    c->L.resize(n);
-    c->nf.resize(n);
-    c->nc.resize(n);
-    c->wave_first_h.resize(n);
-    c->nwave_h.resize(n);
-    c->elem_off.resize(n);
-    c->soff.assign(20L * n, 0);
-    long elems = 0, maxdense = 0;
-    int waves = 0;
-    c->nfluid = 0;
+    std::vector<BatchSampleIn> in(n);
     for (int s = 0; s < n; ++s) {
         const SampleHost& h = c->host[s];
-        const Layout L = make_layout(h.nx, h.ny, h.nz);
-        if (L.total > (1L << 31) - 1) return fail(c, MFLBM_ERR_ARG, "sample %d too large for 32-bit offsets", s);
-        c->L[s] = L;
-        std::vector<int8_t> wp(L.total, 1);
-        pack_host(L, 2, h.walls_h2.data(), wp.data());
-        const TileOrder T{c->tile, L};
-        build_compact(L, T, wp, false, true, h.jper == 1, {}, cs[s]);
-        build_links(L, cs[s], ls[s]);
-        c->nf[s] = cs[s].nfl;
-        c->nc[s] = (int)cs[s].cpos.size();
-        c->nfluid += cs[s].nfl;
-        c->wave_first_h[s] = waves;
-        c->nwave_h[s] = (cs[s].nfl + WAVE - 1) / WAVE;
-        waves += c->nwave_h[s];
-        long* so = &c->soff[20L * s];
-        for (int q = 0; q < 19; ++q) {
-            const long len = (((long)c->nc[s] + (q ? ls[s].nlink[lat::OPP[q]] : 0) + 63) / 64) * 64;
-            if (len * esz >= (1L << 32)) return fail(c, MFLBM_ERR_ARG, "sample %d: slice exceeds 32-bit byte offsets", s);
-            so[q + 1] = so[q] + len;
-        }
-        c->elem_off[s] = elems;
-        elems += so[19];
-        maxdense = std::max(maxdense, L.total);
+        in[s] = BatchSampleIn{h.nx, h.ny, h.nz, h.jper, h.walls_h2.data()};
     }
-    c->nwaves = waves;
+    BatchPlan P;
+    {
+        const std::string why = plan_batch(in, c->tile, esz, P);
+        if (!why.empty()) return fail(c, MFLBM_ERR_ARG, "%s", why.c_str());
+    }
+    const std::vector<CompactSet>& cs = P.cs;
+    const std::vector<LinkSet>& ls = P.ls;
+    c->L = P.L;
+    c->nf = P.nf;
+    c->nc = P.nc;
+    c->wave_first_h = P.wave_first;
+    c->nwave_h = P.nwave;
+    c->elem_off = P.elem_off;
+    c->soff = P.soff;
+    c->nfluid = P.nfluid;
+    c->nwaves = P.nwaves;
+    const long elems = P.elems, maxdense = P.maxdense;
+    const int waves = P.nwaves;
     // device buffers
     HIPCHK(c, hipMalloc(&c->pop, (size_t)(esz * std::max(elems, 1L))));
     std::vector<SDev<double>> d64(n);
@@ -510,9 +495,9 @@ extern "C" int mflbm_spb_commit(mflbm_spb_ctx* c) {
     HIPCHK(c, hipMalloc(&c->devs, dsz * n));
     if (c->fp32) HIPCHK(c, hipMemcpy(c->devs, d32.data(), dsz * n, hipMemcpyHostToDevice));
     else HIPCHK(c, hipMemcpy(c->devs, d64.data(), dsz * n, hipMemcpyHostToDevice));
-    std::vector<int> ws((size_t)std::max(waves, 1), 0), nzs(n), yp(n);
+    const std::vector<int>& ws = P.wave_sample;
+    std::vector<int> nzs(n), yp(n);
     for (int s = 0; s < n; ++s) {
-        for (int w = 0; w < c->nwave_h[s]; ++w) ws[(size_t)c->wave_first_h[s] + w] = s;
         nzs[s] = c->host[s].nz;
         yp[s] = c->host[s].jper;
     }

@@ -125,6 +125,47 @@ module mflbm_c_binding
             integer(c_int8_t), intent(in) :: walls_global(*)
             integer(c_int8_t), intent(inout) :: walls_h2(*)
         end function
+        ! ---- host set-up (geometry_preprocessing_new, initialization_new_multi, the inlet profile)
+        integer(c_int) function mflbm_geometry_new(params, walls_global, theta, geom) bind(c, name="mflbm_geometry_new")
+            import :: c_int, c_int8_t, c_double, c_ptr, mflbm_params
+            type(mflbm_params), intent(in) :: params
+            integer(c_int8_t), intent(in) :: walls_global(*)
+            real(c_double), value :: theta
+            type(c_ptr), intent(out) :: geom
+        end function
+        integer(c_int) function mflbm_geometry_counts(geom, n_solid, n_fluid) bind(c, name="mflbm_geometry_counts")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: geom
+            integer(c_int32_t), intent(out) :: n_solid, n_fluid
+        end function
+        integer(c_int) function mflbm_geometry_export(geom, s_ijk, s_nfluid, s_nbr, s_weight, f_ijk, f_nw, f_theta) &
+                bind(c, name="mflbm_geometry_export")
+            import :: c_int, c_ptr, c_int32_t, c_double
+            type(c_ptr), value :: geom
+            integer(c_int32_t), intent(inout) :: s_ijk(*), s_nfluid(*), s_nbr(*), f_ijk(*)
+            real(c_double), intent(inout) :: s_weight(*), f_nw(*), f_theta(*)
+        end function
+        subroutine mflbm_geometry_free(geom) bind(c, name="mflbm_geometry_free")
+            import :: c_ptr
+            type(c_ptr), value :: geom
+        end subroutine
+        ! u0, v0, w0: c_null_ptr for a run starting at rest; f, g: 19 addresses each
+        integer(c_int) function mflbm_init_fields(params, option, interface_z0, u0, v0, w0, f, g, phi, f_convec, &
+                g_convec, phi_convec) bind(c, name="mflbm_init_fields")
+            import :: c_int, c_ptr, c_int32_t, c_double, mflbm_params
+            type(mflbm_params), intent(in) :: params
+            integer(c_int32_t), value :: option
+            real(c_double), value :: interface_z0
+            type(c_ptr), value :: u0, v0, w0
+            type(c_ptr), intent(in) :: f(19), g(19)
+            real(c_double), intent(inout) :: phi(*), f_convec(*), g_convec(*), phi_convec(*)
+        end function
+        integer(c_int) function mflbm_inlet_profile(params, uin_avg, la_x, la_y, w_in) bind(c, name="mflbm_inlet_profile")
+            import :: c_int, c_double, mflbm_params
+            type(mflbm_params), intent(in) :: params
+            real(c_double), value :: uin_avg, la_x, la_y
+            real(c_double), intent(inout) :: w_in(*)
+        end function
         ! ---- single-phase solver: main_iteration_kernel (singlephase_3D/0.src/Main.F90:291-422) -> mflbm_sp_step
         integer(c_int) function mflbm_sp_create(params, ctx) bind(c, name="mflbm_sp_create")
             import :: c_int, c_ptr, mflbm_sp_params

@@ -169,6 +169,13 @@ class REVBatch:
         self._call("download_state", i, native.ptr_array(list(f)))
         return f
 
+    def macro(self, i: int) -> Dict[str, np.ndarray]:
+        """u, v, w, rho of sample i (after an even step) in the halo-1 layout, solid nodes 0."""
+        nx, ny, nz = self.samples[i].shape
+        out = {k: np.zeros((nz + 2, ny + 2, nx + 2)) for k in ("u", "v", "w", "rho")}
+        self._call("download_macro", i, *(native.dptr(out[k]) for k in ("u", "v", "w", "rho")))
+        return out
+
     def upload(self, i: int, f: np.ndarray) -> None:
         f = np.ascontiguousarray(f, dtype=np.float64)
         self._keep = f

@@ -4,6 +4,8 @@
   (hipcc, FP contraction off so the FP64
   arithmetic rounds exactly as written; packed-FP32 instructions disabled).
 * ``oracle/liboracle.so``: the CPU checker (g++, OpenMP), built only when its sources are present.
+* ``mf-lbm_amd/fortran/bin/hotpath_run``: the Fortran main program that drives the hot path through the
+  ISO_C_BINDING module (flang), when flang is present.
 """
 from __future__ import annotations
 
@@ -63,9 +65,31 @@ def build_oracle(verbose: bool = False) -> str | None:
     return ORACLE_LIB
 
 
+FORTRAN_DIR = os.path.join(ROOT, "fortran")
+FORTRAN_BIN = os.path.join(FORTRAN_DIR, "bin", "hotpath_run")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The Fortran driver, linked against the in-tree libmflbm.so (found at run time relative to the binary)."""
+    flang = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    if not os.path.exists(flang):
+        return None
+    out_dir = os.path.dirname(FORTRAN_BIN)
+    mod_dir = os.path.join(out_dir, "mod")
+    os.makedirs(mod_dir, exist_ok=True)
+    obj = os.path.join(mod_dir, "mflbm_c_binding.o")
+    _run([flang, "-O2", "-c", os.path.join(FORTRAN_DIR, "mflbm_c_binding.f90"), "-module-dir", mod_dir, "-o", obj])
+    tmp = FORTRAN_BIN + ".tmp"
+    _run([flang, "-O2", os.path.join(FORTRAN_DIR, "hotpath_run.f90"), obj, f"-I{mod_dir}",
+          f"-L{os.path.dirname(LIB)}", "-lmflbm", "-Wl,-rpath,$ORIGIN/../../mflbm/lib", "-o", tmp])
+    os.replace(tmp, FORTRAN_BIN)
+    return FORTRAN_BIN
+
+
 def build_all(verbose: bool = False) -> None:
     build_device(verbose)
     build_oracle(verbose)
+    build_fortran(verbose)
 
 
 if __name__ == "__main__":
