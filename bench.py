@@ -43,6 +43,7 @@ B_ALG32 = 321.0        # two-phase FP32 storage: 2 x 19 x 4 B x (read + write) +
 B_ALG_SP = 305.0       # bytes per fluid node per step, single-phase FP64
 B_ALG_SP32 = 153.0     # bytes per fluid node per step, single-phase FP32 deviation storage
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+DEVELOP_STEPS = 0      # two-phase: untimed steps before the warm-up
 
 
 def workload(n: int = 256, precision: str = "fp64"):
@@ -87,10 +88,14 @@ def traffic_from_profiles(name: str):
     return None
 
 
-def timed_run(sim, args, nwhich):
-    """Warm-up, timed steps, then a second pass of the same length with HIP-event kernel timing."""
+def timed_run(sim, args, nwhich, develop=0):
+    """`develop` untimed steps, warm-up, timed steps, then a second pass of the same length with HIP-event kernel
+    timing."""
     import ctypes as C
     ntime = 1
+    if develop:
+        sim.run(ntime, develop)
+        ntime += develop
     sim.run(ntime, args.warmup)
     ntime += args.warmup
     sim.synchronize()
@@ -110,14 +115,63 @@ def timed_run(sim, args, nwhich):
     return dt, means
 
 
-def cpu_leg(args, case, st, single_phase=False):
+OMP_BINDINGS = {"spread": {"OMP_PLACES": "cores", "OMP_PROC_BIND": "spread"}, "unbound": {}}
+
+
+def host_description() -> dict:
+    import platform
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "machine": platform.machine()}
+    try:
+        txt = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for line in txt.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[key.strip()] = val.strip()
+    except OSError:
+        pass
+    return info
+
+
+def cpu_child(args) -> None:
+    """The CPU leg's measurement, in a process of its own so that its OpenMP runtime starts with the binding
+    given in its environment (the parent's runtime may already be up)."""
+    single = args.workload == "singlephase"
+    case = workload_sp(args.size, "fp64") if single else workload(args.size, "fp64")
+    threads = int(os.environ["OMP_NUM_THREADS"])
+    v, used = cpu_baseline(case, case.initial_state(), args.cpu_steps, threads, single)
+    print(json.dumps({"mlups": v, "threads": used}))
+
+
+def cpu_leg(args, single_phase=False):
+    """The oracle on the host cores: every core this process may use (OMP_NUM_THREADS when the environment
+    sets it, as the GPU machines do for their CPU share), once with OMP_PLACES=cores OMP_PROC_BIND=spread and
+    once without a binding; the faster is reported, both are in the sample text."""
     if args.no_cpu:
         return None
-    threads = min(os.cpu_count() or 1, 16)
-    v, used = cpu_baseline(case, st, args.cpu_steps, threads, single_phase)
-    return {"value": round(v, 3), "unit": "MLUPS", "cores": used, "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same {args.size}^3 workload after 2 untimed steps, "
-                      f"C++/OpenMP oracle, {used} threads"}
+    import subprocess
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    runs = {}
+    for name, env_add in OMP_BINDINGS.items():
+        env = {k: v for k, v in os.environ.items() if k not in ("OMP_PLACES", "OMP_PROC_BIND")}
+        env.update(env_add, OMP_NUM_THREADS=str(threads))
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--workload", args.workload, "--size",
+               str(args.size), "--cpu-steps", str(args.cpu_steps)]
+        r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU leg ({name}) failed with status {r.returncode}")
+        runs[name] = json.loads(r.stdout.strip().splitlines()[-1])
+        print(f"[bench] cpu leg {name}: {runs[name]['mlups']:.2f} MLUPS on {runs[name]['threads']} threads",
+              file=sys.stderr, flush=True)
+    best = max(runs, key=lambda k: runs[k]["mlups"])
+    host = host_description()
+    return {"value": round(runs[best]["mlups"], 3), "unit": "MLUPS", "cores": runs[best]["threads"], "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same {args.size}^3 workload (FP64, from its initial state) after "
+                      f"2 untimed steps, C++/OpenMP oracle on {runs[best]['threads']} threads; "
+                      + ", ".join(f"{k} {runs[k]['mlups']:.2f} MLUPS" for k in runs)
+                      + f" ({OMP_BINDINGS['spread']['OMP_PLACES']}/{OMP_BINDINGS['spread']['OMP_PROC_BIND']} vs "
+                      "no binding); host " + json.dumps(host, sort_keys=True),
+            "binding": best, "host": host}
 
 
 def roofline(b_alg, n_fluid, kernel_ms, traffic, kernel, **extra):
@@ -136,7 +190,7 @@ def run_twophase(args) -> dict:
     st = case.initial_state()
     print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
     sim = Simulation(case, state=st)
-    dt, means = timed_run(sim, args, (0, 1, 2))
+    dt, means = timed_run(sim, args, (0, 1, 2), develop=args.develop)
     mon = sim.monitor()
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
@@ -152,12 +206,13 @@ def run_twophase(args) -> dict:
                                + ({"fp32": "FP32 population storage", "mixed": "FP32 storage, FP64 collision"}
                                   .get(args.precision, "FP64")) + "; the reference's "
                                "colour model (two D3Q19 population sets: the reference has no D3Q7 lattice)",
-                   "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3},
+                   "fluid_nodes": case.n_fluid, "lattice": [args.size] * 3,
+                   "timed_from_step": args.develop + args.warmup + 1},
         "roofline": roofline(B_ALG32 if fp32 else B_ALG, case.n_fluid, means[0],
                              traffic_from_profiles(f"{'fp32_' if fp32 else ''}sphere_pack_{args.size}"),
                              "k_collide (even/odd mean)", gradient_chain_ms=round(means[1], 4),
                              step_ms_events=round(means[2], 4)),
-        "cpu_baseline": cpu_leg(args, workload(args.size) if fp32 else case, st),
+        "cpu_baseline": cpu_leg(args),
         "check": {"saturation": mon.saturation, "umax": mon.umax, "failed": mon.failed},
     }
 
@@ -186,7 +241,7 @@ def run_singlephase(args) -> dict:
         "roofline": roofline(B_ALG_SP32 if fp32 else B_ALG_SP, case.n_fluid, means[0],
                              traffic_from_profiles(f"sp{'32' if fp32 else ''}_sphere_pack_{args.size}"),
                              "k_sp_collide (even/odd mean)", step_ms_events=round(means[1], 4)),
-        "cpu_baseline": cpu_leg(args, case, st, single_phase=True),
+        "cpu_baseline": cpu_leg(args, single_phase=True),
         "check": {"flowrate": mon.flowrate, "umax": mon.umax, "failed": mon.failed},
     }
 
@@ -398,7 +453,14 @@ def main() -> None:
     ap.add_argument("--batch-max-steps", type=int, default=100000)
     ap.add_argument("--precision", choices=("fp64", "fp32", "mixed"), default="fp64",
                     help="population storage: fp64 (reference arithmetic) or fp32")
+    ap.add_argument("--develop", type=int, default=DEVELOP_STEPS,
+                    help="two-phase: untimed steps before the warm-up, so that the timed steps see a developed "
+                         "interface")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs a single domain on one MI355X (multi-GPU z-slab "
               "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)

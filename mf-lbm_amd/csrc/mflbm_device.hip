@@ -64,20 +64,23 @@ struct Dev {
     float* cnz32;
     float* cnorm32;
     double* curv;     // dense curvature, filled only for diagnostics
-    // Quiet-node flags of the list chain.  The normal field is exactly +0 (all bits clear) away from the
-    // interface, and there the curvature force of a fluid node is a constant: the lattice update reads the
-    // stencil only where a flag says a bit may be set.
-    //   act[m]:    fluid node m has a normal with a bit set at itself or an ISO4 neighbour (set by the chain,
-    //              cleared by the lattice update that consumes it)
-    //   xmark[t]:  an entry of the normal-extrapolation list has a neighbour whose normal has a bit set
-    //   xdirty[t]: the entry's last extrapolated normal had a bit set
-    //   xskip:     every neighbour the extrapolation reads is a node of the normal list (checked at upload),
-    //              so an entry without a mark extrapolates to +0
-    unsigned char* act;
-    unsigned char* xmark;
-    unsigned char* xdirty;
-    const int* role;  // padded offset -> -(normal-extrapolation list index) - 2 for its entries
-    int xskip;
+    // Brick flags of the list chain.  The padded grid is cut into bricks of 8 x 8 x 8 nodes (brick_of).
+    // Each stage of the colour-gradient chain is a function of phi within 3 nodes, so where phi has not
+    // changed within a brick and its neighbour bricks since the chain last ran, every value the chain
+    // would write there equals the one stored, and the stage skips the brick.  Away from the interface
+    // the normal field is exactly +0 (all bits clear), and there the curvature force of a fluid node is a
+    // constant: the lattice update reads the normal stencil only in bricks near a set bit.
+    //   chg[b]:  a writer of phi stored a value with other bits than before in brick b (the lattice update,
+    //            boundary conditions, periodic exchange, the solid zeroing of the macroscopic pass); the
+    //            buffer of the current step, read and then cleared by the next chain (two buffers in turn)
+    //   actb[b]: brick b or one of its 26 neighbours changed: the chain stages evaluate it
+    //   nzb[b]:  a stored normal of brick b has a bit set (recomputed for the bricks the chain evaluates)
+    //   nzd[b]:  nzb of brick b or one of its 26 neighbours is set: the lattice update reads the stencil
+    unsigned char* chg;
+    unsigned char* actb;
+    unsigned char* nzb;
+    unsigned char* nzd;
+    int nbx, nby, nbz;  // bricks per axis of the padded grid
     const int8_t* walls;
     double* w_in;     // (nx+2)(ny+2), reference layout
     double* cf;       // convective-outlet history f: 19 planes (nx+2)(ny+2)
@@ -144,6 +147,21 @@ template <class T>
 __device__ __forceinline__ Pop<T> gpop(const Dev& d) { return Pop<T>{&d, 19}; }
 __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
+// brick of padded offset n (8 x 8 x 8 padded nodes per brick)
+__device__ __forceinline__ int brick_of(const Dev& d, int n) {
+    const int kp = n / d.L.pxy;
+    const int r = n - kp * d.L.pxy;
+    const int jp = r / d.L.px;
+    const int ip = r - jp * d.L.px;
+    return (ip >> 3) + d.nbx * ((jp >> 3) + d.nby * (kp >> 3));
+}
+__device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
+// phi store of every writer outside the chain: flags the brick when the stored bits change
+__device__ __forceinline__ void store_phi(const Dev& d, int n, double v) {
+    if (!same_bits(d.phi[n], v)) d.chg[brick_of(d, n)] = 1;
+    d.phi[n] = v;
+}
+
 
 // ------------------------------------------------------------------------------------------------------
 // lattice update: collision + recolouring + AA-pattern streaming of both colours, one thread per fluid node
@@ -185,15 +203,15 @@ k_collide(Dev d, CollideParams P, int per) {
     const SlotTable& S = d.S;
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
+    const int nn = d.cpos[m];
+    const double phi_old = d.phi[nn];
     double cnx, cny, cnz, tf;
     if (STENCIL) {
-        if (d.act[m]) {
-            const int nn = d.cpos[m];
+        if (d.nzd[brick_of(d, nn)]) {
             cnx = d.cnx[nn];
             cny = d.cny[nn];
             cnz = d.cnz[nn];
             tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
-            d.act[m] = 0;
         } else {   // every normal of the stencil is +0: the same expressions on those values
             cnx = 0.0;
             cny = 0.0;
@@ -216,7 +234,7 @@ k_collide(Dev d, CollideParams P, int per) {
         b[0] = v.y;
     }
     if (ODD) {
-        const int n = d.cpos[m];
+        const int n = nn;
         // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
         const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
         const unsigned lane = (unsigned)m & 63u;
@@ -242,7 +260,8 @@ k_collide(Dev d, CollideParams P, int per) {
         }
     }
     const double phi = collide_node_tf(a, b, cnx, cny, cnz, tf, P);
-    d.phi[d.cpos[m]] = phi;
+    if (!same_bits(phi_old, phi)) d.chg[brick_of(d, nn)] = 1;
+    d.phi[nn] = phi;
     st_pair(S.p[0], m, a[0], b[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {   // odd: slot opp(q) of x + e_q
@@ -261,14 +280,14 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
+    const double phi_old = d.phi[nn];
     C cnx, cny, cnz, tf;
     if (STENCIL) {
-        if (d.act[m]) {
+        if (d.nzd[brick_of(d, nn)]) {
             cnx = (C)d.cnx32[nn];
             cny = (C)d.cny32[nn];
             cnz = (C)d.cnz32[nn];
             tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
-            d.act[m] = 0;
         } else {
             cnx = (C)0.0f;
             cny = (C)0.0f;
@@ -314,6 +333,7 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
         }
     }
     const C phi = collide_node_dev<C>(s, a, cnx, cny, cnz, tf, P);
+    if (!same_bits(phi_old, (double)phi)) d.chg[brick_of(d, nn)] = 1;
     d.phi[nn] = (double)phi;
     st_pair32(S.p[0], (unsigned)m, (float)s[0], (float)a[0]);
 #pragma unroll
@@ -481,7 +501,7 @@ __global__ void k_wrap_phi(Dev d, int wrap_y, int wrap_z) {
     if (gz && !wrap_z) return;
     const int js = gy ? (j < 1 ? j + L.ny : j - L.ny) : j;
     const int ks = gz ? (k < 1 ? k + L.nz : k - L.nz) : k;
-    d.phi[L.at(i, j, k)] = d.phi[L.at(i, js, ks)];
+    store_phi(d, L.at(i, j, k), d.phi[L.at(i, js, ks)]);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -491,12 +511,11 @@ __global__ void k_wrap_phi(Dev d, int wrap_y, int wrap_z) {
 
 __device__ __forceinline__ void set_inlet_phi(const Dev& d, int i, int j, double phi_inlet, int wi) {
     const Layout L = d.L;
-    double* ph = d.phi;
-    ph[L.at(i, j, 0)] = phi_inlet * (double)(1 - wi) + ph[L.at(i, j, 0)] * (double)wi;
-    const double v = ph[L.at(i, j, 0)];
-    ph[L.at(i, j, -1)] = v;
-    ph[L.at(i, j, -2)] = v;
-    ph[L.at(i, j, -3)] = v;
+    const double v = phi_inlet * (double)(1 - wi) + d.phi[L.at(i, j, 0)] * (double)wi;
+    store_phi(d, L.at(i, j, 0), v);
+    store_phi(d, L.at(i, j, -1), v);
+    store_phi(d, L.at(i, j, -2), v);
+    store_phi(d, L.at(i, j, -3), v);
 }
 
 // inlet_bounce_back_velocity_BC_before_odd (Boundary_multiphase_inlet.F90:6-53)
@@ -618,13 +637,13 @@ __device__ __forceinline__ void bc_outlet_convective_before(const Dev& d, int t,
     const double om = (double)(1 - wi), wd = (double)wi;
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
-    double* ph = d.phi;
-    ph[L.at(i, j, nz + 1)] = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
-    d.cphi[pc] = ph[L.at(i, j, nz + 1)];
-    const double v = ph[L.at(i, j, nz + 1)];
-    ph[L.at(i, j, nz + 2)] = v;
-    ph[L.at(i, j, nz + 3)] = v;
-    ph[L.at(i, j, nz + 4)] = v;
+    const double* ph = d.phi;
+    const double v = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    d.cphi[pc] = v;
+    store_phi(d, L.at(i, j, nz + 1), v);
+    store_phi(d, L.at(i, j, nz + 2), v);
+    store_phi(d, L.at(i, j, nz + 3), v);
+    store_phi(d, L.at(i, j, nz + 4), v);
     // (slot, x offset, y offset) of the five unknown populations at the outlet halo
     const int QS[5] = {6, 13, 14, 17, 18};
     const int DI[5] = {0, -1, 1, 0, 0};
@@ -654,13 +673,13 @@ __device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, 
     const double om = (double)(1 - wi), wd = (double)wi;
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
-    double* ph = d.phi;
-    ph[L.at(i, j, nz + 1)] = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
-    d.cphi[pc] = ph[L.at(i, j, nz + 1)];
-    const double v = ph[L.at(i, j, nz + 1)];
-    ph[L.at(i, j, nz + 2)] = v;
-    ph[L.at(i, j, nz + 3)] = v;
-    ph[L.at(i, j, nz + 4)] = v;
+    const double* ph = d.phi;
+    const double v = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    d.cphi[pc] = v;
+    store_phi(d, L.at(i, j, nz + 1), v);
+    store_phi(d, L.at(i, j, nz + 2), v);
+    store_phi(d, L.at(i, j, nz + 3), v);
+    store_phi(d, L.at(i, j, nz + 4), v);
     // slot written at z = nz and the history slot it relaxes toward
     const int QW[5] = {5, 11, 12, 15, 16};
     const int QH[5] = {6, 14, 13, 18, 17};
@@ -685,12 +704,11 @@ __device__ __forceinline__ void bc_outlet_pressure_before(const Dev& d, int t, d
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
-    double* ph = d.phi;
-    const double pz = ph[L.at(i, j, nz)];
-    ph[L.at(i, j, nz + 1)] = pz;
-    ph[L.at(i, j, nz + 2)] = pz;
-    ph[L.at(i, j, nz + 3)] = pz;
-    ph[L.at(i, j, nz + 4)] = pz;
+    const double pz = d.phi[L.at(i, j, nz)];
+    store_phi(d, L.at(i, j, nz + 1), pz);
+    store_phi(d, L.at(i, j, nz + 2), pz);
+    store_phi(d, L.at(i, j, nz + 3), pz);
+    store_phi(d, L.at(i, j, nz + 4), pz);
     const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
     const int a0 = L.at(i, j, nz), am = L.at(i - 1, j, nz), ap = L.at(i + 1, j, nz), bm = L.at(i, j - 1, nz),
               bp = L.at(i, j + 1, nz), mm = L.at(i - 1, j - 1, nz), pm = L.at(i + 1, j - 1, nz),
@@ -724,12 +742,11 @@ __device__ __forceinline__ void bc_outlet_pressure_after(const Dev& d, int t, do
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
-    double* ph = d.phi;
-    const double pz = ph[L.at(i, j, nz)];
-    ph[L.at(i, j, nz + 1)] = pz;
-    ph[L.at(i, j, nz + 2)] = pz;
-    ph[L.at(i, j, nz + 3)] = pz;
-    ph[L.at(i, j, nz + 4)] = pz;
+    const double pz = d.phi[L.at(i, j, nz)];
+    store_phi(d, L.at(i, j, nz + 1), pz);
+    store_phi(d, L.at(i, j, nz + 2), pz);
+    store_phi(d, L.at(i, j, nz + 3), pz);
+    store_phi(d, L.at(i, j, nz + 4), pz);
     const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
     const int o = L.at(i, j, nz);
     double tmp1 = (f(0, o) + f(2, o) + f(1, o) + f(4, o) + f(3, o) + f(8, o) + f(7, o) + f(10, o) + f(9, o) +
@@ -966,14 +983,6 @@ __device__ __forceinline__ Normals<float> normals<float>(const Dev& d) { return 
 // any bit of the stored value set (-0.0 and NaN count as set)
 __device__ __forceinline__ bool bits_set(double v) { return __double_as_longlong(v) != 0; }
 __device__ __forceinline__ bool bits_set(float v) { return __float_as_uint(v) != 0u; }
-// flag the interior fluid nodes among node n (q0 = 0) and its 18 neighbours: their curvature stencil holds n
-__device__ __forceinline__ void mark_fluid(const Dev& d, int n, int q0) {
-#pragma unroll
-    for (int q = q0; q < 19; ++q) {
-        const int m = d.map[n + d.L.off(q)];
-        if ((unsigned)m < (unsigned)d.nf) d.act[m] = 1;
-    }
-}
 struct ZeroAt {
     __device__ __forceinline__ double operator()(int, int, int) const { return 0.0; }
 };
@@ -984,6 +993,7 @@ __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, i
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= s.n) return;
     const int n = s.pos[t];
+    if (!d.actb[brick_of(d, n)]) return;
     const unsigned mk = s.mask[t];
     double acc = 0.0;
 #pragma unroll
@@ -1029,29 +1039,22 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     if (t >= nl.n) return;
     const Layout L = d.L;
     const int n = nl.pos[t];
+    const int b = brick_of(d, n);
+    if (!d.actb[b]) return;
     double cx = dx4(d.phi, n, L.px, L.pxy);
     double cy = dy4(d.phi, n, L.px, L.pxy);
     double cz = dz4(d.phi, n, L.px, L.pxy);
     double cn;
     normalise(cx, cy, cz, cn);
-    const int b = wb[t];
-    if (b >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * b], fl.nw[3 * b + 1], fl.nw[3 * b + 2], fl.cos_t[b], fl.sin_t[b]);
+    const int fb = wb[t];
+    if (fb >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
     const Normals<NT> N = normals<NT>(d);
     const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
     N.x[n] = sx;
     N.y[n] = sy;
     N.z[n] = sz;
     N.n[n] = sn;
-    if (bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn)) {
-        // fluid nodes whose curvature stencil holds this node, and extrapolation entries that read it
-        mark_fluid(d, n, 0);
-#pragma unroll
-        for (int q = 1; q < 19; ++q) {
-            const int o = n + L.off(q);
-            const int r = d.role[o];
-            if (r <= -2) d.xmark[-r - 2] = 1;
-        }
-    }
+    if (bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn)) d.nzb[b] = 1;
 }
 // normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110)
 template <class NT>
@@ -1060,19 +1063,8 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     if (t >= s.n) return;
     const Normals<NT> N = normals<NT>(d);
     const int n = s.pos[t];
-    if (d.xskip) {
-        if (!d.xmark[t]) {
-            // every neighbour's normal is +0, so the weighted mean is +0: store it once, then nothing to do
-            if (d.xdirty[t]) {
-                N.x[n] = (NT)0.0;
-                N.y[n] = (NT)0.0;
-                N.z[n] = (NT)0.0;
-                d.xdirty[t] = 0;
-            }
-            return;
-        }
-        d.xmark[t] = 0;
-    }
+    const int b = brick_of(d, n);
+    if (!d.actb[b]) return;
     const unsigned mk = s.mask[t];
     double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
@@ -1087,9 +1079,39 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     N.x[n] = sx;
     N.y[n] = sy;
     N.z[n] = sz;
-    const bool set = bits_set(sx) | bits_set(sy) | bits_set(sz);
-    if (d.xskip) d.xdirty[t] = set ? 1 : 0;
-    if (set) mark_fluid(d, n, 1);
+    if (bits_set(sx) | bits_set(sy) | bits_set(sz)) d.nzb[b] = 1;
+}
+// Brick flags of one chain (see Dev): actb = chg dilated by one brick (every brick when `full`), nzb cleared
+// where the chain is about to recompute it, and the other change buffer cleared for the next step.
+__global__ void k_brick_active(Dev d, const unsigned char* __restrict__ chg, unsigned char* __restrict__ chg_next, int full) {
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    const int nb = d.nbx * d.nby * d.nbz;
+    if (b >= nb) return;
+    const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
+    bool a = full != 0;
+    for (int z = bz - 1; z <= bz + 1 && !a; ++z)
+        for (int y = by - 1; y <= by + 1 && !a; ++y)
+            for (int x = bx - 1; x <= bx + 1 && !a; ++x)
+                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz &&
+                    chg[x + d.nbx * (y + d.nby * z)])
+                    a = true;
+    d.actb[b] = a ? 1 : 0;
+    if (a) d.nzb[b] = 0;
+    chg_next[b] = 0;
+}
+// nzd = nzb dilated by one brick, after the chain has written the normal field
+__global__ void k_brick_normals(Dev d) {
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    const int nb = d.nbx * d.nby * d.nbz;
+    if (b >= nb) return;
+    const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
+    bool a = false;
+    for (int z = bz - 1; z <= bz + 1 && !a; ++z)
+        for (int y = by - 1; y <= by + 1 && !a; ++y)
+            for (int x = bx - 1; x <= bx + 1 && !a; ++x)
+                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz && d.nzb[x + d.nbx * (y + d.nby * z)])
+                    a = true;
+    d.nzd[b] = a ? 1 : 0;
 }
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
@@ -1340,7 +1362,7 @@ __global__ void k_zero_solid_phi(Dev d) {
     const int i = r - (j - 1) * L.nx + 1;
     const int n = L.at(i, j, k);
     const int wi = d.walls[n];
-    if (wi != 0) d.phi[n] = 0.0 * (double)wi + d.phi[n] * (double)(1 - wi);
+    if (wi != 0) store_phi(d, n, 0.0 * (double)wi + d.phi[n] * (double)(1 - wi));
 }
 // one block per z layer: the seven layer sums of Monitor.F90:62-84 and per-layer partials of the global
 // scalars, reduced in a fixed tree (deterministic run to run)
@@ -1494,7 +1516,10 @@ struct mflbm_ctx {
     double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
     float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
     int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
-    unsigned char* qbuf = nullptr;   // quiet-node flags: act (per fluid node), xmark, xdirty (per extrapolation entry)
+    unsigned char* qbuf = nullptr;   // brick flags: chg (two buffers), actb, nzb, nzd (Dev)
+    long nbricks = 0;
+    int chg_cur = 0;                 // change buffer the writers of phi fill until the next chain
+    bool full_chain = true;          // the next chain evaluates every brick (new geometry or state)
     bool chain_fused = false;   // one fused pass for normals, extrapolation and curvature (MFLBM_CHAIN=fused)
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
@@ -1540,10 +1565,24 @@ template <class NT>
 int launch_normals_dense(mflbm_ctx* c) {
     LAUNCH(c, k_normals<NT>, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
     LAUNCH(c, k_normal_extrapolate<NT>, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
+    LAUNCH(c, k_brick_normals, nblocks(c->nbricks), c->d);
     return MFLBM_OK;
 }
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c) {
+    // the bricks this chain evaluates: those near a change of phi since the last chain (every brick after new
+    // geometry or state, and always with the fused chain, which does not use the flags); then the writers of
+    // phi switch to the other change buffer, which this launch clears
+    {
+        const long nb = c->nbricks;
+        unsigned char* cur = c->qbuf + (size_t)c->chg_cur * nb;
+        unsigned char* next = c->qbuf + (size_t)(c->chg_cur ^ 1) * nb;
+        const int full = (c->full_chain || c->chain_fused) ? 1 : 0;
+        LAUNCH(c, k_brick_active, nblocks(nb), c->d, (const unsigned char*)cur, next, full);
+        c->chg_cur ^= 1;
+        c->d.chg = next;
+        c->full_chain = false;
+    }
     LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
     if (c->chain_fused) {
         const Layout& L = c->L;
@@ -1908,15 +1947,6 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     for (int t = 0; t < nf; ++t)
         if (wp[fpos[t]] == 0) role[fpos[t]] = t;
     for (size_t u = 0; u < s1pos.size(); ++u) role[s1pos[u]] = -(int)u - 2;
-    // the extrapolation may skip entries without a mark only if every neighbour it reads is a node of the
-    // normal list (non-solid), so that the normal kernel's marks see every value it reads
-    bool xskip = true;
-    for (size_t u = 0; u < s1pos.size() && xskip; ++u)
-        for (int q = 1; q < 19; ++q)
-            if ((s1mask[u] & (1u << q)) && wp[s1pos[u] + L.off(q)] != 0) {
-                xskip = false;
-                break;
-            }
     // device buffers
     free_geometry(c);
     // slice q: nc compact slots, then the link slots of direction opp(q), each slot the pair (colour 1, colour
@@ -1969,10 +1999,15 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMemcpy(c->map, cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->role, sizeof(int) * L.total));
     HIPCHK(c, hipMemcpy(c->role, role.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
-    // quiet-node flags start set: the first chain and lattice update evaluate everything
-    const size_t nact = (size_t)fs, nx1 = s1pos.size() + 1;
-    HIPCHK(c, hipMalloc(&c->qbuf, nact + 2 * nx1));
-    HIPCHK(c, hipMemset(c->qbuf, 1, nact + 2 * nx1));
+    // brick flags (Dev): two change buffers, actb, nzb, nzd; all set, so the first chain and lattice update
+    // evaluate everything
+    const int nbx = L.px / 8, nby = (L.ny + 8 + 7) / 8, nbz = (L.nz + 8 + 7) / 8;
+    const size_t nbr = (size_t)nbx * nby * nbz;
+    HIPCHK(c, hipMalloc(&c->qbuf, 5 * nbr));
+    HIPCHK(c, hipMemset(c->qbuf, 1, 5 * nbr));
+    c->nbricks = (long)nbr;
+    c->chg_cur = 0;
+    c->full_chain = true;
     HIPCHK(c, hipMemcpy(c->ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->mbuf, hm.data(), sizeof(unsigned) * hm.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dbuf, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
@@ -1993,11 +2028,13 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     d.fs = fs;
     d.wmask = c->wmask;
     d.wbase = c->wbase;
-    d.act = c->qbuf;
-    d.xmark = c->qbuf + nact;
-    d.xdirty = c->qbuf + nact + nx1;
-    d.role = c->role;
-    d.xskip = xskip ? 1 : 0;
+    d.chg = c->qbuf;
+    d.actb = c->qbuf + 2 * nbr;
+    d.nzb = c->qbuf + 3 * nbr;
+    d.nzd = c->qbuf + 4 * nbr;
+    d.nbx = nbx;
+    d.nby = nby;
+    d.nbz = nbz;
     const int* ip = c->ibuf + nc;
     c->nrm = NodeList{(int)nrm.size(), ip};
     ip += nrm.size();
@@ -2042,6 +2079,7 @@ extern "C" int mflbm_upload_state(mflbm_ctx* c, const double* const* f, const do
     // sees the whole field as changed, as the reference's unset phi_old does)
     HIPCHK(c, hipMemset(c->phi_old, 0, sizeof(double) * c->ps));
     c->have_state = true;
+    c->full_chain = true;
     return MFLBM_OK;
 }
 
@@ -2067,6 +2105,7 @@ extern "C" int mflbm_init_state_device(mflbm_ctx* c, int32_t option, double inte
     // sees the whole field as changed, as the reference's unset phi_old does)
     HIPCHK(c, hipMemset(c->phi_old, 0, sizeof(double) * c->ps));
     c->have_state = true;
+    c->full_chain = true;
     return MFLBM_OK;
 }
 
@@ -2101,6 +2140,7 @@ extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* cons
 extern "C" int mflbm_color_gradient(mflbm_ctx* c) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_geom || !c->have_state) return fail(c, MFLBM_ERR_STATE, "upload geometry and state first");
+    c->full_chain = true;
     return launch_gradient(c);
 }
 
