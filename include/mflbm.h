@@ -320,6 +320,18 @@ int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, i
 int64_t mflbm_nn_conv3d_dw_work(int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout);
 int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* dw, float* dbias, float* work, int64_t work_floats,
                                int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight training loops, harnesses that forbid hipMalloc): mflbm_nn_conv3d_dw_ordered with the workspace taken
+ * from a fixed array of mflbm_nn_conv3d_dw_scratch_floats() floats (16 MiB) that the library holds on each GPU
+ * until the process ends.  It calls no hipMalloc/hipFree; the scratch address of the current device is looked
+ * up once per device on the first call there.  MFLBM_ERR_ARG when mflbm_nn_conv3d_dw_work(...) exceeds the array
+ * (or for the arguments mflbm_nn_conv3d_dw_ordered refuses).  Results are bit-identical to
+ * mflbm_nn_conv3d_dw_ordered.  Every call on one GPU uses that GPU's one array: calls that overlap on one GPU
+ * (other host threads, or other streams without ordering between them) race; `stream` must belong to the current
+ * device. */
+int64_t mflbm_nn_conv3d_dw_scratch_floats(void);
+int mflbm_nn_conv3d_dw_ordered_noalloc(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d,
+                                       int32_t h, int32_t w, int32_t cin, int32_t cout, void* stream);
 int mflbm_nn_relu_bwd(void* g, const void* y, int64_t n, void* stream);
 /* channels 0..2 of the BF16 channels-last input x (d, h, w, cin): fluid indicator (fluid: int8 1 fluid, 0 solid),
  * d / 8 and (d / 8)^2, d the Euclidean distance to the nearest solid node capped at 16 (wrapped along the axes

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "mflbm.h"
 
@@ -404,17 +405,16 @@ extern "C" int64_t mflbm_nn_conv3d_dw_work(int32_t n, int32_t d, int32_t h, int3
     return ((M + DW_CHUNK - 1) / DW_CHUNK) * cout * 27L * cin + ((M + DB_CHUNK - 1) / DB_CHUNK) * cout;
 }
 
-extern "C" int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* dw, float* dbias, float* work,
-                                          int64_t work_floats, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
-                                          int32_t cout, void* stream) {
-    if (!x || !dy || !dw || !work || n < 1 || d < 1 || h < 1 || w < 8 || w % 8 || cin < 8 || cin % 8 || cout < 1)
-        return MFLBM_ERR_ARG;
-    if (work_floats < mflbm_nn_conv3d_dw_work(n, d, h, w, cin, cout)) return MFLBM_ERR_ARG;
-    const Shape S{n, d, h, w};
+namespace {
+
+// The launches of the fixed-order weight and bias gradients on a workspace of at least
+// mflbm_nn_conv3d_dw_work(...) floats.  Every workspace element the sums read is written first by the same
+// call (each chunk slice of the weight and bias partials is stored whole), so the workspace needs no clearing.
+int dw_ordered_launch(const void* x, const void* dy, float* dw, float* dbias, float* work, const Shape& S, int cin,
+                      int cout, hipStream_t st) {
     const long M = S.count();
     const long K = 27L * cin, nzw = (M + DW_CHUNK - 1) / DW_CHUNK, nzb = (M + DB_CHUNK - 1) / DB_CHUNK;
     float* bpart = work + nzw * cout * K;
-    const hipStream_t st = (hipStream_t)stream;
     const dim3 grid((unsigned)((cout + 31) / 32), (unsigned)((K + 127) / 128), (unsigned)nzw);
     hipLaunchKernelGGL(k_conv_dw<true>, grid, dim3(NT), 0, st, (const __bf16*)x, (const __bf16*)dy, work, S, (int)cin,
                        (int)cout, DW_CHUNK);
@@ -427,6 +427,60 @@ extern "C" int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* 
                            (long)cout, (int)nzb);
     }
     return launched(st);
+}
+
+// Workspace of the allocation-free weight gradient: one array per GPU in the library's code object, held until
+// the process ends.  Host code never passes its name to a kernel (on the host it is a stand-in, not a device
+// address): the device address of each GPU's copy comes from hipGetSymbolAddress once per device, with that
+// device current, and only that address is used.
+constexpr long DW_SCRATCH_FLOATS = 4L << 20;   // 16 MiB
+static __device__ __align__(256) float scratch[DW_SCRATCH_FLOATS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+float* scratch_ptr[MAX_DEVICES] = {};
+
+// the current device's scratch address, resolved on the first call on that device
+int scratch_of_current_device(float** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return MFLBM_ERR_NODEV;
+    if (dev < 0 || dev >= MAX_DEVICES) return MFLBM_ERR_ARG;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MFLBM_ERR_HIP;
+        scratch_ptr[dev] = (float*)p;
+    }
+    *out = scratch_ptr[dev];
+    return MFLBM_OK;
+}
+
+bool dw_args_ok(const void* x, const void* dy, const float* dw, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
+                int32_t cout) {
+    return x && dy && dw && n >= 1 && d >= 1 && h >= 1 && w >= 8 && w % 8 == 0 && cin >= 8 && cin % 8 == 0 && cout >= 1;
+}
+
+}  // namespace
+
+extern "C" int mflbm_nn_conv3d_dw_ordered(const void* x, const void* dy, float* dw, float* dbias, float* work,
+                                          int64_t work_floats, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
+                                          int32_t cout, void* stream) {
+    if (!work || !dw_args_ok(x, dy, dw, n, d, h, w, cin, cout)) return MFLBM_ERR_ARG;
+    if (work_floats < mflbm_nn_conv3d_dw_work(n, d, h, w, cin, cout)) return MFLBM_ERR_ARG;
+    return dw_ordered_launch(x, dy, dw, dbias, work, Shape{n, d, h, w}, cin, cout, (hipStream_t)stream);
+}
+
+extern "C" int64_t mflbm_nn_conv3d_dw_scratch_floats(void) { return DW_SCRATCH_FLOATS; }
+
+extern "C" int mflbm_nn_conv3d_dw_ordered_noalloc(const void* x, const void* dy, float* dw, float* dbias, int32_t n,
+                                                  int32_t d, int32_t h, int32_t w, int32_t cin, int32_t cout,
+                                                  void* stream) {
+    if (!dw_args_ok(x, dy, dw, n, d, h, w, cin, cout)) return MFLBM_ERR_ARG;
+    if (mflbm_nn_conv3d_dw_work(n, d, h, w, cin, cout) > DW_SCRATCH_FLOATS) return MFLBM_ERR_ARG;
+    float* work = nullptr;
+    const int rc = scratch_of_current_device(&work);
+    if (rc) return rc;
+    return dw_ordered_launch(x, dy, dw, dbias, work, Shape{n, d, h, w}, cin, cout, (hipStream_t)stream);
 }
 
 extern "C" int mflbm_nn_weights_to_bf16(const float* w, void* wf, void* wb, int32_t cin, int32_t cout, void* stream) {

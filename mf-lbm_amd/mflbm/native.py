@@ -141,6 +141,8 @@ _NN_ENTRY_POINTS = {
     "mflbm_nn_conv3d_dw_work": (C.c_int64, [c_int, c_int, c_int, c_int, c_int, c_int]),
     "mflbm_nn_conv3d_dw_ordered": (c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int64, c_int, c_int, c_int, c_int, c_int, c_int,
                                            _VP]),
+    "mflbm_nn_conv3d_dw_scratch_floats": (C.c_int64, []),
+    "mflbm_nn_conv3d_dw_ordered_noalloc": (c_int, [_VP, _VP, _VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
     "mflbm_nn_relu_bwd": (c_int, [_VP, _VP, C.c_int64, _VP]),
     "mflbm_nn_pool2": (c_int, [_VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),
     "mflbm_nn_up2": (c_int, [_VP, _VP, c_int, c_int, c_int, c_int, c_int, c_int, _VP]),

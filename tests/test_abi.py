@@ -36,3 +36,17 @@ def test_create_without_gpu_fails_cleanly():
     ctx = C.c_void_p()
     assert fn(C.byref(p), C.byref(ctx)) == 1
     assert b"at least" in lib.mflbm_create_error()
+
+
+def test_allocation_free_weight_gradient_refuses_oversize_before_any_device_call():
+    """The size check of mflbm_nn_conv3d_dw_ordered_noalloc runs on the host before the scratch lookup or any
+    launch, so it holds without a GPU (the pointers are never dereferenced)."""
+    lib = native.load()
+    assert lib.mflbm_nn_conv3d_dw_scratch_floats() == 4 << 20
+    shape, cin, cout = (1, 32, 32, 160), 64, 64
+    assert lib.mflbm_nn_conv3d_dw_work(*shape, cin, cout) > lib.mflbm_nn_conv3d_dw_scratch_floats()
+    fake = C.c_void_p(4096)
+    assert lib.mflbm_nn_conv3d_dw_ordered_noalloc(fake, fake, fake, None, *shape, cin, cout, None) == 1
+    # and the argument checks of the workspace form
+    assert lib.mflbm_nn_conv3d_dw_ordered_noalloc(None, fake, fake, None, 1, 8, 8, 8, 8, 8, None) == 1
+    assert lib.mflbm_nn_conv3d_dw_ordered_noalloc(fake, fake, fake, None, 1, 8, 8, 12, 8, 8, None) == 1
