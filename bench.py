@@ -112,7 +112,7 @@ def timed_run(sim, args, nwhich, develop=0):
         sim._call("timing", which, C.byref(m), C.byref(cnt))
         means.append(m.value)
     sim._call("timing_enable", 0)
-    return dt, means
+    return dt, means, ntime + args.steps
 
 
 OMP_BINDINGS = {"spread": {"OMP_PLACES": "cores", "OMP_PROC_BIND": "spread"}, "unbound": {}}
@@ -190,8 +190,20 @@ def run_twophase(args) -> dict:
     st = case.initial_state()
     print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
     sim = Simulation(case, state=st)
-    dt, means = timed_run(sim, args, (0, 1, 2), develop=args.develop)
+    dt, means, ntime = timed_run(sim, args, (0, 1, 2), develop=args.develop)
     mon = sim.monitor()
+    developed = None
+    if args.developed_at > ntime:
+        # the same run continued to a developed interface, then the same number of steps timed again
+        sim.run(ntime, args.developed_at - ntime)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        sim.run(args.developed_at, args.steps)
+        sim.synchronize()
+        dtd = time.perf_counter() - t1
+        md = sim.monitor()
+        developed = {"timed_from_step": args.developed_at, "value": round(case.n_fluid * args.steps / dtd / 1e6, 3),
+                     "ms_per_step": round(dtd / args.steps * 1e3, 4), "saturation": md.saturation}
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
     fp32 = args.precision != "fp64"
@@ -212,6 +224,7 @@ def run_twophase(args) -> dict:
                              traffic_from_profiles(f"{'fp32_' if fp32 else ''}sphere_pack_{args.size}"),
                              "k_collide (even/odd mean)", gradient_chain_ms=round(means[1], 4),
                              step_ms_events=round(means[2], 4)),
+        "developed_interface": developed,
         "cpu_baseline": cpu_leg(args),
         "check": {"saturation": mon.saturation, "umax": mon.umax, "failed": mon.failed},
     }
@@ -224,7 +237,7 @@ def run_singlephase(args) -> dict:
     st = case.initial_state()
     print(f"[bench] set-up {time.time() - t0:.1f}s, fluid nodes {case.n_fluid}", file=sys.stderr, flush=True)
     sim = SPSimulation(case, state=st)
-    dt, means = timed_run(sim, args, (0, 2))
+    dt, means, _ = timed_run(sim, args, (0, 2))
     mon = sim.monitor()
     sim.close()
     mlups = case.n_fluid * args.steps / dt / 1e6
@@ -456,6 +469,9 @@ def main() -> None:
     ap.add_argument("--develop", type=int, default=DEVELOP_STEPS,
                     help="two-phase: untimed steps before the warm-up, so that the timed steps see a developed "
                          "interface")
+    ap.add_argument("--developed-at", type=int, default=3000,
+                    help="two-phase: after the measurement, continue the run to this step and time the same "
+                         "number of steps again (0: off)")
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_child:

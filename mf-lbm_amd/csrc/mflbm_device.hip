@@ -64,20 +64,28 @@ struct Dev {
     float* cnz32;
     float* cnorm32;
     double* curv;     // dense curvature, filled only for diagnostics
-    // Brick flags of the list chain.  The padded grid is cut into bricks of 8 x 8 x 8 nodes (brick_of).
-    // Each stage of the colour-gradient chain is a function of phi within 3 nodes, so where phi has not
-    // changed within a brick and its neighbour bricks since the chain last ran, every value the chain
-    // would write there equals the one stored, and the stage skips the brick.  Away from the interface
-    // the normal field is exactly +0 (all bits clear), and there the curvature force of a fluid node is a
-    // constant: the lattice update reads the normal stencil only in bricks near a set bit.
-    //   chg[b]:  a writer of phi stored a value with other bits than before in brick b (the lattice update,
-    //            boundary conditions, periodic exchange, the solid zeroing of the macroscopic pass); the
-    //            buffer of the current step, read and then cleared by the next chain (two buffers in turn)
-    //   actb[b]: brick b or one of its 26 neighbours changed: the chain stages evaluate it
-    //   nzb[b]:  a stored normal of brick b has a bit set (recomputed for the bricks the chain evaluates)
+    // Brick flags of the list chain.  The padded grid is cut into bricks of 4 x 4 x 4 nodes (brick_of).  Away
+    // from the interface phi is +-1 to within rounding, and there the ISO4 gradient is below the 1e-6 at which
+    // the reference zeroes the normal (Phase_gradient.F90:63-79): if every phi a node's normal depends on (its
+    // stencil, and the fluid neighbours of the solid nodes in it) lies within PHI_BAND of the same +-1, the
+    // normal is exactly +0 whatever the values, and so is every normal extrapolated from such normals.
+    //   notm[b], notp[b]: a fluid node of brick b holds a phi farther than PHI_BAND from -1 (from +1); set by
+    //            the lattice update of the current step, read and cleared by the next chain (two buffers in turn)
+    //   bstatic[b]: brick b holds a non-solid node outside the interior (halo planes that boundary conditions,
+    //            the periodic exchange or nobody write) or an interior node within 4 of a periodic face (whose
+    //            phi the periodic exchange copies into the halo): never quiet
+    //   q1b[b]:  every brick within one of b is quiet with one sign: the normals of b are +0 (not evaluated)
+    //   q2b[b]:  q1b of every brick within one of b: the extrapolated normals of b are +0, and no evaluated
+    //            normal reads the extrapolated phi of b's solid nodes (not evaluated; downloads refresh them)
+    //   zfill[b]: the stored normals of b may have a bit set (nzb before this chain): skipped stages write +0
+    //   nzb[b]:  a stored normal of brick b has a bit set
     //   nzd[b]:  nzb of brick b or one of its 26 neighbours is set: the lattice update reads the stencil
-    unsigned char* chg;
-    unsigned char* actb;
+    unsigned char* notm;
+    unsigned char* notp;
+    const unsigned char* bstatic;
+    unsigned char* q1b;
+    unsigned char* q2b;
+    unsigned char* zfill;
     unsigned char* nzb;
     unsigned char* nzd;
     int nbx, nby, nbz;  // bricks per axis of the padded grid
@@ -147,19 +155,24 @@ template <class T>
 __device__ __forceinline__ Pop<T> gpop(const Dev& d) { return Pop<T>{&d, 19}; }
 __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
-// brick of padded offset n (8 x 8 x 8 padded nodes per brick)
+// brick of padded offset n (4 x 4 x 4 padded nodes per brick)
+constexpr int BRICK_SHIFT = 2;
 __device__ __forceinline__ int brick_of(const Dev& d, int n) {
     const int kp = n / d.L.pxy;
     const int r = n - kp * d.L.pxy;
     const int jp = r / d.L.px;
     const int ip = r - jp * d.L.px;
-    return (ip >> 3) + d.nbx * ((jp >> 3) + d.nby * (kp >> 3));
+    return (ip >> BRICK_SHIFT) + d.nbx * ((jp >> BRICK_SHIFT) + d.nby * (kp >> BRICK_SHIFT));
 }
-__device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
-// phi store of every writer outside the chain: flags the brick when the stored bits change
-__device__ __forceinline__ void store_phi(const Dev& d, int n, double v) {
-    if (!same_bits(d.phi[n], v)) d.chg[brick_of(d, n)] = 1;
-    d.phi[n] = v;
+// Half-width of the band around +-1 within which phi counts as pure.  With every value of an ISO4 stencil within
+// eps of one constant, each difference is at most 2 eps and exact (Sterbenz), so |d/dx| <= (1/6 + 8/12) 2 eps and
+// |grad phi| <= sqrt(3) (5/3) eps = 2.89 eps (plus relative rounding of order 1e-15); a solid node's extrapolated
+// phi is a weighted mean of fluid values within the band, so inside it up to one rounding.  eps = 3e-7 keeps
+// |grad phi| below 8.7e-7 < 1e-6.
+constexpr double PHI_BAND = 3e-7;
+__device__ __forceinline__ void flag_band(const Dev& d, int b, double v) {
+    if (!(fabs(v + 1.0) <= PHI_BAND)) d.notm[b] = 1;
+    if (!(fabs(v - 1.0) <= PHI_BAND)) d.notp[b] = 1;
 }
 
 
@@ -204,7 +217,6 @@ k_collide(Dev d, CollideParams P, int per) {
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
-    const double phi_old = d.phi[nn];
     double cnx, cny, cnz, tf;
     if (STENCIL) {
         if (d.nzd[brick_of(d, nn)]) {
@@ -260,7 +272,7 @@ k_collide(Dev d, CollideParams P, int per) {
         }
     }
     const double phi = collide_node_tf(a, b, cnx, cny, cnz, tf, P);
-    if (!same_bits(phi_old, phi)) d.chg[brick_of(d, nn)] = 1;
+    flag_band(d, brick_of(d, nn), phi);
     d.phi[nn] = phi;
     st_pair(S.p[0], m, a[0], b[0]);
 #pragma unroll
@@ -280,7 +292,6 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
-    const double phi_old = d.phi[nn];
     C cnx, cny, cnz, tf;
     if (STENCIL) {
         if (d.nzd[brick_of(d, nn)]) {
@@ -333,7 +344,7 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
         }
     }
     const C phi = collide_node_dev<C>(s, a, cnx, cny, cnz, tf, P);
-    if (!same_bits(phi_old, (double)phi)) d.chg[brick_of(d, nn)] = 1;
+    flag_band(d, brick_of(d, nn), (double)phi);
     d.phi[nn] = (double)phi;
     st_pair32(S.p[0], (unsigned)m, (float)s[0], (float)a[0]);
 #pragma unroll
@@ -501,7 +512,7 @@ __global__ void k_wrap_phi(Dev d, int wrap_y, int wrap_z) {
     if (gz && !wrap_z) return;
     const int js = gy ? (j < 1 ? j + L.ny : j - L.ny) : j;
     const int ks = gz ? (k < 1 ? k + L.nz : k - L.nz) : k;
-    store_phi(d, L.at(i, j, k), d.phi[L.at(i, js, ks)]);
+    d.phi[L.at(i, j, k)] = d.phi[L.at(i, js, ks)];
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -511,11 +522,12 @@ __global__ void k_wrap_phi(Dev d, int wrap_y, int wrap_z) {
 
 __device__ __forceinline__ void set_inlet_phi(const Dev& d, int i, int j, double phi_inlet, int wi) {
     const Layout L = d.L;
-    const double v = phi_inlet * (double)(1 - wi) + d.phi[L.at(i, j, 0)] * (double)wi;
-    store_phi(d, L.at(i, j, 0), v);
-    store_phi(d, L.at(i, j, -1), v);
-    store_phi(d, L.at(i, j, -2), v);
-    store_phi(d, L.at(i, j, -3), v);
+    double* ph = d.phi;
+    const double v = phi_inlet * (double)(1 - wi) + ph[L.at(i, j, 0)] * (double)wi;
+    ph[L.at(i, j, 0)] = v;
+    ph[L.at(i, j, -1)] = v;
+    ph[L.at(i, j, -2)] = v;
+    ph[L.at(i, j, -3)] = v;
 }
 
 // inlet_bounce_back_velocity_BC_before_odd (Boundary_multiphase_inlet.F90:6-53)
@@ -637,13 +649,13 @@ __device__ __forceinline__ void bc_outlet_convective_before(const Dev& d, int t,
     const double om = (double)(1 - wi), wd = (double)wi;
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
-    const double* ph = d.phi;
+    double* ph = d.phi;
     const double v = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
     d.cphi[pc] = v;
-    store_phi(d, L.at(i, j, nz + 1), v);
-    store_phi(d, L.at(i, j, nz + 2), v);
-    store_phi(d, L.at(i, j, nz + 3), v);
-    store_phi(d, L.at(i, j, nz + 4), v);
+    ph[L.at(i, j, nz + 1)] = v;
+    ph[L.at(i, j, nz + 2)] = v;
+    ph[L.at(i, j, nz + 3)] = v;
+    ph[L.at(i, j, nz + 4)] = v;
     // (slot, x offset, y offset) of the five unknown populations at the outlet halo
     const int QS[5] = {6, 13, 14, 17, 18};
     const int DI[5] = {0, -1, 1, 0, 0};
@@ -673,13 +685,13 @@ __device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, 
     const double om = (double)(1 - wi), wd = (double)wi;
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
-    const double* ph = d.phi;
+    double* ph = d.phi;
     const double v = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
     d.cphi[pc] = v;
-    store_phi(d, L.at(i, j, nz + 1), v);
-    store_phi(d, L.at(i, j, nz + 2), v);
-    store_phi(d, L.at(i, j, nz + 3), v);
-    store_phi(d, L.at(i, j, nz + 4), v);
+    ph[L.at(i, j, nz + 1)] = v;
+    ph[L.at(i, j, nz + 2)] = v;
+    ph[L.at(i, j, nz + 3)] = v;
+    ph[L.at(i, j, nz + 4)] = v;
     // slot written at z = nz and the history slot it relaxes toward
     const int QW[5] = {5, 11, 12, 15, 16};
     const int QH[5] = {6, 14, 13, 18, 17};
@@ -704,11 +716,12 @@ __device__ __forceinline__ void bc_outlet_pressure_before(const Dev& d, int t, d
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
-    const double pz = d.phi[L.at(i, j, nz)];
-    store_phi(d, L.at(i, j, nz + 1), pz);
-    store_phi(d, L.at(i, j, nz + 2), pz);
-    store_phi(d, L.at(i, j, nz + 3), pz);
-    store_phi(d, L.at(i, j, nz + 4), pz);
+    double* ph = d.phi;
+    const double pz = ph[L.at(i, j, nz)];
+    ph[L.at(i, j, nz + 1)] = pz;
+    ph[L.at(i, j, nz + 2)] = pz;
+    ph[L.at(i, j, nz + 3)] = pz;
+    ph[L.at(i, j, nz + 4)] = pz;
     const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
     const int a0 = L.at(i, j, nz), am = L.at(i - 1, j, nz), ap = L.at(i + 1, j, nz), bm = L.at(i, j - 1, nz),
               bp = L.at(i, j + 1, nz), mm = L.at(i - 1, j - 1, nz), pm = L.at(i + 1, j - 1, nz),
@@ -742,11 +755,12 @@ __device__ __forceinline__ void bc_outlet_pressure_after(const Dev& d, int t, do
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
-    const double pz = d.phi[L.at(i, j, nz)];
-    store_phi(d, L.at(i, j, nz + 1), pz);
-    store_phi(d, L.at(i, j, nz + 2), pz);
-    store_phi(d, L.at(i, j, nz + 3), pz);
-    store_phi(d, L.at(i, j, nz + 4), pz);
+    double* ph = d.phi;
+    const double pz = ph[L.at(i, j, nz)];
+    ph[L.at(i, j, nz + 1)] = pz;
+    ph[L.at(i, j, nz + 2)] = pz;
+    ph[L.at(i, j, nz + 3)] = pz;
+    ph[L.at(i, j, nz + 4)] = pz;
     const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
     const int o = L.at(i, j, nz);
     double tmp1 = (f(0, o) + f(2, o) + f(1, o) + f(4, o) + f(3, o) + f(8, o) + f(7, o) + f(10, o) + f(9, o) +
@@ -989,11 +1003,12 @@ struct ZeroAt {
 __device__ __forceinline__ double quiet_curvature() { return curvature_f(ZeroAt{}, ZeroAt{}, ZeroAt{}); }
 
 // phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30)
-__global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per) {
+// (all = 0: only where an evaluated normal may read the value; all = 1: every entry, before a download)
+__global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per, int all) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= s.n) return;
     const int n = s.pos[t];
-    if (!d.actb[brick_of(d, n)]) return;
+    if (!all && d.q2b[brick_of(d, n)]) return;
     const unsigned mk = s.mask[t];
     double acc = 0.0;
 #pragma unroll
@@ -1040,7 +1055,16 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     const Layout L = d.L;
     const int n = nl.pos[t];
     const int b = brick_of(d, n);
-    if (!d.actb[b]) return;
+    const Normals<NT> N = normals<NT>(d);
+    if (d.q1b[b]) {   // every phi this normal depends on is within the band: the normal is +0
+        if (d.zfill[b]) {
+            N.x[n] = (NT)0.0;
+            N.y[n] = (NT)0.0;
+            N.z[n] = (NT)0.0;
+            N.n[n] = (NT)0.0;
+        }
+        return;
+    }
     double cx = dx4(d.phi, n, L.px, L.pxy);
     double cy = dy4(d.phi, n, L.px, L.pxy);
     double cz = dz4(d.phi, n, L.px, L.pxy);
@@ -1048,7 +1072,6 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     normalise(cx, cy, cz, cn);
     const int fb = wb[t];
     if (fb >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
-    const Normals<NT> N = normals<NT>(d);
     const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
     N.x[n] = sx;
     N.y[n] = sy;
@@ -1064,7 +1087,14 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     const Normals<NT> N = normals<NT>(d);
     const int n = s.pos[t];
     const int b = brick_of(d, n);
-    if (!d.actb[b]) return;
+    if (d.q2b[b]) {   // every normal it reads is +0: so is the weighted mean
+        if (d.zfill[b]) {
+            N.x[n] = (NT)0.0;
+            N.y[n] = (NT)0.0;
+            N.z[n] = (NT)0.0;
+        }
+        return;
+    }
     const unsigned mk = s.mask[t];
     double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
@@ -1081,23 +1111,44 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     N.z[n] = sz;
     if (bits_set(sx) | bits_set(sy) | bits_set(sz)) d.nzb[b] = 1;
 }
-// Brick flags of one chain (see Dev): actb = chg dilated by one brick (every brick when `full`), nzb cleared
-// where the chain is about to recompute it, and the other change buffer cleared for the next step.
-__global__ void k_brick_active(Dev d, const unsigned char* __restrict__ chg, unsigned char* __restrict__ chg_next, int full) {
+// Brick flags of one chain (see Dev): q1b from the band flags of the step (no brick quiet when `full`), zfill
+// from nzb, nzb cleared (the chain rewrites or keeps +0 every normal), the other band buffers cleared for the
+// next step.
+__global__ void k_brick_quiet(Dev d, const unsigned char* __restrict__ notm, const unsigned char* __restrict__ notp,
+                              unsigned char* __restrict__ next_m, unsigned char* __restrict__ next_p, int full) {
     const int b = blockIdx.x * BLOCK + threadIdx.x;
     const int nb = d.nbx * d.nby * d.nbz;
     if (b >= nb) return;
     const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
-    bool a = full != 0;
-    for (int z = bz - 1; z <= bz + 1 && !a; ++z)
-        for (int y = by - 1; y <= by + 1 && !a; ++y)
-            for (int x = bx - 1; x <= bx + 1 && !a; ++x)
-                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz &&
-                    chg[x + d.nbx * (y + d.nby * z)])
-                    a = true;
-    d.actb[b] = a ? 1 : 0;
-    if (a) d.nzb[b] = 0;
-    chg_next[b] = 0;
+    bool m = full != 0, p = full != 0;
+    for (int z = bz - 1; z <= bz + 1; ++z)
+        for (int y = by - 1; y <= by + 1; ++y)
+            for (int x = bx - 1; x <= bx + 1; ++x)
+                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) {
+                    const int c = x + d.nbx * (y + d.nby * z);
+                    const bool st = d.bstatic[c] != 0;
+                    m = m || st || notm[c];
+                    p = p || st || notp[c];
+                }
+    d.q1b[b] = (!m || !p) ? 1 : 0;
+    d.zfill[b] = d.nzb[b];
+    d.nzb[b] = 0;
+    next_m[b] = 0;
+    next_p[b] = 0;
+}
+// q2b = q1b of the brick and all 26 neighbours
+__global__ void k_brick_quiet2(Dev d) {
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    const int nb = d.nbx * d.nby * d.nbz;
+    if (b >= nb) return;
+    const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
+    bool q = true;
+    for (int z = bz - 1; z <= bz + 1 && q; ++z)
+        for (int y = by - 1; y <= by + 1 && q; ++y)
+            for (int x = bx - 1; x <= bx + 1 && q; ++x)
+                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz && !d.q1b[x + d.nbx * (y + d.nby * z)])
+                    q = false;
+    d.q2b[b] = q ? 1 : 0;
 }
 // nzd = nzb dilated by one brick, after the chain has written the normal field
 __global__ void k_brick_normals(Dev d) {
@@ -1362,7 +1413,7 @@ __global__ void k_zero_solid_phi(Dev d) {
     const int i = r - (j - 1) * L.nx + 1;
     const int n = L.at(i, j, k);
     const int wi = d.walls[n];
-    if (wi != 0) store_phi(d, n, 0.0 * (double)wi + d.phi[n] * (double)(1 - wi));
+    if (wi != 0) d.phi[n] = 0.0 * (double)wi + d.phi[n] * (double)(1 - wi);
 }
 // one block per z layer: the seven layer sums of Monitor.F90:62-84 and per-layer partials of the global
 // scalars, reduced in a fixed tree (deterministic run to run)
@@ -1516,10 +1567,11 @@ struct mflbm_ctx {
     double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
     float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
     int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
-    unsigned char* qbuf = nullptr;   // brick flags: chg (two buffers), actb, nzb, nzd (Dev)
+    unsigned char* qbuf = nullptr;   // brick flags: notm, notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd
     long nbricks = 0;
-    int chg_cur = 0;                 // change buffer the writers of phi fill until the next chain
+    int band_cur = 0;                // band buffers the lattice update fills until the next chain
     bool full_chain = true;          // the next chain evaluates every brick (new geometry or state)
+    bool solid_zeroed = false;       // the macroscopic pass zeroed phi in solids since the last chain
     bool chain_fused = false;   // one fused pass for normals, extrapolation and curvature (MFLBM_CHAIN=fused)
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
@@ -1570,20 +1622,25 @@ int launch_normals_dense(mflbm_ctx* c) {
 }
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c) {
-    // the bricks this chain evaluates: those near a change of phi since the last chain (every brick after new
-    // geometry or state, and always with the fused chain, which does not use the flags); then the writers of
-    // phi switch to the other change buffer, which this launch clears
+    // the bricks this chain may skip (none after new geometry or state, and none with the fused chain, which does
+    // not use the flags); then the lattice update switches to the other band buffers, which this launch clears
     {
         const long nb = c->nbricks;
-        unsigned char* cur = c->qbuf + (size_t)c->chg_cur * nb;
-        unsigned char* next = c->qbuf + (size_t)(c->chg_cur ^ 1) * nb;
+        unsigned char* m_cur = c->qbuf + (size_t)c->band_cur * nb;
+        unsigned char* p_cur = c->qbuf + (size_t)(2 + c->band_cur) * nb;
+        unsigned char* m_next = c->qbuf + (size_t)(c->band_cur ^ 1) * nb;
+        unsigned char* p_next = c->qbuf + (size_t)(2 + (c->band_cur ^ 1)) * nb;
         const int full = (c->full_chain || c->chain_fused) ? 1 : 0;
-        LAUNCH(c, k_brick_active, nblocks(nb), c->d, (const unsigned char*)cur, next, full);
-        c->chg_cur ^= 1;
-        c->d.chg = next;
+        LAUNCH(c, k_brick_quiet, nblocks(nb), c->d, (const unsigned char*)m_cur, (const unsigned char*)p_cur, m_next,
+               p_next, full);
+        LAUNCH(c, k_brick_quiet2, nblocks(nb), c->d);
+        c->band_cur ^= 1;
+        c->d.notm = m_next;
+        c->d.notp = p_next;
         c->full_chain = false;
+        c->solid_zeroed = false;
     }
-    LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
+    LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n), 0);
     if (c->chain_fused) {
         const Layout& L = c->L;
         const int ntx = (L.nx + CT - 1) / CT, nty = (L.ny + CT - 1) / CT, nseg = (L.nz + CSEG - 1) / CSEG;
@@ -1999,15 +2056,34 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMemcpy(c->map, cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->role, sizeof(int) * L.total));
     HIPCHK(c, hipMemcpy(c->role, role.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
-    // brick flags (Dev): two change buffers, actb, nzb, nzd; all set, so the first chain and lattice update
-    // evaluate everything
-    const int nbx = L.px / 8, nby = (L.ny + 8 + 7) / 8, nbz = (L.nz + 8 + 7) / 8;
+    // brick flags (Dev): notm and notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd.  The band flags,
+    // zfill, nzb and nzd start set, so the first chain and lattice update evaluate everything; bstatic from the
+    // walls: bricks with a non-solid node outside the interior, and bricks with an interior node within 4 of a
+    // periodic face
+    const int nbx = L.px >> BRICK_SHIFT, nby = (L.ny + 8 + 3) >> BRICK_SHIFT, nbz = (L.nz + 8 + 3) >> BRICK_SHIFT;
     const size_t nbr = (size_t)nbx * nby * nbz;
-    HIPCHK(c, hipMalloc(&c->qbuf, 5 * nbr));
-    HIPCHK(c, hipMemset(c->qbuf, 1, 5 * nbr));
+    std::vector<unsigned char> hstatic(nbr, 0);
+    {
+        const int py = L.ny + 8, pz = L.nz + 8;
+        for (int kp = 0; kp < pz; ++kp)
+            for (int jp = 0; jp < py; ++jp)
+                for (int ip = 0; ip < L.px; ++ip) {
+                    const int i = ip - XO, j = jp - YO, k = kp - ZO;
+                    const bool inside = i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && k >= 1 && k <= L.nz;
+                    const long o = (long)ip + (long)jp * L.px + (long)kp * L.pxy;
+                    const bool edge = inside && ((c->p.kper == 1 && (k <= 4 || k > L.nz - 4)) ||
+                                                 (c->p.jper == 1 && (j <= 4 || j > L.ny - 4)));
+                    if ((!inside && wp[o] == 0) || edge)
+                        hstatic[(ip >> BRICK_SHIFT) + nbx * ((jp >> BRICK_SHIFT) + (size_t)nby * (kp >> BRICK_SHIFT))] = 1;
+                }
+    }
+    HIPCHK(c, hipMalloc(&c->qbuf, 10 * nbr));
+    HIPCHK(c, hipMemset(c->qbuf, 1, 10 * nbr));
+    HIPCHK(c, hipMemcpy(c->qbuf + 4 * nbr, hstatic.data(), nbr, hipMemcpyHostToDevice));
     c->nbricks = (long)nbr;
-    c->chg_cur = 0;
+    c->band_cur = 0;
     c->full_chain = true;
+    c->solid_zeroed = false;
     HIPCHK(c, hipMemcpy(c->ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->mbuf, hm.data(), sizeof(unsigned) * hm.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dbuf, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
@@ -2028,10 +2104,14 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     d.fs = fs;
     d.wmask = c->wmask;
     d.wbase = c->wbase;
-    d.chg = c->qbuf;
-    d.actb = c->qbuf + 2 * nbr;
-    d.nzb = c->qbuf + 3 * nbr;
-    d.nzd = c->qbuf + 4 * nbr;
+    d.notm = c->qbuf;
+    d.notp = c->qbuf + 2 * nbr;
+    d.bstatic = c->qbuf + 4 * nbr;
+    d.q1b = c->qbuf + 5 * nbr;
+    d.q2b = c->qbuf + 6 * nbr;
+    d.zfill = c->qbuf + 7 * nbr;
+    d.nzb = c->qbuf + 8 * nbr;
+    d.nzd = c->qbuf + 9 * nbr;
     d.nbx = nbx;
     d.nby = nby;
     d.nbz = nbz;
@@ -2109,6 +2189,17 @@ extern "C" int mflbm_init_state_device(mflbm_ctx* c, int32_t option, double inte
     return MFLBM_OK;
 }
 
+// phi at the solid nodes as the reference holds it: the chain leaves the extrapolated phi of skipped bricks as it
+// was (no evaluated normal reads it), so before phi leaves the device every entry is extrapolated from the
+// current fluid phi, as the last chain of the reference did, and zeroed again inside solids when the macroscopic
+// pass has run since that chain (compute_macro_vars, Misc.F90:423)
+int refresh_solid_phi(mflbm_ctx* c) {
+    if (!c->have_geom || !c->have_state) return MFLBM_OK;
+    LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n), 1);
+    if (c->solid_zeroed) LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
+    return MFLBM_OK;
+}
+
 extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* const* g, double* phi, double* f_convec,
                                     double* g_convec, double* phi_convec) {
     if (!c) return MFLBM_ERR_ARG;
@@ -2117,6 +2208,10 @@ extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* cons
     {
         const int rs = c->fp32 ? scatter_state<float>(c) : scatter_state<double>(c);
         if (rs) return rs;
+    }
+    if (phi) {
+        const int rr = refresh_solid_phi(c);
+        if (rr) return rr;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
@@ -2173,6 +2268,7 @@ extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (c->fp32) LAUNCH(c, k_macro<float>, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
     else LAUNCH(c, k_macro<double>, nblocks(c->d.nf), c->d, c->p.force_z, c->u, c->v, c->w, c->rho);
     LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
+    c->solid_zeroed = true;
     return MFLBM_OK;
 }
 
@@ -2229,6 +2325,10 @@ extern "C" int mflbm_monitor(mflbm_ctx* c, double* layers, mflbm_monitor_sums* s
 extern "C" int mflbm_download_macro(mflbm_ctx* c, double* u, double* v, double* w, double* rho, double* phi) {
     if (!c) return MFLBM_ERR_ARG;
     const Layout& L = c->L;
+    if (phi) {
+        const int rr = refresh_solid_phi(c);
+        if (rr) return rr;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
     double* src[5] = {c->u, c->v, c->w, c->rho, c->d.phi};

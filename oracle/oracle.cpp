@@ -39,6 +39,9 @@ struct Arr3 {
         v.assign((size_t)sx * sy * (nz + 2 * h), 0.0);
     }
     double& operator()(int i, int j, int k) { return v[(size_t)(i - lo) + (size_t)(j - lo) * sx + (size_t)(k - lo) * sx * sy]; }
+    // flat index of (i, j, k) and the strides, for loops that walk rows with a running index
+    long at(int i, int j, int k) const { return (long)(i - lo) + (long)(j - lo) * sx + (long)(k - lo) * sx * sy; }
+    long pxy() const { return (long)sx * sy; }
 };
 
 }  // namespace
@@ -210,40 +213,67 @@ double collide(oracle_ctx* c, double* g1, double* g2, double cnx, double cny, do
     return phi;
 }
 
-// kernel_even_color (Kernel_multiphase.F90:371-725) / kernel_odd_color (:6-362) over the whole interior
-void lattice_update(oracle_ctx* c, bool odd) {
+// kernel_even_color (Kernel_multiphase.F90:371-725) / kernel_odd_color (:6-362) over the whole interior.
+// The 38 population arrays share one shape (halo 1), so one flat index per node addresses all of them; the
+// normal arrays (halo 2), curv (halo 1) and phi (halo 4) have their own.
+template <bool ODD>
+void lattice_update_t(oracle_ctx* c) {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-#pragma omp parallel for schedule(static)
+    double* F[19];
+    double* G[19];
+    long off[19];
+    const long sx = c->f[0].sx, sxy = c->f[0].pxy();
+    for (int q = 0; q < 19; ++q) {
+        F[q] = c->f[q].v.data();
+        G[q] = c->g[q].v.data();
+        off[q] = EX[q] + EY[q] * sx + EZ[q] * sxy;
+    }
+    double* const cx = c->cn_x.v.data();
+    double* const cy = c->cn_y.v.data();
+    double* const cz = c->cn_z.v.data();
+    double* const cn = c->c_norm.v.data();
+    double* const cu = c->curv.v.data();
+    double* const ph = c->phi.v.data();
+    // dynamic: nodes near the interface cost more (subnormal populations of the receding colour), so the
+    // layers do not cost the same
+#pragma omp parallel for schedule(dynamic, 1)
     for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j)
-            for (int i = 1; i <= nx; ++i) {
-                if (c->W(i, j, k) != 0) continue;
+        for (int j = 1; j <= ny; ++j) {
+            const long n0 = c->f[0].at(1, j, k), m0 = c->cn_x.at(1, j, k), u0 = c->curv.at(1, j, k), p0 = c->phi.at(1, j, k);
+            const int8_t* wrow = &c->W(1, j, k);
+            for (int i = 0; i < nx; ++i) {
+                if (wrow[i] != 0) continue;
+                const long n = n0 + i;
                 double g1[19], g2[19];
-                g1[0] = c->f[0](i, j, k);
-                g2[0] = c->g[0](i, j, k);
+                g1[0] = F[0][n];
+                g2[0] = G[0][n];
                 for (int q = 1; q < 19; ++q) {
-                    if (odd) {   // pull from the upwind neighbour
-                        g1[q] = c->f[q](i - EX[q], j - EY[q], k - EZ[q]);
-                        g2[q] = c->g[q](i - EX[q], j - EY[q], k - EZ[q]);
+                    if (ODD) {   // pull from the upwind neighbour
+                        g1[q] = F[q][n - off[q]];
+                        g2[q] = G[q][n - off[q]];
                     } else {     // own slots, stored swapped after the previous odd step
-                        g1[q] = c->f[OPC[q]](i, j, k);
-                        g2[q] = c->g[OPC[q]](i, j, k);
+                        g1[q] = F[OPC[q]][n];
+                        g2[q] = G[OPC[q]][n];
                     }
                 }
-                c->phi(i, j, k) = collide(c, g1, g2, c->cn_x(i, j, k), c->cn_y(i, j, k), c->cn_z(i, j, k), c->curv(i, j, k),
-                                          c->c_norm(i, j, k));
-                c->f[0](i, j, k) = g1[0];
-                c->g[0](i, j, k) = g2[0];
+                ph[p0 + i] = collide(c, g1, g2, cx[m0 + i], cy[m0 + i], cz[m0 + i], cu[u0 + i], cn[m0 + i]);
+                F[0][n] = g1[0];
+                G[0][n] = g2[0];
                 for (int q = 1; q < 19; ++q) {
-                    if (odd) {   // push into the downwind neighbour's opposite slot
-                        c->f[OPC[q]](i + EX[q], j + EY[q], k + EZ[q]) = g1[q];
-                        c->g[OPC[q]](i + EX[q], j + EY[q], k + EZ[q]) = g2[q];
+                    if (ODD) {   // push into the downwind neighbour's opposite slot
+                        F[OPC[q]][n + off[q]] = g1[q];
+                        G[OPC[q]][n + off[q]] = g2[q];
                     } else {
-                        c->f[q](i, j, k) = g1[q];
-                        c->g[q](i, j, k) = g2[q];
+                        F[q][n] = g1[q];
+                        G[q][n] = g2[q];
                     }
                 }
             }
+        }
+}
+void lattice_update(oracle_ctx* c, bool odd) {
+    if (odd) lattice_update_t<true>(c);
+    else lattice_update_t<false>(c);
 }
 
 // Periodic exchange of one rank with itself (Mpi.F90).  Pull type after even steps (:101-346): the
@@ -584,20 +614,19 @@ void porous_plate(oracle_ctx* c, bool after) {
 }
 
 // ISO4 derivative stencils with the reference's term order (Phase_gradient.F90:40-62, 122-192)
-double ddx(Arr3& A, int i, int j, int k) {
-    return ISO4_1 * (A(i + 1, j, k) - A(i - 1, j, k)) +
-           ISO4_2 * (A(i + 1, j + 1, k) - A(i - 1, j - 1, k) + A(i + 1, j - 1, k) - A(i - 1, j + 1, k) + A(i + 1, j, k + 1) -
-                     A(i - 1, j, k - 1) + A(i + 1, j, k - 1) - A(i - 1, j, k + 1));
+// ISO4 first derivatives at the element a points to (x stride 1, row stride px, plane stride pxy), the
+// reference's term order (Phase_gradient.F90:40-62)
+double ddx(const double* a, long px, long pxy) {
+    return ISO4_1 * (a[1] - a[-1]) +
+           ISO4_2 * (a[1 + px] - a[-1 - px] + a[1 - px] - a[-1 + px] + a[1 + pxy] - a[-1 - pxy] + a[1 - pxy] - a[-1 + pxy]);
 }
-double ddy(Arr3& A, int i, int j, int k) {
-    return ISO4_1 * (A(i, j + 1, k) - A(i, j - 1, k)) +
-           ISO4_2 * (A(i + 1, j + 1, k) - A(i - 1, j - 1, k) + A(i - 1, j + 1, k) - A(i + 1, j - 1, k) + A(i, j + 1, k + 1) -
-                     A(i, j - 1, k - 1) + A(i, j + 1, k - 1) - A(i, j - 1, k + 1));
+double ddy(const double* a, long px, long pxy) {
+    return ISO4_1 * (a[px] - a[-px]) +
+           ISO4_2 * (a[1 + px] - a[-1 - px] + a[-1 + px] - a[1 - px] + a[px + pxy] - a[-px - pxy] + a[px - pxy] - a[-px + pxy]);
 }
-double ddz(Arr3& A, int i, int j, int k) {
-    return ISO4_1 * (A(i, j, k + 1) - A(i, j, k - 1)) +
-           ISO4_2 * (A(i + 1, j, k + 1) - A(i - 1, j, k - 1) + A(i - 1, j, k + 1) - A(i + 1, j, k - 1) + A(i, j + 1, k + 1) -
-                     A(i, j - 1, k - 1) + A(i, j - 1, k + 1) - A(i, j + 1, k - 1));
+double ddz(const double* a, long px, long pxy) {
+    return ISO4_1 * (a[pxy] - a[-pxy]) +
+           ISO4_2 * (a[1 + pxy] - a[-1 - pxy] + a[-1 + pxy] - a[1 - pxy] + a[px + pxy] - a[-px - pxy] + a[-px + pxy] - a[px - pxy]);
 }
 
 // color_gradient (Phase_gradient.F90:5-204) with alter_color_gradient_solid_surface (:210-265)
@@ -616,24 +645,34 @@ void color_gradient(oracle_ctx* c) {
         c->phi(i, j, k) = acc / c->s_w[t];
     }
     // normalised colour gradient over [-1, n+2]^3 (:33-79)
+    {
+        const long ppx = c->phi.sx, ppxy = c->phi.pxy();
 #pragma omp parallel for schedule(static)
-    for (int k = -1; k <= nz + 2; ++k)
-        for (int j = -1; j <= ny + 2; ++j)
-            for (int i = -1; i <= nx + 2; ++i) {
-                double cx = ddx(c->phi, i, j, k), cy = ddy(c->phi, i, j, k), cz = ddz(c->phi, i, j, k);
-                double cn = std::sqrt(cx * cx + cy * cy + cz * cz);
-                if (cn < 1e-6 || c->W(i, j, k) == 1) {
-                    cx = cy = cz = cn = 0.0;
-                } else {
-                    cx = cx / cn;
-                    cy = cy / cn;
-                    cz = cz / cn;
+        for (int k = -1; k <= nz + 2; ++k)
+            for (int j = -1; j <= ny + 2; ++j) {
+                const double* prow = &c->phi(-1, j, k);
+                const int8_t* wrow = &c->W(-1, j, k);
+                double* xr = &c->cn_x(-1, j, k);
+                double* yr = &c->cn_y(-1, j, k);
+                double* zr = &c->cn_z(-1, j, k);
+                double* nr = &c->c_norm(-1, j, k);
+                for (int i = 0; i < nx + 4; ++i) {
+                    double cx = ddx(prow + i, ppx, ppxy), cy = ddy(prow + i, ppx, ppxy), cz = ddz(prow + i, ppx, ppxy);
+                    double cn = std::sqrt(cx * cx + cy * cy + cz * cz);
+                    if (cn < 1e-6 || wrow[i] == 1) {
+                        cx = cy = cz = cn = 0.0;
+                    } else {
+                        cx = cx / cn;
+                        cy = cy / cn;
+                        cz = cz / cn;
+                    }
+                    xr[i] = cx;
+                    yr[i] = cy;
+                    zr[i] = cz;
+                    nr[i] = cn;
                 }
-                c->cn_x(i, j, k) = cx;
-                c->cn_y(i, j, k) = cy;
-                c->cn_z(i, j, k) = cz;
-                c->c_norm(i, j, k) = cn;
             }
+    }
     // wetting: rotate the normal at fluid boundary nodes to the contact angle (:210-265)
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < nf; ++t) {
@@ -678,18 +717,26 @@ void color_gradient(oracle_ctx* c) {
         c->cn_z(i, j, k) = az / c->s_w[t];
     }
     // curvature of the interface (:112-201)
+    {
+        const long npx = c->cn_x.sx, npxy = c->cn_x.pxy();
 #pragma omp parallel for schedule(static)
-    for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j)
-            for (int i = 1; i <= nx; ++i) {
-                const double kxx = ddx(c->cn_x, i, j, k), kyy = ddy(c->cn_y, i, j, k), kzz = ddz(c->cn_z, i, j, k);
-                const double kxy = ddy(c->cn_x, i, j, k), kxz = ddz(c->cn_x, i, j, k);
-                const double kyx = ddx(c->cn_y, i, j, k), kyz = ddz(c->cn_y, i, j, k);
-                const double kzx = ddx(c->cn_z, i, j, k), kzy = ddy(c->cn_z, i, j, k);
-                const double x = c->cn_x(i, j, k), y = c->cn_y(i, j, k), z = c->cn_z(i, j, k);
-                c->curv(i, j, k) = (x * x - 1.0) * kxx + (y * y - 1.0) * kyy + (z * z - 1.0) * kzz + x * y * (kxy + kyx) +
-                                   x * z * (kxz + kzx) + y * z * (kzy + kyz);
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= ny; ++j) {
+                const double* X = &c->cn_x(1, j, k);
+                const double* Y = &c->cn_y(1, j, k);
+                const double* Z = &c->cn_z(1, j, k);
+                double* cr = &c->curv(1, j, k);
+                for (int i = 0; i < nx; ++i) {
+                    const double kxx = ddx(X + i, npx, npxy), kyy = ddy(Y + i, npx, npxy), kzz = ddz(Z + i, npx, npxy);
+                    const double kxy = ddy(X + i, npx, npxy), kxz = ddz(X + i, npx, npxy);
+                    const double kyx = ddx(Y + i, npx, npxy), kyz = ddz(Y + i, npx, npxy);
+                    const double kzx = ddx(Z + i, npx, npxy), kzy = ddy(Z + i, npx, npxy);
+                    const double x = X[i], y = Y[i], z = Z[i];
+                    cr[i] = (x * x - 1.0) * kxx + (y * y - 1.0) * kyy + (z * z - 1.0) * kzz + x * y * (kxy + kyx) +
+                            x * z * (kxz + kzx) + y * z * (kzy + kyz);
+                }
             }
+    }
 }
 
 // main_iteration_kernel (Main_multiphase.F90:341-486) for one rank without neighbours

@@ -162,24 +162,41 @@ void sp_collide(const oracle_sp_ctx* c, double* ft) {
     ft[18] = s4 - s6 - s7 - s8 * 2.0 + m[14] - m[17] + m[18];
 }
 
-void sp_lattice_update(oracle_sp_ctx* c, bool odd) {
+// kernel_odd / kernel_even (singlephase_3D/0.src/Kernel.F90:5-400); the 19 arrays share one shape, so one flat
+// index per node addresses all of them
+template <bool ODD>
+void sp_lattice_update_t(oracle_sp_ctx* c) {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-#pragma omp parallel for schedule(static)
+    double* F[19];
+    long off[19];
+    const long sx = c->f[0].sx, sxy = (long)c->f[0].sx * c->f[0].sy;
+    for (int q = 0; q < 19; ++q) {
+        F[q] = c->f[q].v.data();
+        off[q] = EX[q] + EY[q] * sx + EZ[q] * sxy;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
     for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j)
-            for (int i = 1; i <= nx; ++i) {
-                if (c->W(i, j, k) != 0) continue;
+        for (int j = 1; j <= ny; ++j) {
+            const long n0 = 1 + (long)j * sx + (long)k * sxy;
+            const int8_t* wrow = &c->W(1, j, k);
+            for (int i = 0; i < nx; ++i) {
+                if (wrow[i] != 0) continue;
+                const long n = n0 + i;
                 double ft[19];
-                ft[0] = c->f[0](i, j, k);
-                for (int q = 1; q < 19; ++q)
-                    ft[q] = odd ? c->f[q](i - EX[q], j - EY[q], k - EZ[q]) : c->f[OPC[q]](i, j, k);
+                ft[0] = F[0][n];
+                for (int q = 1; q < 19; ++q) ft[q] = ODD ? F[q][n - off[q]] : F[OPC[q]][n];
                 sp_collide(c, ft);
-                c->f[0](i, j, k) = ft[0];
+                F[0][n] = ft[0];
                 for (int q = 1; q < 19; ++q) {
-                    if (odd) c->f[OPC[q]](i + EX[q], j + EY[q], k + EZ[q]) = ft[q];
-                    else c->f[q](i, j, k) = ft[q];
+                    if (ODD) F[OPC[q]][n + off[q]] = ft[q];
+                    else F[q][n] = ft[q];
                 }
             }
+        }
+}
+void sp_lattice_update(oracle_sp_ctx* c, bool odd) {
+    if (odd) sp_lattice_update_t<true>(c);
+    else sp_lattice_update_t<false>(c);
 }
 
 // periodic exchange of a single rank with itself (Mpi.F90:73-448)

@@ -1,7 +1,8 @@
-"""The colour-gradient chain skips bricks (8^3 nodes) where phi has not changed within one brick since the
-chain last ran, and the lattice update skips the normal stencil in bricks with no set bit nearby.  These
-tests run domains large enough for most bricks to be skipped, with monitor calls in between (the macroscopic
-pass zeroes phi at solid nodes, which the chain must then re-extrapolate), and check:
+"""The colour-gradient chain skips bricks (4^3 nodes) whose neighbourhood holds phi within 3e-7 of one of +-1
+(there every normal is +0), leaves the extrapolated phi of solid nodes there as it was until a download needs
+it, and the lattice update skips the normal stencil in bricks with no set bit nearby.  These tests run
+domains large enough for most bricks to be skipped, with monitor calls in between and at the last step (the
+macroscopic pass zeroes phi at solid nodes), and check:
 
 * FP64: every population and phi bit for bit against the oracle, which evaluates every node every step;
 * FP32 storage (FP32 and mixed collision): bit for bit against the same run with a full chain forced after
@@ -16,7 +17,7 @@ from oracle_backend import OracleSimulation, set_threads
 
 pytestmark = pytest.mark.gpu
 
-MONITOR_AT = (25, 60, 61, 140)
+MONITOR_AT = (25, 60, 61, 140, 160, 200)
 
 
 def _drainage(precision="fp64"):
@@ -65,6 +66,10 @@ def test_brick_skipping_chain_is_bit_identical_to_the_oracle(make, nsteps):
     gg, go = gpu.gradient(), ora.gradient()
     for k in gg:
         assert np.array_equal(gg[k], go[k]), k
+    mg, mo = gpu.macro(), ora.macro()
+    for k in ("u", "v", "w", "rho"):
+        assert np.array_equal(mg[k][1:-1, 1:-1, 1:-1], mo[k][1:-1, 1:-1, 1:-1]), k
+    assert np.array_equal(mg["phi"].view(np.uint64), mo["phi"].view(np.uint64))
     gpu.close()
     ora.close()
 
