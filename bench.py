@@ -420,6 +420,69 @@ def run_warmstart(args) -> dict:
             "results": results, "total_seconds": round(time.perf_counter() - t0, 1)}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix-core peak
+
+
+def run_nn(args) -> dict:
+    """Matrix-core throughput of the warm-start network's convolution kernels: the forward convolution
+    (mflbm_nn_conv3d_fwd, implicit GEMM on v_mfma_f32_16x16x32_bf16) and the fixed-order weight gradient
+    (mflbm_nn_conv3d_dw_ordered) on the layer shapes of the network's levels at the training shape (4 fields of
+    64^3: level l is 64 / 2^l per side with 32 * 2^l channels).  2 * positions * 27 * cin * cout flops per call,
+    timed with HIP events on the stream the kernels run on; ``value`` is the forward kernel's TFLOP/s on the
+    first level."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from mflbm import native
+    lib = native.load()
+    P = lambda t: C.c_void_p(t.data_ptr())
+    st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+    rows = []
+    for lvl, (side, ch) in enumerate(((64, 32), (32, 64), (16, 128), (8, 256))):
+        n, cin, cout = 4, ch, ch
+        shape = (n, side, side, side)
+        M = n * side ** 3
+        kpad = (27 * cin + 31) // 32 * 32
+        x = (torch.randn(M * cin, device="cuda") * 0.5).to(torch.bfloat16)
+        g = (torch.randn(M * cout, device="cuda") * 0.5).to(torch.bfloat16)
+        w = torch.randn(cout * 27 * cin, device="cuda") * 0.05
+        wf = torch.zeros(cout * kpad, dtype=torch.bfloat16, device="cuda")
+        wb = torch.zeros(cin * ((27 * cout + 31) // 32 * 32), dtype=torch.bfloat16, device="cuda")
+        bias = torch.zeros(cout, device="cuda")
+        y = torch.empty(M * cout, dtype=torch.bfloat16, device="cuda")
+        dw = torch.zeros(cout * 27 * cin, device="cuda")
+        db = torch.zeros(cout, device="cuda")
+        need = int(lib.mflbm_nn_conv3d_dw_work(*shape, cin, cout))
+        work = torch.empty(need, device="cuda")
+        assert lib.mflbm_nn_weights_to_bf16(P(w), P(wf), P(wb), cin, cout, sp) == 0
+        flops = 2.0 * M * 27 * cin * cout
+        out = {"level": lvl, "positions": M, "cin": cin, "cout": cout, "gflop_per_call": round(flops / 1e9, 2)}
+        for name, call in (("fwd", lambda: lib.mflbm_nn_conv3d_fwd(P(x), P(wf), P(bias), P(y), *shape, cin, cout, 1, sp)),
+                           ("dw", lambda: lib.mflbm_nn_conv3d_dw_ordered(P(x), P(g), P(dw), P(db), P(work), need, *shape,
+                                                                         cin, cout, sp))):
+            for _ in range(3):
+                assert call() == 0
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 10
+            e0.record(st)
+            for _ in range(reps):
+                call()
+            e1.record(st)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            tf = flops / (ms * 1e-3) / 1e12
+            out[name] = {"ms": round(ms, 4), "tflops": round(tf, 2), "frac_of_bf16_peak": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
+        rows.append(out)
+        print(f"[bench] nn level {lvl}: {out}", file=sys.stderr, flush=True)
+    return {"metric": "warm-start convolution throughput (BF16 matrix cores)", "value": rows[0]["fwd"]["tflops"],
+            "unit": "TFLOP/s", "n_gpus": 1, "dtype": "bf16", "data": "synthetic",
+            "roofline": {"bound": "mfma", "achieved": rows[0]["fwd"]["tflops"], "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": rows[0]["fwd"]["frac_of_bf16_peak"], "traffic": None,
+                         "kernel": "k_conv_fwd (level 0: 4 x 64^3, 32 -> 32 channels)"},
+            "layers": rows}
+
+
 def run_copy(args) -> dict:
     """Calibration workload for the hardware byte counters: a plain device-to-device copy of args.copy_mib MiB.
     Under ``rocprofv3 --pmc FETCH_SIZE`` (or ``WRITE_SIZE``) the copy kernel reads (writes) exactly that many
@@ -454,7 +517,7 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report", "warmstart", "copy"),
+    ap.add_argument("--workload", choices=("twophase", "singlephase", "batch", "fp32report", "warmstart", "copy", "nn"),
                     default="twophase")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: MiB copied per dispatch")
     ap.add_argument("--batch-samples", type=int, default=96)
@@ -482,7 +545,7 @@ def main() -> None:
               "decomposition is a design, not part of the build); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     run = {"singlephase": run_singlephase, "batch": run_batch, "fp32report": run_fp32_report,
-           "warmstart": run_warmstart, "copy": run_copy}.get(args.workload, run_twophase)
+           "warmstart": run_warmstart, "copy": run_copy, "nn": run_nn}.get(args.workload, run_twophase)
     out = run(args)
     print(json.dumps(out))
 

@@ -34,6 +34,10 @@ struct BatchPlan {
 
 // Lays the batch out (z periodic, y periodic where jper, one node order of `tile` rows per y tile); esz is the
 // size of a stored population (4 or 8 bytes).  Returns an empty string, or why the batch cannot be laid out.
+// every sample is periodic along z, and along y where jper; z is folded into the neighbour map (layout.h) where it
+// is the only periodic axis
+inline Fold batch_fold(int jper) { return Fold{false, jper != 1}; }
+
 inline std::string plan_batch(const std::vector<BatchSampleIn>& in, int tile, long esz, BatchPlan& P) {
     const int n = (int)in.size();
     P = BatchPlan{};
@@ -56,7 +60,7 @@ inline std::string plan_batch(const std::vector<BatchSampleIn>& in, int tile, lo
         std::vector<int8_t> wp(L.total, 1);
         pack_host(L, 2, h.walls_h2, wp.data());
         const TileOrder T{tile, L};
-        build_compact(L, T, wp, false, true, h.jper == 1, {}, P.cs[s]);
+        build_compact(L, T, wp, false, true, h.jper == 1, {}, P.cs[s], batch_fold(h.jper));
         build_links(L, P.cs[s], P.ls[s]);
         P.nf[s] = P.cs[s].nfl;
         P.nc[s] = (int)P.cs[s].cpos.size();

@@ -63,6 +63,51 @@ __device__ __forceinline__ void slot_st(T* slice, unsigned i, T v) {
     *(T*)((char*)slice + i * (unsigned)sizeof(T)) = v;
 }
 
+// Exchanged halo slots of a folded domain (layout.h, Fold): the halo slot (q, h) of a direction q that points
+// into the domain along h's halo axes, and the slot (q, y) of h's image.  The reference's self-exchange leaves them
+// equal after every step.
+//   fold_exchanged: calls f(h, y, q) for each such pair of the one-layer halo node (i, j, k) (x interior).
+//   k_fold_dense (downloads): the compact storage kept the value at y, except where y is not fluid and the
+//     interior node x = h + e_q is: then it is x's link slot of direction opp(q), which the link download wrote
+//     at h, and y takes it from there; and where y holds no slot at all (no step touches the pair, and the first
+//     push exchange left (q, y) equal to the (q, h) the staging still holds).  One thread per node of the
+//     one-layer halo box; `ncol` population sets of 19 arrays each in the dense staging.
+template <class Fn>
+__device__ __forceinline__ void fold_exchanged(const Layout& L, const Fold& F, long t, Fn&& f) {
+    const long ny2 = L.ny + 2, nz2 = L.nz + 2;
+    if (t >= (long)L.nx * ny2 * nz2) return;
+    const int i = (int)(t % L.nx) + 1;
+    const int j = (int)((t / L.nx) % ny2);
+    const int k = (int)(t / ((long)L.nx * ny2));
+    int y;
+    if (!fold_image(L, F, i, j, k, y)) return;
+    const int h = L.at(i, j, k);
+    const bool gy = j == 0 || j == L.ny + 1, gz = k == 0 || k == L.nz + 1;
+    const int ey = j == 0 ? 1 : -1, ez = k == 0 ? 1 : -1;
+#pragma unroll
+    for (int q = 1; q < 19; ++q) {
+        if ((gy && lat::EY[q] != ey) || (gz && lat::EZ[q] != ez)) continue;
+        f(h, y, q);
+    }
+}
+inline unsigned fold_blocks(const Layout& L) { return nblocks((long)L.nx * (L.ny + 2) * (L.nz + 2)); }
+static __global__ void k_fold_dense(Layout L, Fold F, const int* __restrict__ map, int nf, double* __restrict__ dense,
+                                    long ps, int ncol) {
+    fold_exchanged(L, F, (long)blockIdx.x * BLOCK + threadIdx.x, [&](int h, int y, int q) {
+        const bool y_fluid = (unsigned)map[y] < (unsigned)nf;
+        const int x = h + L.off(q);
+        int xi, xj, xk;
+        decode(L, x, xi, xj, xk);
+        const bool x_inside = xi >= 1 && xi <= L.nx && xj >= 1 && xj <= L.ny && xk >= 1 && xk <= L.nz;
+        const bool from_h = map[y] < 0 || (!y_fluid && x_inside && (unsigned)map[x] < (unsigned)nf);
+        for (int c = 0; c < ncol; ++c) {
+            double* a = dense + (long)(19 * c + q) * ps;
+            if (from_h) a[y] = a[h];
+            else a[h] = a[y];
+        }
+    });
+}
+
 // message of the last failed create call (contexts keep their own)
 inline thread_local std::string g_create_error;
 

@@ -86,21 +86,60 @@ inline int tile_rows() {
     return v > 0 ? v : 32;
 }
 
+// A periodic axis folded into the neighbour map.  On one domain periodic along one axis only, the reference's
+// self-exchange (Mpi.F90:73-448) keeps every exchanged halo slot equal to the slot of its periodic image: after an
+// even step the pull copies image -> halo, after an odd step the push copies halo -> image, between the two no
+// node touches both, and each image slot has one halo slot.  A folded axis therefore maps the exchanged halo
+// nodes (x interior, one layer) to the compact index of their image, and the lattice update pulls from and pushes
+// to the image directly: the exchange launches disappear.  With both y and z periodic, a slot of an edge node has
+// three halo aliases (face, face, edge) that the face and edge exchanges leave unequal after an odd step, so
+// such domains keep their exchange.
+struct Fold {
+    bool y = false, z = false;
+};
+MFLBM_HD inline void decode(const Layout& L, int o, int& i, int& j, int& k) {
+    const int ip = o % L.px, jp = (o / L.px) % (L.ny + 8), kp = o / L.pxy;
+    i = ip - XO;
+    j = jp - YO;
+    k = kp - ZO;
+}
+// the periodic image of (i, j, k) when that node is an exchanged halo node of a folded axis
+MFLBM_HD inline bool fold_image(const Layout& L, const Fold& F, int i, int j, int k, int& io) {
+    if (i < 1 || i > L.nx) return false;
+    const bool gy = j == 0 || j == L.ny + 1, gz = k == 0 || k == L.nz + 1;
+    if (j < 0 || j > L.ny + 1 || k < 0 || k > L.nz + 1) return false;
+    if (gy && gz) {
+        if (!(F.y && F.z)) return false;
+    } else if (gz) {
+        if (!F.z) return false;
+    } else if (gy) {
+        if (!F.y) return false;
+    } else {
+        return false;
+    }
+    const int jj = gy ? (j == 0 ? L.ny : 1) : j, kk = gz ? (k == 0 ? L.nz : 1) : k;
+    io = L.at(i, jj, kk);
+    return true;
+}
+
 // Compact node set of a wall array (padded layout, walls != 0 is not fluid).
 //   open_z: the z faces carry inlet/outlet conditions (planes k = 0, 1, 2, nz-1, nz, nz+1 are addressed)
-//   kper / jper: periodic exchange along z / y (planes 0, 1, n, n+1)
+//   kper / jper: periodic exchange along z / y (planes 0, 1, n, n+1), unless the axis is folded
 //   extra_z: further z planes addressed by a boundary condition (e.g. a porous plate and its neighbours)
+//   fold: periodic axes folded into the map (Fold); a folded halo node takes its image's compact index
 struct CompactSet {
     std::vector<int> cidx;   // padded offset -> compact index or -1
     std::vector<int> cpos;   // compact index -> padded offset
     int nfl = 0;             // interior fluid nodes (the first nfl entries)
+    Fold fold;
 };
 
 inline void build_compact(const Layout& L, const TileOrder& T, const std::vector<int8_t>& wp, bool open_z, bool kper,
-                          bool jper, const std::vector<int>& extra_z, CompactSet& cs) {
+                          bool jper, const std::vector<int>& extra_z, CompactSet& cs, Fold fold = Fold{}) {
     cs.cidx.assign(L.total, -1);
     cs.cpos.clear();
     cs.cpos.reserve((size_t)L.nx * L.ny * L.nz);
+    cs.fold = fold;
     T.visit(1, L.nx, 1, L.ny, 1, L.nz, [&](int i, int j, int k) {
         const int o = L.at(i, j, k);
         if (wp[o] == 0) {
@@ -111,7 +150,14 @@ inline void build_compact(const Layout& L, const TileOrder& T, const std::vector
     cs.nfl = (int)cs.cpos.size();
     std::vector<uint8_t> need(L.total, 0);
     for (int m = 0; m < cs.nfl; ++m)
-        for (int q = 1; q < 19; ++q) need[cs.cpos[m] + L.off(q)] = 1;
+        for (int q = 1; q < 19; ++q) {
+            int o = cs.cpos[m] + L.off(q), i, j, k, io;
+            decode(L, o, i, j, k);
+            if (fold_image(L, fold, i, j, k, io)) o = io;
+            need[o] = 1;
+        }
+    if (fold.z) kper = false;
+    if (fold.y) jper = false;
     auto plane_z = [&](int k) {
         for (int j = 0; j <= L.ny + 1; ++j)
             for (int i = 0; i <= L.nx + 1; ++i) need[L.at(i, j, k)] = 1;
@@ -137,6 +183,13 @@ inline void build_compact(const Layout& L, const TileOrder& T, const std::vector
                     cs.cpos.push_back(o);
                 }
             }
+    if (fold.y || fold.z)
+        for (int k = 0; k <= L.nz + 1; ++k)
+            for (int j = 0; j <= L.ny + 1; ++j)
+                for (int i = 1; i <= L.nx; ++i) {
+                    int io;
+                    if (fold_image(L, fold, i, j, k, io)) cs.cidx[L.at(i, j, k)] = cs.cidx[io];
+                }
 }
 
 // Link slots.  In the AA update the slot (x + e_r, opp(r)) of a non-fluid neighbour of fluid node x is read and

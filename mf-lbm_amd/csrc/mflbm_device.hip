@@ -409,6 +409,20 @@ __global__ void k_links(Dev d, double* __restrict__ dense, long ps) {
     }
 }
 
+// Upload of a folded domain: the exchanged slot (q, y) of each image takes the value of its halo slot (q, h).  A
+// state is uploaded as the state before an odd step: that step pulls (q, h), and the push exchange after it leaves
+// (q, y) equal to (q, h) (every reference run starts at step 1; a checkpoint written after either exchange already
+// holds equal values).  The dense staging keeps the state as given.
+template <class T>
+__global__ void k_fold_upload(Dev d, Fold F, const double* __restrict__ dense, long ps) {
+    fold_exchanged(d.L, F, (long)blockIdx.x * BLOCK + threadIdx.x, [&](int h, int y, int q) {
+        if (d.map[y] < 0) return;
+        const Pop<T> f = fpop<T>(d), g = gpop<T>(d);
+        f(q, y) = dense[(long)q * ps + h];
+        g(q, y) = dense[(long)(19 + q) * ps + h];
+    });
+}
+
 // ------------------------------------------------------------------------------------------------------
 // periodic wrap of populations and phi (one rank of the reference's Cartesian MPI exchange, Mpi.F90)
 // ------------------------------------------------------------------------------------------------------
@@ -1111,25 +1125,65 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     N.z[n] = sz;
     if (bits_set(sx) | bits_set(sy) | bits_set(sz)) d.nzb[b] = 1;
 }
+// Brick flag kernels.  A work-group owns 8 x 8 x 4 bricks, stages them with a halo of one brick in LDS and
+// combines each brick's 27-brick neighbourhood from there.
+constexpr int TBX = 8, TBY = 8, TBZ = 4;                     // TBX * TBY * TBZ == BLOCK
+constexpr int TSX = TBX + 2, TSY = TBY + 2, TSZ = TBZ + 2;   // staged bricks with the halo
+static_assert(TBX * TBY * TBZ == BLOCK, "one brick per thread");
+// stage value of every brick of the tile and its halo (outside the grid: `outside`), then the index of this
+// thread's brick in the tile (its 27 neighbours are t + dx + TSX dy + TSX TSY dz, dx, dy, dz in 0..2), or -1
+template <class F>
+__device__ __forceinline__ int stage_bricks(const Dev& d, unsigned char* sh, unsigned char outside, const F& value,
+                                            int& b) {
+    const int bx0 = blockIdx.x * TBX, by0 = blockIdx.y * TBY, bz0 = blockIdx.z * TBZ;
+    for (int t = threadIdx.x; t < TSX * TSY * TSZ; t += BLOCK) {
+        const int x = bx0 + t % TSX - 1, y = by0 + (t / TSX) % TSY - 1, z = bz0 + t / (TSX * TSY) - 1;
+        sh[t] = (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz)
+                    ? value(x + d.nbx * (y + d.nby * z)) : outside;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % TBX, ty = (threadIdx.x / TBX) % TBY, tz = threadIdx.x / (TBX * TBY);
+    const int x = bx0 + tx, y = by0 + ty, z = bz0 + tz;
+    if (x >= d.nbx || y >= d.nby || z >= d.nbz) return -1;
+    b = x + d.nbx * (y + d.nby * z);
+    return tx + TSX * (ty + TSY * tz);
+}
+__device__ __forceinline__ unsigned char or27(const unsigned char* sh, int t) {
+    unsigned char a = 0;
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) a |= sh[t + dx + TSX * (dy + TSY * dz)];
+    return a;
+}
+__device__ __forceinline__ unsigned char and27(const unsigned char* sh, int t) {
+    unsigned char a = 1;
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) a &= sh[t + dx + TSX * (dy + TSY * dz)];
+    return a;
+}
 // Brick flags of one chain (see Dev): q1b from the band flags of the step (no brick quiet when `full`), zfill
 // from nzb, nzb cleared (the chain rewrites or keeps +0 every normal), the other band buffers cleared for the
 // next step.
-__global__ void k_brick_quiet(Dev d, const unsigned char* __restrict__ notm, const unsigned char* __restrict__ notp,
-                              unsigned char* __restrict__ next_m, unsigned char* __restrict__ next_p, int full) {
-    const int b = blockIdx.x * BLOCK + threadIdx.x;
-    const int nb = d.nbx * d.nby * d.nbz;
-    if (b >= nb) return;
-    const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
-    bool m = full != 0, p = full != 0;
-    for (int z = bz - 1; z <= bz + 1; ++z)
-        for (int y = by - 1; y <= by + 1; ++y)
-            for (int x = bx - 1; x <= bx + 1; ++x)
-                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) {
-                    const int c = x + d.nbx * (y + d.nby * z);
-                    const bool st = d.bstatic[c] != 0;
-                    m = m || st || notm[c];
-                    p = p || st || notp[c];
-                }
+__global__ void __launch_bounds__(BLOCK) k_brick_quiet(Dev d, const unsigned char* __restrict__ notm,
+                                                       const unsigned char* __restrict__ notp,
+                                                       unsigned char* __restrict__ next_m,
+                                                       unsigned char* __restrict__ next_p, int full) {
+    __shared__ unsigned char sh[TSX * TSY * TSZ];
+    int b = 0;
+    const int t = stage_bricks(d, sh, 0, [&](int c) {
+        const bool st = d.bstatic[c] != 0;
+        return (unsigned char)((st || notm[c] ? 1 : 0) | (st || notp[c] ? 2 : 0));
+    }, b);
+    if (t < 0) return;
+    const unsigned char a = or27(sh, t);
+    const bool m = full != 0 || (a & 1), p = full != 0 || (a & 2);
     d.q1b[b] = (!m || !p) ? 1 : 0;
     d.zfill[b] = d.nzb[b];
     d.nzb[b] = 0;
@@ -1137,32 +1191,20 @@ __global__ void k_brick_quiet(Dev d, const unsigned char* __restrict__ notm, con
     next_p[b] = 0;
 }
 // q2b = q1b of the brick and all 26 neighbours
-__global__ void k_brick_quiet2(Dev d) {
-    const int b = blockIdx.x * BLOCK + threadIdx.x;
-    const int nb = d.nbx * d.nby * d.nbz;
-    if (b >= nb) return;
-    const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
-    bool q = true;
-    for (int z = bz - 1; z <= bz + 1 && q; ++z)
-        for (int y = by - 1; y <= by + 1 && q; ++y)
-            for (int x = bx - 1; x <= bx + 1 && q; ++x)
-                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz && !d.q1b[x + d.nbx * (y + d.nby * z)])
-                    q = false;
-    d.q2b[b] = q ? 1 : 0;
+__global__ void __launch_bounds__(BLOCK) k_brick_quiet2(Dev d) {
+    __shared__ unsigned char sh[TSX * TSY * TSZ];
+    int b = 0;
+    const int t = stage_bricks(d, sh, 1, [&](int c) { return d.q1b[c]; }, b);
+    if (t < 0) return;
+    d.q2b[b] = and27(sh, t);
 }
 // nzd = nzb dilated by one brick, after the chain has written the normal field
-__global__ void k_brick_normals(Dev d) {
-    const int b = blockIdx.x * BLOCK + threadIdx.x;
-    const int nb = d.nbx * d.nby * d.nbz;
-    if (b >= nb) return;
-    const int bx = b % d.nbx, by = (b / d.nbx) % d.nby, bz = b / (d.nbx * d.nby);
-    bool a = false;
-    for (int z = bz - 1; z <= bz + 1 && !a; ++z)
-        for (int y = by - 1; y <= by + 1 && !a; ++y)
-            for (int x = bx - 1; x <= bx + 1 && !a; ++x)
-                if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz && d.nzb[x + d.nbx * (y + d.nby * z)])
-                    a = true;
-    d.nzd[b] = a ? 1 : 0;
+__global__ void __launch_bounds__(BLOCK) k_brick_normals(Dev d) {
+    __shared__ unsigned char sh[TSX * TSY * TSZ];
+    int b = 0;
+    const int t = stage_bricks(d, sh, 0, [&](int c) { return d.nzb[c]; }, b);
+    if (t < 0) return;
+    d.nzd[b] = or27(sh, t) ? 1 : 0;
 }
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
@@ -1450,30 +1492,38 @@ __global__ void __launch_bounds__(BLOCK) k_monitor_layers(Dev d, const double* u
             dphi = fmax(dphi, fabs(ph - phi_old[n]) * om);
         }
     }
-    __shared__ double sh[BLOCK];
-    double out[NPART + 3];
-    for (int m = 0; m < NPART + 3; ++m) {
-        double x = m < NPART ? s[m] : (m == NPART ? umax : (m == NPART + 1 ? dphi : 0.0));
-        if (m == NPART + 2) {   // count of fluid-1 nodes (phi > 0.99) for the capillary-pressure monitor
-            double c = 0.0;
-            for (int r = threadIdx.x; r < nxy; r += BLOCK) {
-                const int n = L.at(r % L.nx + 1, r / L.nx + 1, k);
-                if (d.walls[n] == 0 && d.phi[n] > 0.99) c += 1.0;
-            }
-            x = c;
-        }
-        sh[threadIdx.x] = x;
-        __syncthreads();
-        for (int off = BLOCK / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) {
-                const double o = sh[threadIdx.x + off];
-                sh[threadIdx.x] = (m == NPART || m == NPART + 1) ? fmax(sh[threadIdx.x], o) : sh[threadIdx.x] + o;
-            }
-            __syncthreads();
-        }
-        out[m] = sh[0];
-        __syncthreads();
+    // count of fluid-1 nodes (phi > 0.99) for the capillary-pressure monitor
+    double c1 = 0.0;
+    for (int r = threadIdx.x; r < nxy; r += BLOCK) {
+        const int n = L.at(r % L.nx + 1, r / L.nx + 1, k);
+        if (d.walls[n] == 0 && d.phi[n] > 0.99) c1 += 1.0;
     }
+    // wavefront reductions (a fixed butterfly over the 64 lanes), then the four waves' results in wave order
+    constexpr int NQ = NPART + 3, NW = BLOCK / 64;
+    double q15[NQ];
+    for (int m = 0; m < NPART; ++m) q15[m] = s[m];
+    q15[NPART] = umax;
+    q15[NPART + 1] = dphi;
+    q15[NPART + 2] = c1;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(q15[m], off, 64);
+            q15[m] = (m == NPART || m == NPART + 1) ? fmax(q15[m], o) : q15[m] + o;
+        }
+    __shared__ double sh[NQ][NW];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int m = 0; m < NQ; ++m) sh[m][wave] = q15[m];
+    __syncthreads();
+    double out[NQ];
+    if (threadIdx.x == 0)
+        for (int m = 0; m < NQ; ++m) {
+            double a = sh[m][0];
+            for (int w = 1; w < NW; ++w) a = (m == NPART || m == NPART + 1) ? fmax(a, sh[m][w]) : a + sh[m][w];
+            out[m] = a;
+        }
     if (threadIdx.x == 0) {
         for (int m = 0; m < 7; ++m) layers[m * L.nz + (k - 1)] = out[m];
         for (int m = 7; m < NPART + 3; ++m) part[(m - 7) * L.nz + (k - 1)] = out[m];
@@ -1572,6 +1622,8 @@ struct mflbm_ctx {
     int band_cur = 0;                // band buffers the lattice update fills until the next chain
     bool full_chain = true;          // the next chain evaluates every brick (new geometry or state)
     bool solid_zeroed = false;       // the macroscopic pass zeroed phi in solids since the last chain
+    Fold fold;                       // periodic axes folded into the neighbour map
+    bool stepped = false;            // a step ran since the state was uploaded or built
     bool chain_fused = false;   // one fused pass for normals, extrapolation and curvature (MFLBM_CHAIN=fused)
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
@@ -1611,13 +1663,30 @@ CollideParams collide_params(const mflbm_params& p) {
 }
 
 bool open_z(const mflbm_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
+// The periodic axis folded into the neighbour map (layout.h): the one periodic axis of a domain periodic along z
+// only, or along y only without an inlet/outlet face (which addresses the y halo rows); never with a porous
+// plate, whose planes address the halo rows
+Fold fold_of(const mflbm_params& p) {
+    const bool plate = p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2;
+    return Fold{p.jper == 1 && p.kper == 0 && !open_z(p) && !plate, p.kper == 1 && p.jper == 0 && !plate};
+}
+
+// brick flag kernels: one work-group per TBX x TBY x TBZ bricks
+#define BRICK_LAUNCH(c, kern, ...)                                                                           \
+    do {                                                                                                     \
+        const dim3 g_((unsigned)(((c)->d.nbx + TBX - 1) / TBX), (unsigned)(((c)->d.nby + TBY - 1) / TBY),       \
+                      (unsigned)(((c)->d.nbz + TBZ - 1) / TBZ));                                             \
+        hipLaunchKernelGGL(kern, g_, dim3(BLOCK), 0, (c)->stream, __VA_ARGS__);                              \
+        const hipError_t e_ = hipGetLastError();                                                             \
+        if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(e_));  \
+    } while (0)
 
 // dense normal field by the list kernels (the chain of the reference, one pass per stage)
 template <class NT>
 int launch_normals_dense(mflbm_ctx* c) {
     LAUNCH(c, k_normals<NT>, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
     LAUNCH(c, k_normal_extrapolate<NT>, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
-    LAUNCH(c, k_brick_normals, nblocks(c->nbricks), c->d);
+    BRICK_LAUNCH(c, k_brick_normals, c->d);
     return MFLBM_OK;
 }
 template <class NT>
@@ -1631,9 +1700,8 @@ int launch_gradient_t(mflbm_ctx* c) {
         unsigned char* m_next = c->qbuf + (size_t)(c->band_cur ^ 1) * nb;
         unsigned char* p_next = c->qbuf + (size_t)(2 + (c->band_cur ^ 1)) * nb;
         const int full = (c->full_chain || c->chain_fused) ? 1 : 0;
-        LAUNCH(c, k_brick_quiet, nblocks(nb), c->d, (const unsigned char*)m_cur, (const unsigned char*)p_cur, m_next,
-               p_next, full);
-        LAUNCH(c, k_brick_quiet2, nblocks(nb), c->d);
+        BRICK_LAUNCH(c, k_brick_quiet, c->d, (const unsigned char*)m_cur, (const unsigned char*)p_cur, m_next, p_next, full);
+        BRICK_LAUNCH(c, k_brick_quiet2, c->d);
         c->band_cur ^= 1;
         c->d.notm = m_next;
         c->d.notp = p_next;
@@ -1662,6 +1730,7 @@ template <class T>
 int launch_faces(mflbm_ctx* c, bool odd);
 
 int launch_step(mflbm_ctx* c, int ntime) {
+    c->stepped = true;
     const mflbm_params& p = c->p;
     const CollideParams P = collide_params(p);
     const bool odd = (ntime % 2) != 0;
@@ -1719,14 +1788,17 @@ int launch_faces(mflbm_ctx* c, bool odd) {
     const Layout& L = c->L;
     // periodic wrap (the reference's MPI exchange of one rank with itself)
     if (p.kper == 1 || p.jper == 1) {
+        // population exchange of the axes not folded into the neighbour map
+        const Fold F = fold_of(p);
+        const bool wz = p.kper == 1 && !F.z, wy = p.jper == 1 && !F.y, we = p.kper == 1 && p.jper == 1 && !(F.y && F.z);
         if (!odd) {
-            if (p.kper == 1) LAUNCH(c, k_wrap_pull_z<T>, nblocks((long)L.nx * L.ny), c->d);
-            if (p.jper == 1) LAUNCH(c, k_wrap_pull_y<T>, nblocks((long)L.nx * L.nz), c->d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_pull_edge<T>, nblocks(L.nx), c->d);
+            if (wz) LAUNCH(c, k_wrap_pull_z<T>, nblocks((long)L.nx * L.ny), c->d);
+            if (wy) LAUNCH(c, k_wrap_pull_y<T>, nblocks((long)L.nx * L.nz), c->d);
+            if (we) LAUNCH(c, k_wrap_pull_edge<T>, nblocks(L.nx), c->d);
         } else {
-            if (p.kper == 1) LAUNCH(c, k_wrap_push_z<T>, nblocks((long)L.nx * L.ny), c->d);
-            if (p.jper == 1) LAUNCH(c, k_wrap_push_y<T>, nblocks((long)L.nx * L.nz), c->d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_wrap_push_edge<T>, nblocks(L.nx), c->d);
+            if (wz) LAUNCH(c, k_wrap_push_z<T>, nblocks((long)L.nx * L.ny), c->d);
+            if (wy) LAUNCH(c, k_wrap_push_y<T>, nblocks((long)L.nx * L.nz), c->d);
+            if (we) LAUNCH(c, k_wrap_push_edge<T>, nblocks(L.nx), c->d);
         }
         LAUNCH(c, k_wrap_phi, nblocks((long)L.nx * (L.ny + 8) * (L.nz + 8)), c->d, p.jper, p.kper);
     }
@@ -1768,6 +1840,9 @@ template <class T>
 int gather_state(mflbm_ctx* c) {
     LAUNCH(c, k_gather<T>, nblocks(c->d.nc), c->d, (const double*)c->dense_pdf, c->ps);
     LAUNCH(c, (k_links<false, T>), nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
+    if (c->fold.y || c->fold.z)
+        LAUNCH(c, k_fold_upload<T>, fold_blocks(c->L), c->d, c->fold, (const double*)c->dense_pdf, c->ps);
+    c->stepped = false;
     return MFLBM_OK;
 }
 template <class T>
@@ -1942,7 +2017,8 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
         extra_z = {p.z_porous_plate - 1, p.z_porous_plate, p.z_porous_plate + 1};
     const TileOrder T{tile_rows(), L};
     CompactSet cset;
-    build_compact(L, T, wp, open_z(p), p.kper == 1, p.jper == 1, extra_z, cset);
+    build_compact(L, T, wp, open_z(p), p.kper == 1, p.jper == 1, extra_z, cset, fold_of(p));
+    c->fold = cset.fold;
     const std::vector<int>& cidx = cset.cidx;
     const std::vector<int>& cpos = cset.cpos;
     const int nfl = cset.nfl;
@@ -2205,9 +2281,12 @@ extern "C" int mflbm_download_state(mflbm_ctx* c, double* const* f, double* cons
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
     const Layout& L = c->L;
-    {
+    const bool folded = c->fold.y || c->fold.z;
+    if (!folded || c->stepped) {   // (before the first step of a folded domain the staging holds the state as given)
         const int rs = c->fp32 ? scatter_state<float>(c) : scatter_state<double>(c);
         if (rs) return rs;
+        if (folded)
+            LAUNCH(c, k_fold_dense, fold_blocks(L), L, c->fold, (const int*)c->map, c->d.nf, c->dense_pdf, c->ps, 2);
     }
     if (phi) {
         const int rr = refresh_solid_phi(c);

@@ -46,14 +46,14 @@ __global__ void __launch_bounds__(NT) k_spb_collide(const SDev<T>* __restrict__ 
     sp_update_node<ODD, T, C>(d, P, m);
 }
 
-// periodic self-exchange of every active sample; KIND 0..5 = pull z, pull y, pull edge, push z, push y, push
-// edge (Mpi.F90:73-448); y kinds only for samples with y periodic (flag bit 0)
+// periodic self-exchange of every active sample periodic along y as well as z (the samples periodic along z only
+// have it folded into their neighbour map, batch_plan.h); KIND 0..5 = pull z, pull y, pull edge, push z, push y,
+// push edge (Mpi.F90:73-448)
 template <class T, int KIND>
 __global__ void k_spb_wrap(const SDev<T>* __restrict__ devs, const int* __restrict__ active,
                            const int* __restrict__ yper) {
     const int s = blockIdx.y;
-    if (!active[s]) return;
-    if ((KIND == 1 || KIND == 2 || KIND == 4 || KIND == 5) && !yper[s]) return;
+    if (!active[s] || !yper[s]) return;
     const SDev<T>& d = devs[s];
     const int t = blockIdx.x * BLOCK + threadIdx.x;
     if (KIND == 0) sp_pull_z(d, t);
@@ -184,7 +184,8 @@ struct mflbm_spb_ctx {
     int* wave_first = nullptr;
     int* nwave = nullptr;
     int* nzs = nullptr;
-    int* yper = nullptr;
+    int* yper = nullptr;                 // per sample: periodic along y too (exchange kernels, no fold)
+    bool any_yper = false;
     double* wave_sum = nullptr;
     Monitor M{};
     int* mon_int = nullptr;              // active, conv_step, period, nactive
@@ -260,7 +261,7 @@ int step_t(mflbm_spb_ctx* c, int ntime) {
         case 512: rc = launch_collide<T, 512>(c, odd); break;
         default: rc = launch_collide<T, 256>(c, odd); break;
     }
-    if (rc) return rc;
+    if (rc || !c->any_yper) return rc;
     long fz = 0, fy = 0, fe = 0;
     for (const Layout& L : c->L) {
         fz = std::max(fz, (long)L.nx * L.ny);
@@ -500,6 +501,7 @@ extern "C" int mflbm_spb_commit(mflbm_spb_ctx* c) {
     for (int s = 0; s < n; ++s) {
         nzs[s] = c->host[s].nz;
         yp[s] = c->host[s].jper;
+        c->any_yper = c->any_yper || yp[s] == 1;
     }
     HIPCHK(c, hipMalloc(&c->wave_sample, sizeof(int) * ws.size()));
     HIPCHK(c, hipMalloc(&c->wave_first, sizeof(int) * n));
@@ -562,6 +564,8 @@ int gather_sample(mflbm_spb_ctx* c, int s) {
     const long ps = c->L[s].total;
     LAUNCH(c, k_sp_gather<T>, nblocks(d.nc), d, (const double*)c->staging, ps);
     LAUNCH(c, (k_sp_links<false, T>), nblocks(d.nf), d, c->staging, ps);
+    const Fold F = batch_fold(c->host[s].jper);
+    if (F.y || F.z) LAUNCH(c, k_sp_fold_upload<T>, fold_blocks(c->L[s]), d, F, (const double*)c->staging, ps);
     return MFLBM_OK;
 }
 template <class T>
@@ -570,6 +574,8 @@ int scatter_sample(mflbm_spb_ctx* c, int s) {
     const long ps = c->L[s].total;
     LAUNCH(c, k_sp_scatter<T>, nblocks(d.nc), d, c->staging, ps);
     LAUNCH(c, (k_sp_links<true, T>), nblocks(d.nf), d, c->staging, ps);
+    const Fold F = batch_fold(c->host[s].jper);
+    if (F.y || F.z) LAUNCH(c, k_fold_dense, fold_blocks(c->L[s]), c->L[s], F, d.map, d.nf, c->staging, ps, 1);
     return MFLBM_OK;
 }
 }  // namespace

@@ -250,6 +250,11 @@ __global__ void __launch_bounds__(BLOCK) k_sp_monitor_layers(Layout L, const dou
 }
 
 bool sp_open_z(const mflbm_sp_params& p) { return p.kper == 0 && p.wall_z_min == 0 && p.wall_z_max == 0; }
+// the periodic axis folded into the neighbour map (layout.h): the one periodic axis of a domain periodic along z
+// only, or along y only without an inlet/outlet face (which addresses the y halo rows)
+Fold sp_fold_of(const mflbm_sp_params& p) {
+    return Fold{p.jper == 1 && p.kper == 0 && !sp_open_z(p), p.kper == 1 && p.jper == 0};
+}
 
 }  // namespace
 
@@ -259,6 +264,7 @@ struct mflbm_sp_ctx {
     SPCollideParamsT<float> cp32{};
     bool fp32 = false;           // precision 1 or 2: FP32 deviation storage
     bool mixed = false;          // precision 2: the FP32 deviations collided in FP64
+    bool stepped = false;        // a step ran since the state was uploaded or built
     Layout L{};
     SDev<double> d{};            // slice pointers of the FP64 mode; the other fields are shared by both views
     SDev<float> d32{};
@@ -314,14 +320,17 @@ int sp_launch_step_t(mflbm_sp_ctx* c, int ntime, hipEvent_t* ev) {
     }
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     if (p.kper == 1 || p.jper == 1) {
+        // exchange of the periodic axes not folded into the neighbour map
+        const Fold F = sp_fold_of(p);
+        const bool wz = p.kper == 1 && !F.z, wy = p.jper == 1 && !F.y, we = p.kper == 1 && p.jper == 1 && !(F.y && F.z);
         if (!odd) {
-            if (p.kper == 1) LAUNCH(c, k_sp_pull_z<T>, nblocks((long)L.nx * L.ny), d);
-            if (p.jper == 1) LAUNCH(c, k_sp_pull_y<T>, nblocks((long)L.nx * L.nz), d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_pull_edge<T>, nblocks(L.nx), d);
+            if (wz) LAUNCH(c, k_sp_pull_z<T>, nblocks((long)L.nx * L.ny), d);
+            if (wy) LAUNCH(c, k_sp_pull_y<T>, nblocks((long)L.nx * L.nz), d);
+            if (we) LAUNCH(c, k_sp_pull_edge<T>, nblocks(L.nx), d);
         } else {
-            if (p.kper == 1) LAUNCH(c, k_sp_push_z<T>, nblocks((long)L.nx * L.ny), d);
-            if (p.jper == 1) LAUNCH(c, k_sp_push_y<T>, nblocks((long)L.nx * L.nz), d);
-            if (p.kper == 1 && p.jper == 1) LAUNCH(c, k_sp_push_edge<T>, nblocks(L.nx), d);
+            if (wz) LAUNCH(c, k_sp_push_z<T>, nblocks((long)L.nx * L.ny), d);
+            if (wy) LAUNCH(c, k_sp_push_y<T>, nblocks((long)L.nx * L.nz), d);
+            if (we) LAUNCH(c, k_sp_push_edge<T>, nblocks(L.nx), d);
         }
     }
     if (sp_open_z(p)) {
@@ -342,6 +351,7 @@ int sp_launch_step_t(mflbm_sp_ctx* c, int ntime, hipEvent_t* ev) {
 }
 
 int sp_launch_step(mflbm_sp_ctx* c, int ntime) {
+    c->stepped = true;
     hipEvent_t* ev = nullptr;
     if (c->timing) {
         if (c->ev_used + 3 > c->ev.size()) {
@@ -366,13 +376,21 @@ int sp_gather_t(mflbm_sp_ctx* c) {
     const SDev<T>& d = view<T>(c);
     LAUNCH(c, k_sp_gather<T>, nblocks(d.nc), d, (const double*)c->dense_pdf, c->ps);
     LAUNCH(c, (k_sp_links<false, T>), nblocks(d.nf), d, c->dense_pdf, c->ps);
+    const Fold F = sp_fold_of(c->p);
+    if (F.y || F.z) LAUNCH(c, k_sp_fold_upload<T>, fold_blocks(c->L), d, F, (const double*)c->dense_pdf, c->ps);
+    c->stepped = false;
     return MFLBM_OK;
 }
+// download into the dense staging (before the first step of a folded domain the staging holds the state as given)
 template <class T>
 int sp_scatter_t(mflbm_sp_ctx* c) {
     const SDev<T>& d = view<T>(c);
+    const Fold F = sp_fold_of(c->p);
+    const bool folded = F.y || F.z;
+    if (folded && !c->stepped) return MFLBM_OK;
     LAUNCH(c, k_sp_scatter<T>, nblocks(d.nc), d, c->dense_pdf, c->ps);
     LAUNCH(c, (k_sp_links<true, T>), nblocks(d.nf), d, c->dense_pdf, c->ps);
+    if (folded) LAUNCH(c, k_fold_dense, fold_blocks(c->L), c->L, F, d.map, d.nf, c->dense_pdf, c->ps, 1);
     return MFLBM_OK;
 }
 template <class T>
@@ -510,7 +528,7 @@ extern "C" int mflbm_sp_upload_geometry(mflbm_sp_ctx* c, const int8_t* walls_h2,
     pack_host(L, 2, walls_h2, wp.data());
     const TileOrder T{tile_rows(), L};
     CompactSet cset;
-    build_compact(L, T, wp, sp_open_z(p), p.kper == 1, p.jper == 1, {}, cset);
+    build_compact(L, T, wp, sp_open_z(p), p.kper == 1, p.jper == 1, {}, cset, sp_fold_of(p));
     const int nc = (int)cset.cpos.size();
     LinkSet links;
     build_links(L, cset, links);

@@ -166,6 +166,17 @@ __global__ void k_sp_links(SDev<T> d, double* __restrict__ dense, long ps) {
     }
 }
 
+// Upload of a folded domain (layout.h, Fold): the exchanged slot (q, y) of each image takes the value of its halo
+// slot (q, h), as the push exchange after the first (odd) step leaves it; the dense staging keeps the state as
+// given.  (A checkpoint written after either exchange already holds equal values.)
+template <class T>
+__global__ void k_sp_fold_upload(SDev<T> d, Fold F, const double* __restrict__ dense, long ps) {
+    fold_exchanged(d.L, F, (long)blockIdx.x * BLOCK + threadIdx.x, [&](int h, int y, int q) {
+        if (d.map[y] < 0) return;
+        spop(d)(q, y) = dense[(long)q * ps + h];
+    });
+}
+
 // ------------------------------------------------------------------------------------------------------
 // periodic wrap: the reference's MPI exchange of one rank with itself (Mpi.F90:73-448)
 // ------------------------------------------------------------------------------------------------------

@@ -4,7 +4,9 @@
 // compute from the plan and checks it against the buffer it addresses:
 //   * the lattice update: a wave's sample from the wave table, its fluid node, the slot of each direction in
 //     the own node (even step) and in the neighbour or the node's link slot (odd step, AA pull/push);
-//   * the periodic exchange planes z = 0, 1, nz, nz + 1 (and y where periodic): every addressed node is stored;
+//   * the folded periodic halo of the samples periodic along z only: every exchanged halo node (z = 0, nz + 1)
+//     holds its periodic image's compact index, and the image lies in the grid; the periodic exchange planes
+//     z = 0, 1, nz, nz + 1 of the other samples: every addressed node is stored;
 //   * the monitor tables (flux per wave, per-sample reduction ranges).
 // Input: int32 n, then per sample int32 nx, ny, nz, jper and (nx + 4)(ny + 4)(nz + 4) int8 walls with halo 2.
 // Exit status 0 and "ok" on success; a message and status 1 on the first violation.
@@ -42,7 +44,7 @@ static long slot_of(const BatchPlan& P, int s, int q, int o) {
     return c;
 }
 
-static int check(const BatchPlan& P, long esz) {
+static int check(const BatchPlan& P, long esz, const std::vector<BatchSampleIn>& in) {
     const int n = (int)P.L.size();
     if ((int)P.wave_sample.size() < std::max(P.nwaves, 1)) return bad("wave table size", -1, (long)P.wave_sample.size(), P.nwaves);
     long elems = 0;
@@ -85,7 +87,7 @@ static int check(const BatchPlan& P, long esz) {
                 if (idx < 0 || idx >= so[q + 1] - so[q]) return bad("odd-step slot outside its slice", s, idx, so[q + 1] - so[q]);
             }
         }
-        // periodic exchange planes
+        // periodic exchange planes of the samples that keep their exchange (periodic along y and z)
         auto plane_ok = [&](int i, int j, int k) -> bool {
             const int o = L.at(i, j, k);
             if (o < 0 || o >= L.total) return false;
@@ -95,10 +97,22 @@ static int check(const BatchPlan& P, long esz) {
             }
             return true;
         };
-        for (int k : {0, 1, L.nz, L.nz + 1})
-            for (int j = 1; j <= L.ny; ++j)
-                for (int i = 1; i <= L.nx; ++i)
-                    if (!plane_ok(i, j, k)) return bad("z exchange plane node not stored", s, i + 1000L * j, k);
+        if (!cs.fold.z)
+            for (int k : {0, 1, L.nz, L.nz + 1})
+                for (int j = 1; j <= L.ny; ++j)
+                    for (int i = 1; i <= L.nx; ++i)
+                        if (!plane_ok(i, j, k)) return bad("z exchange plane node not stored", s, i + 1000L * j, k);
+        // folded periodic halo: each exchanged halo node reads its image's compact index
+        if (cs.fold.y || cs.fold.z != (in.at(s).jper != 1)) return bad("fold of the periodic axes", s, cs.fold.z, cs.fold.y);
+        for (int k = 0; k <= L.nz + 1; ++k)
+            for (int j = 0; j <= L.ny + 1; ++j)
+                for (int i = 1; i <= L.nx; ++i) {
+                    int io;
+                    if (!fold_image(L, cs.fold, i, j, k, io)) continue;
+                    if (io < 0 || io >= L.total) return bad("fold image outside the grid", s, io, L.total);
+                    if (cs.cidx.at(L.at(i, j, k)) != cs.cidx.at(io))
+                        return bad("folded halo node without its image's index", s, i + 1000L * j, k);
+                }
         if (ls.wmask.size() < 18 * (size_t)((cs.nfl + 63) / 64)) return bad("link mask table", s, (long)ls.wmask.size(), cs.nfl);
     }
     if (elems != P.elems) return bad("total elements", -1, elems, P.elems);
@@ -133,7 +147,7 @@ int main(int argc, char** argv) {
                 std::printf("plan refused: %s\n", why.c_str());
                 return 1;
             }
-            if (check(P, esz)) {
+            if (check(P, esz, in)) {
                 std::printf("(tile %d, %ld-byte storage)\n", tile, esz);
                 return 1;
             }

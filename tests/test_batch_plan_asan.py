@@ -1,7 +1,7 @@
 """The batched solver's host-side layout (csrc/batch_plan.h) under AddressSanitizer and UndefinedBehaviorSanitizer
 on the CPU, for the sample sets the batch workload and the tests use: tests/native/batch_plan_check.cpp plans
 the set for every tile height the autotuner tries and replays every index the batched kernels derive from
-the plan (wave tables, AA pull/push slots including link slots, periodic exchange planes) against the buffers
+the plan (wave tables, AA pull/push slots including link slots, the folded periodic halo, the exchange planes) against the buffers
 they address."""
 import os
 import shutil
