@@ -64,7 +64,7 @@ struct Dev {
     float* cnz32;
     float* cnorm32;
     double* curv;     // dense curvature, filled only for diagnostics
-    // Brick flags of the list chain.  The padded grid is cut into bricks of 4 x 4 x 4 nodes (brick_of).  Away
+    // Brick flags of the list chain.  The padded grid is cut into bricks of 64 nodes (brick_of).  Away
     // from the interface phi is +-1 to within rounding, and there the ISO4 gradient is below the 1e-6 at which
     // the reference zeroes the normal (Phase_gradient.F90:63-79): if every phi a node's normal depends on (its
     // stencil, and the fluid neighbours of the solid nodes in it) lies within PHI_BAND of the same +-1, the
@@ -155,14 +155,24 @@ template <class T>
 __device__ __forceinline__ Pop<T> gpop(const Dev& d) { return Pop<T>{&d, 19}; }
 __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i + j * (d.L.nx + 2); }
 
-// brick of padded offset n (4 x 4 x 4 padded nodes per brick)
-constexpr int BRICK_SHIFT = 2;
+// brick of padded offset n: 2^BRICK_SX x 2^BRICK_SY x 2^BRICK_SZ padded nodes (16 x 2 x 2: a brick's rows are
+// whole 128-byte lines of a dense FP64 field, so the chain stages that take one brick per wave touch few lines
+// per access).  The brick flags need every brick at least 2 nodes wide in each axis (a normal depends on phi
+// within 2 nodes: its stencil and the fluid neighbours of the stencil's solid nodes).
+#ifndef MFLBM_BRICK_SX
+#define MFLBM_BRICK_SX 4
+#define MFLBM_BRICK_SY 1
+#define MFLBM_BRICK_SZ 1
+#endif
+constexpr int BRICK_SX = MFLBM_BRICK_SX, BRICK_SY = MFLBM_BRICK_SY, BRICK_SZ = MFLBM_BRICK_SZ;
+static_assert(BRICK_SX >= 1 && BRICK_SY >= 1 && BRICK_SZ >= 1 && BRICK_SX + BRICK_SY + BRICK_SZ == 6,
+              "64-node bricks at least 2 nodes wide");
 __device__ __forceinline__ int brick_of(const Dev& d, int n) {
     const int kp = n / d.L.pxy;
     const int r = n - kp * d.L.pxy;
     const int jp = r / d.L.px;
     const int ip = r - jp * d.L.px;
-    return (ip >> BRICK_SHIFT) + d.nbx * ((jp >> BRICK_SHIFT) + d.nby * (kp >> BRICK_SHIFT));
+    return (ip >> BRICK_SX) + d.nbx * ((jp >> BRICK_SY) + d.nby * (kp >> BRICK_SZ));
 }
 // Half-width of the band around +-1 within which phi counts as pure.  With every value of an ISO4 stencil within
 // eps of one constant, each difference is at most 2 eps and exact (Sterbenz), so |d/dx| <= (1/6 + 8/12) 2 eps and
@@ -838,8 +848,7 @@ struct FaceArgs {
     double phi_inlet, sa_inject, relaxation, rho_in, u_convec, rho_out;
 };
 template <class T, int IN, int OUT, bool AFTER>
-__global__ void __launch_bounds__(BLOCK) k_open_faces(Dev d, FaceArgs a, int gface) {
-    const int b = (int)blockIdx.x;
+__device__ __forceinline__ void open_faces_block(const Dev& d, const FaceArgs& a, int gface, int b) {
     if (b < gface) {
         const int t = b * BLOCK + (int)threadIdx.x;
         if (IN == 1) {
@@ -859,6 +868,10 @@ __global__ void __launch_bounds__(BLOCK) k_open_faces(Dev d, FaceArgs a, int gfa
             else bc_outlet_pressure_before<T>(d, t, a.rho_out);
         }
     }
+}
+template <class T, int IN, int OUT, bool AFTER>
+__global__ void __launch_bounds__(BLOCK) k_open_faces(Dev d, FaceArgs a, int gface) {
+    open_faces_block<T, IN, OUT, AFTER>(d, a, gface, (int)blockIdx.x);
 }
 
 // porous_plate_BC_before_odd / _after_odd (Boundary_multiphase_other.F90:8-184); `blocked` is the colour
@@ -1016,19 +1029,25 @@ struct ZeroAt {
 };
 __device__ __forceinline__ double quiet_curvature() { return curvature_f(ZeroAt{}, ZeroAt{}, ZeroAt{}); }
 
-// phi extrapolated into solid boundary nodes from their fluid neighbours (Phase_gradient.F90:12-30)
-// (all = 0: only where an evaluated normal may read the value; all = 1: every entry, before a download)
-__global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per, int all) {
-    const int t = xcd_block(per) * BLOCK + threadIdx.x;
-    if (t >= s.n) return;
+// phi extrapolated into a solid boundary node from its fluid neighbours (Phase_gradient.F90:12-30).  Solid list
+// nodes lie within [-2, n+3]^3, so all 18 neighbours are inside the padded grid: they are loaded unconditionally
+// (one round trip instead of one per direction) and the fluid ones summed in the reference's order.
+__device__ __forceinline__ void phi_extrapolate_at(const Dev& d, const SolidList& s, int t) {
     const int n = s.pos[t];
-    if (!all && d.q2b[brick_of(d, n)]) return;
     const unsigned mk = s.mask[t];
+    double v[19];
+#pragma unroll
+    for (int q = 1; q < 19; ++q) v[q] = d.phi[n + d.L.off(q)];
     double acc = 0.0;
 #pragma unroll
-    for (int q = 1; q < 19; ++q)
-        if (mk & (1u << q)) acc = acc + d.phi[n + d.L.off(q)] * lat::W[q];
+    for (int q = 1; q < 19; ++q) acc = (mk & (1u << q)) ? acc + v[q] * lat::W[q] : acc;
     d.phi[n] = acc / s.w[t];
+}
+// every entry of the list (before a download: the chain skips the bricks no evaluated normal reads)
+__global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per) {
+    const int t = xcd_block(per) * BLOCK + threadIdx.x;
+    if (t >= s.n) return;
+    phi_extrapolate_at(d, s, t);
 }
 // wetting: turn the normal at a fluid node next to the wall to the contact angle (Akai et al. 2018;
 // alter_color_gradient_solid_surface, Phase_gradient.F90:210-265).  cos/sin of the angle come from the host.
@@ -1057,154 +1076,280 @@ __device__ __forceinline__ void wet(double& cx, double& cy, double& cz, double n
         cz = cnzm;
     }
 }
+// The chain stages over the bricks a plan (k_brick_plan) listed.  Each list (normal nodes, solid nodes, solid
+// nodes within one halo layer) is sorted by brick, off[b] .. off[b + 1] being brick b's entries (at most 64, the
+// nodes of a brick).  A group of G lanes takes one listed brick at a time; each group owns a contiguous run of the
+// work list (groups in the XCD-aware block order, so a die works on neighbouring bricks), and the grid is fixed:
+// the list length is known on the device only.
+struct BrickWork {
+    const int4* list;   // bricks to visit: (brick, first entry, end entry, brick quiet)
+    const int* cnt;     // their number
+};
+#ifndef MFLBM_CHAIN_BLOCKS
+#define MFLBM_CHAIN_BLOCKS 2048
+#endif
+constexpr int CHAIN_BLOCKS = MFLBM_CHAIN_BLOCKS;   // grid of the brick-list kernels: what the GPU holds at once
+constexpr int CHAIN_PER = CHAIN_BLOCKS / NXCD;     // work-groups per XCD
+static_assert(CHAIN_BLOCKS % NXCD == 0, "whole work-groups per XCD");
+// The work items of a group of G lanes.  The list is cut into NXCD contiguous ranges, one per XCD (consecutive
+// items, appended by one plan work-group, are neighbouring bricks: they share the die's L2), and each die's groups
+// take its items in turn.  Work-groups are dispatched to the dies round-robin (blockIdx.x % NXCD).
+template <int G>
+struct GroupItems {
+    static constexpr int NG = BLOCK / G;
+    int i, hi, lane;
+    __device__ __forceinline__ GroupItems(const BrickWork& w) {
+        const int n = *w.cnt;
+        const int x = (int)(blockIdx.x % NXCD);
+        i = (int)(((long)n * x) / NXCD) + (int)(blockIdx.x / NXCD) * NG + (int)threadIdx.x / G;
+        hi = (int)(((long)n * (x + 1)) / NXCD);
+        lane = (int)threadIdx.x % G;
+    }
+    static constexpr int STRIDE = CHAIN_PER * NG;
+};
+// a normal of brick b has a bit set when any lane of the group says so: nzb of b and nzd of its 27-brick
+// neighbourhood (which the lattice update reads) are set.  All lanes of the wave call this together.
+template <int G>
+__device__ __forceinline__ void mark_nonzero(const Dev& d, int b, bool nz, int lane) {
+    const unsigned long long m = __ballot(nz);
+    const unsigned long long gm = G == 64 ? m : (m >> ((threadIdx.x & 63u) & ~(unsigned)(G - 1))) & ((1ull << (G & 63)) - 1ull);
+    if (!gm) return;
+    if (lane == 0) d.nzb[b] = 1;
+    const int bz = b / (d.nbx * d.nby), by = (b / d.nbx) % d.nby, bx = b % d.nbx;
+    for (int u = lane; u < 27; u += G) {
+        const int x = bx + u % 3 - 1, y = by + (u / 3) % 3 - 1, z = bz + u / 9 - 1;
+        if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) d.nzd[x + d.nbx * (y + d.nby * z)] = 1;
+    }
+}
+// phi extrapolation at the solid nodes of the listed bricks (those whose phi an evaluated normal may read)
+__global__ void __launch_bounds__(BLOCK) k_phi_extrapolate_b(Dev d, SolidList s, BrickWork w) {
+    GroupItems<16> g(w);
+    for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
+        const int4 it = w.list[g.i];
+        for (int t = it.y + g.lane; t < it.z; t += 16) phi_extrapolate_at(d, s, t);
+    }
+}
 // interface normal and |grad phi| at the non-solid nodes of [-1, n+2]^3 (Phase_gradient.F90:33-79), with the
 // wetting rotation applied at once where the node is a fluid boundary node (the rotation reads and writes only
 // the node's own normal, so doing it here equals the reference's separate pass).  wb[t] indexes the fluid
 // boundary list or is -1.  Solid nodes keep 0 or their extrapolated value, as after the reference's
-// zero-then-extrapolate sequence.
+// zero-then-extrapolate sequence.  Listed bricks are evaluated, or, when q1b (every phi their normals depend on
+// lies within the band: the normals are +0), cleared.
 template <class NT>
-__global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl, int per) {
-    const int t = xcd_block(per) * BLOCK + threadIdx.x;
-    if (t >= nl.n) return;
+__global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl, BrickWork w) {
+    GroupItems<64> g(w);
     const Layout L = d.L;
-    const int n = nl.pos[t];
-    const int b = brick_of(d, n);
     const Normals<NT> N = normals<NT>(d);
-    if (d.q1b[b]) {   // every phi this normal depends on is within the band: the normal is +0
-        if (d.zfill[b]) {
-            N.x[n] = (NT)0.0;
-            N.y[n] = (NT)0.0;
-            N.z[n] = (NT)0.0;
-            N.n[n] = (NT)0.0;
+    for (; g.i < g.hi; g.i += GroupItems<64>::STRIDE) {
+        const int4 it = w.list[g.i];
+        const int t = it.y + g.lane;
+        bool nz = false;
+        if (t < it.z) {
+            const int n = nl.pos[t];
+            if (it.w) {
+                N.x[n] = (NT)0.0;
+                N.y[n] = (NT)0.0;
+                N.z[n] = (NT)0.0;
+                N.n[n] = (NT)0.0;
+            } else {
+                double cx = dx4(d.phi, n, L.px, L.pxy);
+                double cy = dy4(d.phi, n, L.px, L.pxy);
+                double cz = dz4(d.phi, n, L.px, L.pxy);
+                double cn;
+                normalise(cx, cy, cz, cn);
+                const int fb = wb[t];
+                if (fb >= 0 && cn > 1e-6)
+                    wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
+                const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
+                N.x[n] = sx;
+                N.y[n] = sy;
+                N.z[n] = sz;
+                N.n[n] = sn;
+                nz = bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn);
+            }
         }
-        return;
+        mark_nonzero<64>(d, it.x, nz, g.lane);
     }
-    double cx = dx4(d.phi, n, L.px, L.pxy);
-    double cy = dy4(d.phi, n, L.px, L.pxy);
-    double cz = dz4(d.phi, n, L.px, L.pxy);
-    double cn;
-    normalise(cx, cy, cz, cn);
-    const int fb = wb[t];
-    if (fb >= 0 && cn > 1e-6) wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
-    const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
-    N.x[n] = sx;
-    N.y[n] = sy;
-    N.z[n] = sz;
-    N.n[n] = sn;
-    if (bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn)) d.nzb[b] = 1;
 }
-// normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110)
+// normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110); listed
+// bricks are evaluated, or, when q2b (every normal read is +0, so is the weighted mean), cleared
 template <class NT>
-__global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, int per) {
-    const int t = xcd_block(per) * BLOCK + threadIdx.x;
-    if (t >= s.n) return;
+__global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, BrickWork w) {
+    GroupItems<16> g(w);
     const Normals<NT> N = normals<NT>(d);
-    const int n = s.pos[t];
-    const int b = brick_of(d, n);
-    if (d.q2b[b]) {   // every normal it reads is +0: so is the weighted mean
-        if (d.zfill[b]) {
-            N.x[n] = (NT)0.0;
-            N.y[n] = (NT)0.0;
-            N.z[n] = (NT)0.0;
-        }
-        return;
-    }
-    const unsigned mk = s.mask[t];
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
+        const int4 it = w.list[g.i];
+        bool nz = false;
+        for (int t = it.y + g.lane; t < it.z; t += 16) {
+            const int n = s.pos[t];
+            if (it.w) {
+                N.x[n] = (NT)0.0;
+                N.y[n] = (NT)0.0;
+                N.z[n] = (NT)0.0;
+                continue;
+            }
+            // the neighbours loaded nine directions at a time, unconditionally (inside the padded grid, as in
+            // phi_extrapolate_at), and the fluid ones summed in the reference's order
+            const unsigned mk = s.mask[t];
+            double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll
-    for (int q = 1; q < 19; ++q)
-        if (mk & (1u << q)) {
-            const int o = n + d.L.off(q);
-            ax = ax + (double)N.x[o] * lat::W[q];
-            ay = ay + (double)N.y[o] * lat::W[q];
-            az = az + (double)N.z[o] * lat::W[q];
+            for (int h = 0; h < 2; ++h) {
+                NT vx[9], vy[9], vz[9];
+#pragma unroll
+                for (int u = 0; u < 9; ++u) {
+                    const int o = n + d.L.off(1 + 9 * h + u);
+                    vx[u] = N.x[o];
+                    vy[u] = N.y[o];
+                    vz[u] = N.z[o];
+                }
+#pragma unroll
+                for (int u = 0; u < 9; ++u) {
+                    const int q = 1 + 9 * h + u;
+                    const bool f = (mk & (1u << q)) != 0;
+                    ax = f ? ax + (double)vx[u] * lat::W[q] : ax;
+                    ay = f ? ay + (double)vy[u] * lat::W[q] : ay;
+                    az = f ? az + (double)vz[u] * lat::W[q] : az;
+                }
+            }
+            const NT sx = (NT)(ax / s.w[t]), sy = (NT)(ay / s.w[t]), sz = (NT)(az / s.w[t]);
+            N.x[n] = sx;
+            N.y[n] = sy;
+            N.z[n] = sz;
+            nz = nz || bits_set(sx) || bits_set(sy) || bits_set(sz);
         }
-    const NT sx = (NT)(ax / s.w[t]), sy = (NT)(ay / s.w[t]), sz = (NT)(az / s.w[t]);
-    N.x[n] = sx;
-    N.y[n] = sy;
-    N.z[n] = sz;
-    if (bits_set(sx) | bits_set(sy) | bits_set(sz)) d.nzb[b] = 1;
+        mark_nonzero<16>(d, it.x, nz, g.lane);
+    }
 }
-// Brick flag kernels.  A work-group owns 8 x 8 x 4 bricks, stages them with a halo of one brick in LDS and
-// combines each brick's 27-brick neighbourhood from there.
+// The plan of one chain.  A work-group owns TBX x TBY x TBZ bricks and stages the band flags of them and a halo
+// of two bricks in LDS.  From the band flags the lattice update left (and bstatic) it derives q1b (every brick
+// within one quiet with one sign; no brick when `full`) over the tile and a one-brick halo, then q2b (q1b over the
+// 27-brick neighbourhood); zfill := nzb; nzb, nzd and the next band buffers are cleared (the chain stages set nzb
+// and nzd again where they store a normal with a bit set).  The bricks each stage visits are appended to its work
+// list with their entry range: normals where the brick has normal nodes and (!q1b or zfill), normal extrapolation
+// where it has solid nodes within one halo layer and (!q2b or zfill), phi extrapolation where it has solid nodes
+// and !q2b.  The other set of list counters is cleared for the next chain.
 constexpr int TBX = 8, TBY = 8, TBZ = 4;                     // TBX * TBY * TBZ == BLOCK
-constexpr int TSX = TBX + 2, TSY = TBY + 2, TSZ = TBZ + 2;   // staged bricks with the halo
+constexpr int TSX = TBX + 2, TSY = TBY + 2, TSZ = TBZ + 2;   // q1b region: the tile and a one-brick halo
+constexpr int TUX = TBX + 4, TUY = TBY + 4, TUZ = TBZ + 4;   // band region: the tile and a two-brick halo
 static_assert(TBX * TBY * TBZ == BLOCK, "one brick per thread");
-// stage value of every brick of the tile and its halo (outside the grid: `outside`), then the index of this
-// thread's brick in the tile (its 27 neighbours are t + dx + TSX dy + TSX TSY dz, dx, dy, dz in 0..2), or -1
-template <class F>
-__device__ __forceinline__ int stage_bricks(const Dev& d, unsigned char* sh, unsigned char outside, const F& value,
-                                            int& b) {
-    const int bx0 = blockIdx.x * TBX, by0 = blockIdx.y * TBY, bz0 = blockIdx.z * TBZ;
+struct ChainPlan {
+    const unsigned char* notm;   // band flags of the lattice update since the last chain
+    const unsigned char* notp;
+    unsigned char* next_m;       // band flags the next lattice update fills
+    unsigned char* next_p;
+    const unsigned char* hasb;   // per brick: bit 0 normal nodes, bit 1 solid nodes, bit 2 solid nodes in one halo layer
+    const int* off_n;            // brick offsets of the sorted lists (nbricks + 1 each)
+    const int* off_s;
+    const int* off_1;
+    int4* list_n;                // work lists and their counters
+    int4* list_x;
+    int4* list_p;
+    int* cnt;                    // normals, normal extrapolation, phi extrapolation
+    int* cnt_next;               // the counters of the next chain
+};
+// IN != 0: the first 2 gface work-groups run the inlet and outlet conditions of the step instead (k_open_faces):
+// they touch no brick flag, and the phi they write in the halo planes is read only by the chain stages after the
+// plan, so the latency-bound face work overlaps the plan.  ntx, nty: tiles per axis.
+template <class T, int IN, int OUT, bool AFTER>
+__global__ void __launch_bounds__(BLOCK) k_brick_plan(Dev d, ChainPlan p, int full, FaceArgs a, int gface, int ntx, int nty) {
+    __shared__ unsigned char band[TUX * TUY * TUZ];
+    __shared__ unsigned char q1s[TSX * TSY * TSZ];
+    __shared__ int wcount[3][BLOCK / 64];
+    __shared__ int wbase[3];
+    int tile = (int)blockIdx.x;
+    if (IN != 0) {
+        if (tile < 2 * gface) {
+            open_faces_block<T, IN, OUT, AFTER>(d, a, gface, tile);
+            return;
+        }
+        tile -= 2 * gface;
+    }
+    const int bx0 = (tile % ntx) * TBX, by0 = ((tile / ntx) % nty) * TBY, bz0 = (tile / (ntx * nty)) * TBZ;
+    for (int t = threadIdx.x; t < TUX * TUY * TUZ; t += BLOCK) {
+        const int x = bx0 + t % TUX - 2, y = by0 + (t / TUX) % TUY - 2, z = bz0 + t / (TUX * TUY) - 2;
+        unsigned char v = 0;
+        if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) {
+            const int c = x + d.nbx * (y + d.nby * z);
+            const bool st = d.bstatic[c] != 0;
+            v = (unsigned char)((st || p.notm[c] ? 1 : 0) | (st || p.notp[c] ? 2 : 0));
+        }
+        band[t] = v;
+    }
+    __syncthreads();
+    // q1b of the region brick (lx - 1, ly - 1, lz - 1) of the tile: its neighbourhood is band (lx..lx+2, ...)
     for (int t = threadIdx.x; t < TSX * TSY * TSZ; t += BLOCK) {
-        const int x = bx0 + t % TSX - 1, y = by0 + (t / TSX) % TSY - 1, z = bz0 + t / (TSX * TSY) - 1;
-        sh[t] = (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz)
-                    ? value(x + d.nbx * (y + d.nby * z)) : outside;
+        const int lx = t % TSX, ly = (t / TSX) % TSY, lz = t / (TSX * TSY);
+        const int x = bx0 + lx - 1, y = by0 + ly - 1, z = bz0 + lz - 1;
+        unsigned char q = 1;   // outside the grid: quiet
+        if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) {
+            unsigned char a = 0;
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 3; ++dx) a |= band[(lx + dx) + TUX * ((ly + dy) + TUY * (lz + dz))];
+            const bool m = full != 0 || (a & 1), pl = full != 0 || (a & 2);
+            q = (!m || !pl) ? 1 : 0;
+        }
+        q1s[t] = q;
     }
     __syncthreads();
     const int tx = threadIdx.x % TBX, ty = (threadIdx.x / TBX) % TBY, tz = threadIdx.x / (TBX * TBY);
     const int x = bx0 + tx, y = by0 + ty, z = bz0 + tz;
-    if (x >= d.nbx || y >= d.nby || z >= d.nbz) return -1;
-    b = x + d.nbx * (y + d.nby * z);
-    return tx + TSX * (ty + TSY * tz);
-}
-__device__ __forceinline__ unsigned char or27(const unsigned char* sh, int t) {
-    unsigned char a = 0;
+    const bool valid = x < d.nbx && y < d.nby && z < d.nbz;
+    const int b = valid ? x + d.nbx * (y + d.nby * z) : 0;
+    bool take[3] = {false, false, false};
+    unsigned char q1 = 0, q2 = 0;
+    if (valid) {
+        const int r = tx + TSX * (ty + TSY * tz);   // this brick's neighbourhood corner in the q1b region
+        q1 = q1s[r + 1 + TSX * (1 + TSY)];
+        q2 = 1;
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
+        for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) a |= sh[t + dx + TSX * (dy + TSY * dz)];
-    return a;
-}
-__device__ __forceinline__ unsigned char and27(const unsigned char* sh, int t) {
-    unsigned char a = 1;
+                for (int dx = 0; dx < 3; ++dx) q2 &= q1s[r + dx + TSX * (dy + TSY * dz)];
+        const unsigned char zf = d.nzb[b];
+        const unsigned char h = p.hasb[b];
+        d.q1b[b] = q1;
+        d.q2b[b] = q2;
+        d.zfill[b] = zf;
+        d.nzb[b] = 0;
+        d.nzd[b] = 0;
+        p.next_m[b] = 0;
+        p.next_p[b] = 0;
+        take[0] = (h & 1) && (!q1 || zf);
+        take[1] = (h & 4) && (!q2 || zf);
+        take[2] = (h & 2) && !q2;
+    }
+    // one atomic per list and work-group: wave counts, then the group's base, then each brick's slot
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)threadIdx.x >> 6;
+    unsigned long long m[3];
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz)
+    for (int l = 0; l < 3; ++l) {
+        m[l] = __ballot(take[l]);
+        if (lane == 0) wcount[l][wv] = __popcll(m[l]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int tot = 0;
+        for (int v = 0; v < BLOCK / 64; ++v) tot += wcount[threadIdx.x][v];
+        wbase[threadIdx.x] = tot ? atomicAdd(p.cnt + threadIdx.x, tot) : 0;
+    }
+    __syncthreads();
+    int4* const lists[3] = {p.list_n, p.list_x, p.list_p};
+    const int* const offs[3] = {p.off_n, p.off_1, p.off_s};
+    const unsigned char quiet[3] = {q1, q2, 0};
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) a &= sh[t + dx + TSX * (dy + TSY * dz)];
-    return a;
-}
-// Brick flags of one chain (see Dev): q1b from the band flags of the step (no brick quiet when `full`), zfill
-// from nzb, nzb cleared (the chain rewrites or keeps +0 every normal), the other band buffers cleared for the
-// next step.
-__global__ void __launch_bounds__(BLOCK) k_brick_quiet(Dev d, const unsigned char* __restrict__ notm,
-                                                       const unsigned char* __restrict__ notp,
-                                                       unsigned char* __restrict__ next_m,
-                                                       unsigned char* __restrict__ next_p, int full) {
-    __shared__ unsigned char sh[TSX * TSY * TSZ];
-    int b = 0;
-    const int t = stage_bricks(d, sh, 0, [&](int c) {
-        const bool st = d.bstatic[c] != 0;
-        return (unsigned char)((st || notm[c] ? 1 : 0) | (st || notp[c] ? 2 : 0));
-    }, b);
-    if (t < 0) return;
-    const unsigned char a = or27(sh, t);
-    const bool m = full != 0 || (a & 1), p = full != 0 || (a & 2);
-    d.q1b[b] = (!m || !p) ? 1 : 0;
-    d.zfill[b] = d.nzb[b];
-    d.nzb[b] = 0;
-    next_m[b] = 0;
-    next_p[b] = 0;
-}
-// q2b = q1b of the brick and all 26 neighbours
-__global__ void __launch_bounds__(BLOCK) k_brick_quiet2(Dev d) {
-    __shared__ unsigned char sh[TSX * TSY * TSZ];
-    int b = 0;
-    const int t = stage_bricks(d, sh, 1, [&](int c) { return d.q1b[c]; }, b);
-    if (t < 0) return;
-    d.q2b[b] = and27(sh, t);
-}
-// nzd = nzb dilated by one brick, after the chain has written the normal field
-__global__ void __launch_bounds__(BLOCK) k_brick_normals(Dev d) {
-    __shared__ unsigned char sh[TSX * TSY * TSZ];
-    int b = 0;
-    const int t = stage_bricks(d, sh, 0, [&](int c) { return d.nzb[c]; }, b);
-    if (t < 0) return;
-    d.nzd[b] = or27(sh, t) ? 1 : 0;
+    for (int l = 0; l < 3; ++l) {
+        if (!take[l]) continue;
+        int slot = wbase[l] + __popcll(m[l] & ((1ull << lane) - 1ull));
+        for (int v = 0; v < wv; ++v) slot += wcount[l][v];
+        lists[l][slot] = int4{b, offs[l][b], offs[l][b + 1], (int)quiet[l]};
+    }
+    if (tile == 0 && threadIdx.x < 3) p.cnt_next[threadIdx.x] = 0;
 }
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
@@ -1617,7 +1762,9 @@ struct mflbm_ctx {
     double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
     float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
     int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
-    unsigned char* qbuf = nullptr;   // brick flags: notm, notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd
+    unsigned char* qbuf = nullptr;   // brick flags: notm, notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd, hasb
+    int* cbuf = nullptr;             // chain lists: 3 work lists (nbricks int4 items each), the brick offsets of the
+                                     // 3 sorted lists (nbricks + 1 each), 2 sets of 3 work-list counters
     long nbricks = 0;
     int band_cur = 0;                // band buffers the lattice update fills until the next chain
     bool full_chain = true;          // the next chain evaluates every brick (new geometry or state)
@@ -1671,44 +1818,92 @@ Fold fold_of(const mflbm_params& p) {
     return Fold{p.jper == 1 && p.kper == 0 && !open_z(p) && !plate, p.kper == 1 && p.jper == 0 && !plate};
 }
 
-// brick flag kernels: one work-group per TBX x TBY x TBZ bricks
-#define BRICK_LAUNCH(c, kern, ...)                                                                           \
+
+// work list of one chain stage (0 normals, 1 normal extrapolation, 2 phi extrapolation) and its counter in set `set`
+BrickWork brick_work(const mflbm_ctx* c, int work, int set) {
+    const long nb = c->nbricks;
+    const int4* lists = (const int4*)c->cbuf;
+    return BrickWork{lists + (size_t)work * nb, c->cbuf + 12 * (size_t)nb + 3 * (size_t)(nb + 1) + 3 * set + work};
+}
+#define CHAIN_LAUNCH(c, kern, ...)                                                                           \
     do {                                                                                                     \
-        const dim3 g_((unsigned)(((c)->d.nbx + TBX - 1) / TBX), (unsigned)(((c)->d.nby + TBY - 1) / TBY),       \
-                      (unsigned)(((c)->d.nbz + TBZ - 1) / TBZ));                                             \
-        hipLaunchKernelGGL(kern, g_, dim3(BLOCK), 0, (c)->stream, __VA_ARGS__);                              \
+        hipLaunchKernelGGL(kern, dim3(CHAIN_BLOCKS), dim3(BLOCK), 0, (c)->stream, __VA_ARGS__);              \
         const hipError_t e_ = hipGetLastError();                                                             \
         if (e_ != hipSuccess) return fail((c), MFLBM_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(e_));  \
     } while (0)
 
-// dense normal field by the list kernels (the chain of the reference, one pass per stage)
+// the normal field over the bricks of the work lists of set `set`
 template <class NT>
-int launch_normals_dense(mflbm_ctx* c) {
-    LAUNCH(c, k_normals<NT>, NXCD * xcd_per(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl, xcd_per(c->nrm.n));
-    LAUNCH(c, k_normal_extrapolate<NT>, NXCD * xcd_per(c->sl1.n), c->d, c->sl1, xcd_per(c->sl1.n));
-    BRICK_LAUNCH(c, k_brick_normals, c->d);
+int launch_normals_listed(mflbm_ctx* c, int set) {
+    CHAIN_LAUNCH(c, k_normals<NT>, c->d, c->nrm, (const int*)c->wbuf, c->fl, brick_work(c, 0, set));
+    CHAIN_LAUNCH(c, k_normal_extrapolate<NT>, c->d, c->sl1, brick_work(c, 1, set));
     return MFLBM_OK;
 }
+
+// Open inlet and outlet conditions a step hands to the chain's plan launch (kind 0: none)
+struct FaceWork {
+    int in = 0, out = 0;
+    bool after = false;
+    FaceArgs a{};
+    int gface = 0;
+};
+template <class T, int IN, int OUT, bool AFTER>
+int launch_plan_k(mflbm_ctx* c, const ChainPlan& p, int full, const FaceWork& f) {
+    const int ntx = (c->d.nbx + TBX - 1) / TBX, nty = (c->d.nby + TBY - 1) / TBY, ntz = (c->d.nbz + TBZ - 1) / TBZ;
+    const int nface = IN != 0 ? 2 * f.gface : 0;
+    hipLaunchKernelGGL((k_brick_plan<T, IN, OUT, AFTER>), dim3(nface + ntx * nty * ntz), dim3(BLOCK), 0, c->stream,
+                       c->d, p, full, f.a, f.gface, ntx, nty);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, MFLBM_ERR_HIP, "launch k_brick_plan: %s", hipGetErrorString(e));
+    return MFLBM_OK;
+}
+template <class T>
+int launch_plan(mflbm_ctx* c, const ChainPlan& p, int full, const FaceWork& f) {
+#define MFLBM_PLAN(IN, OUT)                                                                                     \
+    return f.after ? launch_plan_k<T, IN, OUT, true>(c, p, full, f) : launch_plan_k<T, IN, OUT, false>(c, p, full, f);
+    if (f.in == 1 && f.out == 1) { MFLBM_PLAN(1, 1) }
+    if (f.in == 1 && f.out == 2) { MFLBM_PLAN(1, 2) }
+    if (f.in == 2 && f.out == 1) { MFLBM_PLAN(2, 1) }
+    if (f.in == 2 && f.out == 2) { MFLBM_PLAN(2, 2) }
+#undef MFLBM_PLAN
+    return launch_plan_k<double, 0, 0, false>(c, p, full, f);
+}
+
+// The colour-gradient chain of one step (Phase_gradient.F90:5-204): the brick plan, then phi extrapolation,
+// normals and normal extrapolation over the bricks it listed (or the fused chain).  The plan reads the band
+// flags the last lattice update filled and clears the other buffers, which the next lattice update fills.
 template <class NT>
-int launch_gradient_t(mflbm_ctx* c) {
-    // the bricks this chain may skip (none after new geometry or state, and none with the fused chain, which does
-    // not use the flags); then the lattice update switches to the other band buffers, which this launch clears
+int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
+    const long nb = c->nbricks;
+    const int set = c->band_cur;
     {
-        const long nb = c->nbricks;
-        unsigned char* m_cur = c->qbuf + (size_t)c->band_cur * nb;
-        unsigned char* p_cur = c->qbuf + (size_t)(2 + c->band_cur) * nb;
-        unsigned char* m_next = c->qbuf + (size_t)(c->band_cur ^ 1) * nb;
-        unsigned char* p_next = c->qbuf + (size_t)(2 + (c->band_cur ^ 1)) * nb;
+        ChainPlan p;
+        p.notm = c->qbuf + (size_t)set * nb;
+        p.notp = c->qbuf + (size_t)(2 + set) * nb;
+        p.next_m = c->qbuf + (size_t)(set ^ 1) * nb;
+        p.next_p = c->qbuf + (size_t)(2 + (set ^ 1)) * nb;
+        p.hasb = c->qbuf + 10 * (size_t)nb;
+        p.off_n = c->cbuf + 12 * nb;
+        p.off_s = c->cbuf + 12 * nb + (nb + 1);
+        p.off_1 = c->cbuf + 12 * nb + 2 * (nb + 1);
+        int4* lists = (int4*)c->cbuf;
+        p.list_n = lists;
+        p.list_x = lists + nb;
+        p.list_p = lists + 2 * nb;
+        int* cnts = c->cbuf + 12 * (size_t)nb + 3 * (size_t)(nb + 1);
+        p.cnt = cnts + 3 * set;
+        p.cnt_next = cnts + 3 * (set ^ 1);
+        // no brick quiet after new geometry or state, and none with the fused chain, which does not use the flags
         const int full = (c->full_chain || c->chain_fused) ? 1 : 0;
-        BRICK_LAUNCH(c, k_brick_quiet, c->d, (const unsigned char*)m_cur, (const unsigned char*)p_cur, m_next, p_next, full);
-        BRICK_LAUNCH(c, k_brick_quiet2, c->d);
+        const int rp = c->fp32 ? launch_plan<float>(c, p, full, faces) : launch_plan<double>(c, p, full, faces);
+        if (rp) return rp;
         c->band_cur ^= 1;
-        c->d.notm = m_next;
-        c->d.notp = p_next;
+        c->d.notm = p.next_m;
+        c->d.notp = p.next_p;
         c->full_chain = false;
         c->solid_zeroed = false;
     }
-    LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n), 0);
+    CHAIN_LAUNCH(c, k_phi_extrapolate_b, c->d, c->sl, brick_work(c, 2, set));
     if (c->chain_fused) {
         const Layout& L = c->L;
         const int ntx = (L.nx + CT - 1) / CT, nty = (L.ny + CT - 1) / CT, nseg = (L.nz + CSEG - 1) / CSEG;
@@ -1720,14 +1915,14 @@ int launch_gradient_t(mflbm_ctx* c) {
         if (e != hipSuccess) return fail(c, MFLBM_ERR_HIP, "launch k_chain_force: %s", hipGetErrorString(e));
         return MFLBM_OK;
     }
-    const int rc = launch_normals_dense<NT>(c);
-    if (rc) return rc;
-    return MFLBM_OK;
+    return launch_normals_listed<NT>(c, set);
 }
-int launch_gradient(mflbm_ctx* c) { return c->fp32 ? launch_gradient_t<float>(c) : launch_gradient_t<double>(c); }
+int launch_gradient(mflbm_ctx* c, const FaceWork& faces = FaceWork{}) {
+    return c->fp32 ? launch_gradient_t<float>(c, faces) : launch_gradient_t<double>(c, faces);
+}
 
 template <class T>
-int launch_faces(mflbm_ctx* c, bool odd);
+int launch_faces(mflbm_ctx* c, bool odd, FaceWork* defer);
 
 int launch_step(mflbm_ctx* c, int ntime) {
     c->stepped = true;
@@ -1772,18 +1967,21 @@ int launch_step(mflbm_ctx* c, int ntime) {
     }
 #undef MFLBM_COLLIDE32
     if (t) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    const int rc = c->fp32 ? launch_faces<float>(c, odd) : launch_faces<double>(c, odd);
+    FaceWork faces;
+    const int rc = c->fp32 ? launch_faces<float>(c, odd, &faces) : launch_faces<double>(c, odd, &faces);
     if (rc) return rc;
     if (t) HIPCHK(c, hipEventRecord(ev[2], c->stream));
-    const int rg = launch_gradient(c);
+    const int rg = launch_gradient(c, faces);
     if (rg) return rg;
     if (t) HIPCHK(c, hipEventRecord(ev[3], c->stream));
     return MFLBM_OK;
 }
 
-// periodic wrap and boundary conditions of one step, after the lattice update
+// periodic wrap and boundary conditions of one step, after the lattice update.  The combined inlet + outlet
+// condition of a domain without a porous plate is handed to the chain's plan launch through `defer` (the chain
+// follows at once), where it overlaps the plan; otherwise it is launched here.
 template <class T>
-int launch_faces(mflbm_ctx* c, bool odd) {
+int launch_faces(mflbm_ctx* c, bool odd, FaceWork* defer) {
     const mflbm_params& p = c->p;
     const Layout& L = c->L;
     // periodic wrap (the reference's MPI exchange of one rank with itself)
@@ -1807,6 +2005,15 @@ int launch_faces(mflbm_ctx* c, bool odd) {
     if (open_z(p) && both && L.nz >= 8) {
         const FaceArgs a{p.phi_inlet, p.sa_inject, p.relaxation, p.rho_in, p.uin_avg, p.rho_out};
         const int g = (int)gface;
+        const bool plate = p.porous_plate_cmd == 1 || p.porous_plate_cmd == 2;
+        if (defer && !plate) {
+            defer->in = p.inlet_bc;
+            defer->out = p.outlet_bc;
+            defer->after = odd;
+            defer->a = a;
+            defer->gface = g;
+            return MFLBM_OK;
+        }
 #define MFLBM_FACES(IN, OUT)                                                                                    \
     if (odd) LAUNCH(c, (k_open_faces<T, IN, OUT, true>), 2 * gface, c->d, a, g);                                \
     else LAUNCH(c, (k_open_faces<T, IN, OUT, false>), 2 * gface, c->d, a, g);
@@ -1853,7 +2060,7 @@ int scatter_state(mflbm_ctx* c) {
 }
 
 void free_geometry(mflbm_ctx* c) {
-    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role, c->qbuf};
+    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role, c->qbuf, c->cbuf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
@@ -1867,6 +2074,7 @@ void free_geometry(mflbm_ctx* c) {
     c->wbase = nullptr;
     c->role = nullptr;
     c->qbuf = nullptr;
+    c->cbuf = nullptr;
     c->have_geom = false;
     c->have_state = false;
 }
@@ -2032,13 +2240,25 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     T.visit(-1, L.nx + 2, -1, L.ny + 2, -1, L.nz + 2, [&](int i, int j, int k) {
         if (wp[L.at(i, j, k)] == 0) nrm.push_back(L.at(i, j, k));
     });
-    // solid list (all, and the part inside one halo layer for the normal extrapolation), in the tile order;
+    // The chain lists are sorted by brick (brick_of), in the tile order within a brick;
     // every entry is independent, so the order changes no result
+    const int nbx = (L.px + (1 << BRICK_SX) - 1) >> BRICK_SX, nby = (L.ny + 8 + (1 << BRICK_SY) - 1) >> BRICK_SY,
+              nbz = (L.nz + 8 + (1 << BRICK_SZ) - 1) >> BRICK_SZ;
+    const size_t nbr = (size_t)nbx * nby * nbz;
+    auto brick = [&](int o) -> long {
+        const int kp = o / L.pxy, r = o - kp * L.pxy, jp = r / L.px, ip = r - jp * L.px;
+        return (ip >> BRICK_SX) + nbx * ((jp >> BRICK_SY) + (long)nby * (kp >> BRICK_SZ));
+    };
+    std::stable_sort(nrm.begin(), nrm.end(), [&](int a, int b) { return brick(a) < brick(b); });
+    // solid list (all, and the part inside one halo layer for the normal extrapolation)
     std::vector<int> sperm(ns);
     for (int t = 0; t < ns; ++t) sperm[t] = t;
     {
-        std::vector<long> skey(ns);
-        for (int t = 0; t < ns; ++t) skey[t] = T.key(L.at(s_ijk[3 * t], s_ijk[3 * t + 1], s_ijk[3 * t + 2]));
+        std::vector<std::pair<long, long>> skey(ns);
+        for (int t = 0; t < ns; ++t) {
+            const int o = L.at(s_ijk[3 * t], s_ijk[3 * t + 1], s_ijk[3 * t + 2]);
+            skey[t] = {brick(o), T.key(o)};
+        }
         std::stable_sort(sperm.begin(), sperm.end(), [&](int a, int b) { return skey[a] < skey[b]; });
     }
     std::vector<int> spos(ns), s1pos;
@@ -2136,8 +2356,6 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     // zfill, nzb and nzd start set, so the first chain and lattice update evaluate everything; bstatic from the
     // walls: bricks with a non-solid node outside the interior, and bricks with an interior node within 4 of a
     // periodic face
-    const int nbx = L.px >> BRICK_SHIFT, nby = (L.ny + 8 + 3) >> BRICK_SHIFT, nbz = (L.nz + 8 + 3) >> BRICK_SHIFT;
-    const size_t nbr = (size_t)nbx * nby * nbz;
     std::vector<unsigned char> hstatic(nbr, 0);
     {
         const int py = L.ny + 8, pz = L.nz + 8;
@@ -2150,12 +2368,31 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
                     const bool edge = inside && ((c->p.kper == 1 && (k <= 4 || k > L.nz - 4)) ||
                                                  (c->p.jper == 1 && (j <= 4 || j > L.ny - 4)));
                     if ((!inside && wp[o] == 0) || edge)
-                        hstatic[(ip >> BRICK_SHIFT) + nbx * ((jp >> BRICK_SHIFT) + (size_t)nby * (kp >> BRICK_SHIFT))] = 1;
+                        hstatic[(ip >> BRICK_SX) + nbx * ((jp >> BRICK_SY) + (size_t)nby * (kp >> BRICK_SZ))] = 1;
                 }
     }
-    HIPCHK(c, hipMalloc(&c->qbuf, 10 * nbr));
+    // chain lists: the brick offsets of the sorted lists, which of them a brick has entries in (hasb), and the
+    // work lists with their counters (zero: the first plan fills set 0)
+    std::vector<int> hc(12 * nbr + 3 * (nbr + 1) + 6, 0);
+    std::vector<unsigned char> hasb(nbr, 0);
+    {
+        const std::vector<int>* lists[3] = {&nrm, &spos, &s1pos};
+        for (int l = 0; l < 3; ++l) {
+            int* off = hc.data() + 12 * nbr + (size_t)l * (nbr + 1);
+            for (int o : *lists[l]) ++off[brick(o) + 1];
+            for (size_t b = 0; b < nbr; ++b) {
+                if (off[b + 1] > 64) return fail(c, MFLBM_ERR_ARG, "brick with more than 64 chain entries");
+                if (off[b + 1]) hasb[b] |= (unsigned char)(1 << l);
+                off[b + 1] += off[b];
+            }
+        }
+    }
+    HIPCHK(c, hipMalloc(&c->qbuf, 11 * nbr));
     HIPCHK(c, hipMemset(c->qbuf, 1, 10 * nbr));
     HIPCHK(c, hipMemcpy(c->qbuf + 4 * nbr, hstatic.data(), nbr, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->qbuf + 10 * nbr, hasb.data(), nbr, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->cbuf, sizeof(int) * hc.size()));
+    HIPCHK(c, hipMemcpy(c->cbuf, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice));
     c->nbricks = (long)nbr;
     c->band_cur = 0;
     c->full_chain = true;
@@ -2271,7 +2508,7 @@ extern "C" int mflbm_init_state_device(mflbm_ctx* c, int32_t option, double inte
 // pass has run since that chain (compute_macro_vars, Misc.F90:423)
 int refresh_solid_phi(mflbm_ctx* c) {
     if (!c->have_geom || !c->have_state) return MFLBM_OK;
-    LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n), 1);
+    LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
     if (c->solid_zeroed) LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
     return MFLBM_OK;
 }
@@ -2425,8 +2662,10 @@ extern "C" int mflbm_download_gradient(mflbm_ctx* c, double* cn_x, double* cn_y,
     const Layout& L = c->L;
     // the step keeps curvature only at fluid nodes (folded into the force input); the dense field the
     // reference holds is evaluated from the current normals on request
-    if (c->chain_fused) {   // the fused chain does not keep the dense normal field: evaluate it now
-        const int rc = c->fp32 ? launch_normals_dense<float>(c) : launch_normals_dense<double>(c);
+    if (c->chain_fused) {   // the fused chain does not keep the dense normal field: evaluate it now, over the
+                            // lists of the last chain (every brick: the fused chain plans a full chain)
+        const int set = c->band_cur ^ 1;
+        const int rc = c->fp32 ? launch_normals_listed<float>(c, set) : launch_normals_listed<double>(c, set);
         if (rc) return rc;
     }
     if (curv) {
