@@ -54,15 +54,11 @@ struct Dev {
     double* force;    // per fluid node: cn_x, cn_y, cn_z, 0.5*gamma*curv*c_norm (stride fs)
     long fs;
     double* phi;
-    double* cnx;
-    double* cny;
-    double* cnz;
-    double* cnorm;
-    // FP32 storage modes: the normal field in FP32 (cn_x, cn_y, cn_z, c_norm), the same dense layout
-    float* cnx32;
-    float* cny32;
-    float* cnz32;
-    float* cnorm32;
+    // The normal field, one 32-byte record (cn_x, cn_y, cn_z, c_norm) per dense node: a neighbour's normal is one
+    // cache-line access in the stencils that read it (normal extrapolation, curvature)
+    double4* cn;
+    // FP32 storage modes: the normal field in FP32 (16-byte records), the same dense layout
+    float4* cn32;
     double* curv;     // dense curvature, filled only for diagnostics
     // Brick flags of the list chain.  The padded grid is cut into bricks of 64 nodes (brick_of).  Away
     // from the interface phi is +-1 to within rounding, and there the ISO4 gradient is below the 1e-6 at which
@@ -230,10 +226,11 @@ k_collide(Dev d, CollideParams P, int per) {
     double cnx, cny, cnz, tf;
     if (STENCIL) {
         if (d.nzd[brick_of(d, nn)]) {
-            cnx = d.cnx[nn];
-            cny = d.cny[nn];
-            cnz = d.cnz[nn];
-            tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * d.cnorm[nn];
+            const double4 c4 = d.cn[nn];
+            cnx = c4.x;
+            cny = c4.y;
+            cnz = c4.z;
+            tf = 0.5 * P.gamma * curvature_at<double>(d, nn) * c4.w;
         } else {   // every normal of the stencil is +0: the same expressions on those values
             cnx = 0.0;
             cny = 0.0;
@@ -305,10 +302,11 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     C cnx, cny, cnz, tf;
     if (STENCIL) {
         if (d.nzd[brick_of(d, nn)]) {
-            cnx = (C)d.cnx32[nn];
-            cny = (C)d.cny32[nn];
-            cnz = (C)d.cnz32[nn];
-            tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)d.cnorm32[nn]);
+            const float4 c4 = d.cn32[nn];
+            cnx = (C)c4.x;
+            cny = (C)c4.y;
+            cnz = (C)c4.z;
+            tf = (C)(0.5 * gamma * curvature_at<float>(d, nn) * (double)c4.w);
         } else {
             cnx = (C)0.0f;
             cny = (C)0.0f;
@@ -1006,20 +1004,29 @@ __device__ __forceinline__ void normalise(double& cx, double& cy, double& cz, do
 
 // The normal field: FP64 (the reference arithmetic) or, in the FP32 storage modes, FP32 (the gradient and
 // curvature are still evaluated in FP64 from the stored values; the FP32 field halves the bytes the chain and
-// the lattice update's curvature stencil move).
+// the lattice update's curvature stencil move).  Rec<NT>: the record type.
 template <class NT>
-struct Normals {
-    NT* x;
-    NT* y;
-    NT* z;
-    NT* n;
+struct Rec;
+template <>
+struct Rec<double> {
+    using type = double4;
+};
+template <>
+struct Rec<float> {
+    using type = float4;
 };
 template <class NT>
-__device__ __forceinline__ Normals<NT> normals(const Dev& d);
+using Rec4 = typename Rec<NT>::type;
+template <class NT>
+__device__ __forceinline__ Rec4<NT>* normals(const Dev& d);
 template <>
-__device__ __forceinline__ Normals<double> normals<double>(const Dev& d) { return {d.cnx, d.cny, d.cnz, d.cnorm}; }
+__device__ __forceinline__ double4* normals<double>(const Dev& d) { return d.cn; }
 template <>
-__device__ __forceinline__ Normals<float> normals<float>(const Dev& d) { return {d.cnx32, d.cny32, d.cnz32, d.cnorm32}; }
+__device__ __forceinline__ float4* normals<float>(const Dev& d) { return d.cn32; }
+template <class NT>
+__device__ __forceinline__ Rec4<NT> rec4(double x, double y, double z, double n) {
+    return Rec4<NT>{(NT)x, (NT)y, (NT)z, (NT)n};
+}
 
 // any bit of the stored value set (-0.0 and NaN count as set)
 __device__ __forceinline__ bool bits_set(double v) { return __double_as_longlong(v) != 0; }
@@ -1139,7 +1146,7 @@ template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl, BrickWork w) {
     GroupItems<64> g(w);
     const Layout L = d.L;
-    const Normals<NT> N = normals<NT>(d);
+    Rec4<NT>* const N = normals<NT>(d);
     for (; g.i < g.hi; g.i += GroupItems<64>::STRIDE) {
         const int4 it = w.list[g.i];
         const int t = it.y + g.lane;
@@ -1147,10 +1154,7 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
         if (t < it.z) {
             const int n = nl.pos[t];
             if (it.w) {
-                N.x[n] = (NT)0.0;
-                N.y[n] = (NT)0.0;
-                N.z[n] = (NT)0.0;
-                N.n[n] = (NT)0.0;
+                N[n] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
             } else {
                 double cx = dx4(d.phi, n, L.px, L.pxy);
                 double cy = dy4(d.phi, n, L.px, L.pxy);
@@ -1161,10 +1165,7 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
                 if (fb >= 0 && cn > 1e-6)
                     wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
                 const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
-                N.x[n] = sx;
-                N.y[n] = sy;
-                N.z[n] = sz;
-                N.n[n] = sn;
+                N[n] = Rec4<NT>{sx, sy, sz, sn};
                 nz = bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn);
             }
         }
@@ -1172,20 +1173,19 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     }
 }
 // normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110); listed
-// bricks are evaluated, or, when q2b (every normal read is +0, so is the weighted mean), cleared
+// bricks are evaluated, or, when q2b (every normal read is +0, so is the weighted mean), cleared.  The record is
+// stored whole: c_norm of a solid node is never evaluated and stays +0.
 template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, BrickWork w) {
     GroupItems<16> g(w);
-    const Normals<NT> N = normals<NT>(d);
+    Rec4<NT>* const N = normals<NT>(d);
     for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
         const int4 it = w.list[g.i];
         bool nz = false;
         for (int t = it.y + g.lane; t < it.z; t += 16) {
             const int n = s.pos[t];
             if (it.w) {
-                N.x[n] = (NT)0.0;
-                N.y[n] = (NT)0.0;
-                N.z[n] = (NT)0.0;
+                N[n] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
                 continue;
             }
             // the neighbours loaded nine directions at a time, unconditionally (inside the padded grid, as in
@@ -1197,10 +1197,10 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
                 NT vx[9], vy[9], vz[9];
 #pragma unroll
                 for (int u = 0; u < 9; ++u) {
-                    const int o = n + d.L.off(1 + 9 * h + u);
-                    vx[u] = N.x[o];
-                    vy[u] = N.y[o];
-                    vz[u] = N.z[o];
+                    const Rec4<NT> r = N[n + d.L.off(1 + 9 * h + u)];
+                    vx[u] = r.x;
+                    vy[u] = r.y;
+                    vz[u] = r.z;
                 }
 #pragma unroll
                 for (int u = 0; u < 9; ++u) {
@@ -1212,9 +1212,7 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
                 }
             }
             const NT sx = (NT)(ax / s.w[t]), sy = (NT)(ay / s.w[t]), sz = (NT)(az / s.w[t]);
-            N.x[n] = sx;
-            N.y[n] = sy;
-            N.z[n] = sz;
+            N[n] = Rec4<NT>{sx, sy, sz, (NT)0.0};
             nz = nz || bits_set(sx) || bits_set(sy) || bits_set(sz);
         }
         mark_nonzero<16>(d, it.x, nz, g.lane);
@@ -1351,11 +1349,21 @@ __global__ void __launch_bounds__(BLOCK) k_brick_plan(Dev d, ChainPlan p, int fu
     }
     if (tile == 0 && threadIdx.x < 3) p.cnt_next[threadIdx.x] = 0;
 }
+// component C (0 x, 1 y, 2 z) of the normal records around node n
+template <class NT, int C>
+struct RecAt {
+    const Rec4<NT>* r;
+    int n, px, pxy;
+    __device__ __forceinline__ double operator()(int a, int b, int c) const {
+        const Rec4<NT>& v = r[n + a + b * px + c * pxy];
+        return (double)(C == 0 ? v.x : C == 1 ? v.y : v.z);
+    }
+};
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n) {
-    const Normals<NT> N = normals<NT>(d);
+    const Rec4<NT>* N = normals<NT>(d);
     const int px = d.L.px, pxy = d.L.pxy;
-    return curvature_f(DenseAt<NT>{N.x, n, px, pxy}, DenseAt<NT>{N.y, n, px, pxy}, DenseAt<NT>{N.z, n, px, pxy});
+    return curvature_f(RecAt<NT, 0>{N, n, px, pxy}, RecAt<NT, 1>{N, n, px, pxy}, RecAt<NT, 2>{N, n, px, pxy});
 }
 // curvature at the fluid nodes (Phase_gradient.F90:112-201), emitted with the normal as the per-node force
 // input of the next lattice update: (cn_x, cn_y, cn_z, 0.5*gamma*curv*c_norm), Kernel_multiphase.F90:112-116
@@ -1363,13 +1371,13 @@ template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_curvature(Dev d, double gamma, int per) {
     const int m = xcd_block(per) * BLOCK + threadIdx.x;
     if (m >= d.nf) return;
-    const Normals<NT> N = normals<NT>(d);
     const int n = d.cpos[m];
     const double curv = curvature_at<NT>(d, n);
-    d.force[m] = (double)N.x[n];
-    d.force[d.fs + m] = (double)N.y[n];
-    d.force[2 * d.fs + m] = (double)N.z[n];
-    d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)N.n[n];
+    const Rec4<NT> r = normals<NT>(d)[n];
+    d.force[m] = (double)r.x;
+    d.force[d.fs + m] = (double)r.y;
+    d.force[2 * d.fs + m] = (double)r.z;
+    d.force[3 * d.fs + m] = 0.5 * gamma * curv * (double)r.w;
 }
 // The tail of the colour-gradient chain in one pass, for the lattice update's force input: interface normals
 // with the wetting rotation, their extrapolation into the solid boundary nodes and the curvature force
@@ -1759,8 +1767,8 @@ struct mflbm_ctx {
     int device = 0;
     long ps = 0;                // stride of the dense arrays
     double* dense_pdf = nullptr;   // 38 dense arrays: upload/download staging, keeps never-touched slots
-    double* fields = nullptr;   // phi, cnx, cny, cnz, cnorm, curv, u, v, w, rho, phi_old
-    float* fields32 = nullptr;  // FP32 storage modes: cnx, cny, cnz, cnorm in FP32
+    double* fields = nullptr;   // phi, the normal records (4 arrays' worth), curv, u, v, w, rho, phi_old
+    float* fields32 = nullptr;  // FP32 storage modes: the normal records in FP32
     int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
     unsigned char* qbuf = nullptr;   // brick flags: notm, notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd, hasb
     int* cbuf = nullptr;             // chain lists: 3 work lists (nbricks int4 items each), the brick offsets of the
@@ -2139,17 +2147,11 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
         return bail(fail(c, MFLBM_ERR_NOMEM, "device allocation of the FP32 normal field failed"));
     Dev& d = c->d;
     if (c->fp32) {
-        d.cnx32 = c->fields32;
-        d.cny32 = c->fields32 + ps;
-        d.cnz32 = c->fields32 + 2 * ps;
-        d.cnorm32 = c->fields32 + 3 * ps;
+        d.cn32 = (float4*)c->fields32;
     }
     d.L = L;
     d.phi = c->fields;
-    d.cnx = c->fields + ps;
-    d.cny = c->fields + 2 * ps;
-    d.cnz = c->fields + 3 * ps;
-    d.cnorm = c->fields + 4 * ps;
+    d.cn = (double4*)(c->fields + ps);
     d.curv = c->fields + 5 * ps;
     c->u = c->fields + 6 * ps;
     c->v = c->fields + 7 * ps;
@@ -2332,7 +2334,7 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     HIPCHK(c, hipMemset(c->force, 0, sizeof(double) * fs * 4));
     // normals at solid nodes are only ever written by the extrapolation; start them from zero so no value
     // of a previous geometry survives (cn_x, cn_y, cn_z, c_norm, curv, u, v, w, rho)
-    HIPCHK(c, hipMemset(c->d.cnx, 0, sizeof(double) * c->ps * 9));
+    HIPCHK(c, hipMemset(c->d.cn, 0, sizeof(double) * c->ps * 9));
     if (c->fields32) HIPCHK(c, hipMemset(c->fields32, 0, sizeof(float) * c->ps * 4));
     std::vector<int> hi;
     hi.reserve(ni);
@@ -2674,19 +2676,25 @@ extern "C" int mflbm_download_gradient(mflbm_ctx* c, double* cn_x, double* cn_y,
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> buf(L.total);
-    std::vector<float> buf32(c->fp32 ? L.total : 0);
-    double* src[5] = {c->d.cnx, c->d.cny, c->d.cnz, c->d.cnorm, c->d.curv};
-    float* src32[4] = {c->d.cnx32, c->d.cny32, c->d.cnz32, c->d.cnorm32};
     double* dst[5] = {cn_x, cn_y, cn_z, c_norm, curv};
-    for (int m = 0; m < 5; ++m) {
-        if (!dst[m]) continue;
-        if (c->fp32 && m < 4) {   // the FP32 normal field, widened
-            HIPCHK(c, hipMemcpy(buf32.data(), src32[m], sizeof(float) * L.total, hipMemcpyDeviceToHost));
-            for (long k = 0; k < L.total; ++k) buf[k] = (double)buf32[k];
+    if (cn_x || cn_y || cn_z || c_norm) {   // the records, split into the components (FP32 widened)
+        std::vector<double> rec(4 * (size_t)L.total);
+        if (c->fp32) {
+            std::vector<float> rec32(4 * (size_t)L.total);
+            HIPCHK(c, hipMemcpy(rec32.data(), c->d.cn32, sizeof(float) * rec32.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < rec.size(); ++k) rec[k] = (double)rec32[k];
         } else {
-            HIPCHK(c, hipMemcpy(buf.data(), src[m], sizeof(double) * L.total, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(rec.data(), c->d.cn, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
         }
-        unpack_host(L, m == 4 ? 1 : 2, buf.data(), dst[m]);
+        for (int m = 0; m < 4; ++m) {
+            if (!dst[m]) continue;
+            for (long k = 0; k < L.total; ++k) buf[k] = rec[4 * (size_t)k + m];
+            unpack_host(L, 2, buf.data(), dst[m]);
+        }
+    }
+    if (curv) {
+        HIPCHK(c, hipMemcpy(buf.data(), c->d.curv, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        unpack_host(L, 1, buf.data(), curv);
     }
     return MFLBM_OK;
 }
