@@ -46,6 +46,8 @@ struct Dev {
     // consecutive fluid nodes finds its links of one direction contiguous (index nc + wbase + rank in wave).
     SlotTable S;
     const int* map;   // padded offset -> compact index, -1 where nothing is stored
+    const int* pmap;  // padded offset -> compact index of an interior fluid node, -1 elsewhere (a folded halo node
+                      // maps to its image in map but not here: its phi is the dense one the periodic copy writes)
     const int* cpos;  // compact index -> padded offset
     int nf;           // interior fluid nodes
     int nc;           // all compact nodes
@@ -54,6 +56,10 @@ struct Dev {
     double* force;    // per fluid node: cn_x, cn_y, cn_z, 0.5*gamma*curv*c_norm (stride fs)
     long fs;
     double* phi;
+    // phi at the fluid nodes by compact index, as the lattice update writes it (whole cache lines).  The chain and
+    // the boundary conditions read it through phi_now; the dense phi takes these values before any other reader
+    // (k_phi_dense)
+    double* phic;
     // The normal field, one 32-byte record (cn_x, cn_y, cn_z, c_norm) per dense node: a neighbour's normal is one
     // cache-line access in the stencils that read it (normal extrapolation, curvature)
     double4* cn;
@@ -65,23 +71,21 @@ struct Dev {
     // the reference zeroes the normal (Phase_gradient.F90:63-79): if every phi a node's normal depends on (its
     // stencil, and the fluid neighbours of the solid nodes in it) lies within PHI_BAND of the same +-1, the
     // normal is exactly +0 whatever the values, and so is every normal extrapolated from such normals.
-    //   notm[b], notp[b]: a fluid node of brick b holds a phi farther than PHI_BAND from -1 (from +1); set by
-    //            the lattice update of the current step, read and cleared by the next chain (two buffers in turn)
+    //   notm[b], notp[b]: a fluid node of brick b holds a phi farther than PHI_BAND from -1 (from +1), or b is
+    //            static; set by the lattice update of the current step, read and reset to bstatic by the next
+    //            chain's plan (two buffers in turn)
     //   bstatic[b]: brick b holds a non-solid node outside the interior (halo planes that boundary conditions,
     //            the periodic exchange or nobody write) or an interior node within 4 of a periodic face (whose
     //            phi the periodic exchange copies into the halo): never quiet
-    //   q1b[b]:  every brick within one of b is quiet with one sign: the normals of b are +0 (not evaluated)
-    //   q2b[b]:  q1b of every brick within one of b: the extrapolated normals of b are +0, and no evaluated
-    //            normal reads the extrapolated phi of b's solid nodes (not evaluated; downloads refresh them)
-    //   zfill[b]: the stored normals of b may have a bit set (nzb before this chain): skipped stages write +0
     //   nzb[b]:  a stored normal of brick b has a bit set
     //   nzd[b]:  nzb of brick b or one of its 26 neighbours is set: the lattice update reads the stencil
+    // From them each chain's plan (k_brick_plan) derives, per brick, q1b (every brick within one of b is quiet with
+    // one sign: the normals of b are +0, not evaluated) and q2b (q1b of every brick within one of b: the
+    // extrapolated normals of b are +0, and no evaluated normal reads the extrapolated phi of b's solid nodes; not
+    // evaluated, downloads refresh them), and zero-fills skipped normals that had a bit set (nzb).
     unsigned char* notm;
     unsigned char* notp;
     const unsigned char* bstatic;
-    unsigned char* q1b;
-    unsigned char* q2b;
-    unsigned char* zfill;
     unsigned char* nzb;
     unsigned char* nzd;
     int nbx, nby, nbz;  // bricks per axis of the padded grid
@@ -180,6 +184,20 @@ __device__ __forceinline__ void flag_band(const Dev& d, int b, double v) {
     if (!(fabs(v + 1.0) <= PHI_BAND)) d.notm[b] = 1;
     if (!(fabs(v - 1.0) <= PHI_BAND)) d.notp[b] = 1;
 }
+// current phi of padded node n: the compact value at a fluid node, the dense value elsewhere (one load from the
+// selected address)
+__device__ __forceinline__ double phi_now(const Dev& d, int n) {
+    const int c = d.pmap[n];
+    return *((unsigned)c < (unsigned)d.nf ? d.phic + c : d.phi + n);
+}
+// phi_now around node n, for the stencils
+struct PhiNowAt {
+    const Dev* d;
+    int n;
+    __device__ __forceinline__ double operator()(int a, int b, int c) const {
+        return phi_now(*d, n + a + b * d->L.px + c * d->L.pxy);
+    }
+};
 
 
 // ------------------------------------------------------------------------------------------------------
@@ -280,7 +298,7 @@ k_collide(Dev d, CollideParams P, int per) {
     }
     const double phi = collide_node_tf(a, b, cnx, cny, cnz, tf, P);
     flag_band(d, brick_of(d, nn), phi);
-    d.phi[nn] = phi;
+    d.phic[m] = phi;
     st_pair(S.p[0], m, a[0], b[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {   // odd: slot opp(q) of x + e_q
@@ -353,7 +371,7 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
     }
     const C phi = collide_node_dev<C>(s, a, cnx, cny, cnz, tf, P);
     flag_band(d, brick_of(d, nn), (double)phi);
-    d.phi[nn] = (double)phi;
+    d.phic[m] = (double)phi;
     st_pair32(S.p[0], (unsigned)m, (float)s[0], (float)a[0]);
 #pragma unroll
     for (int q = 1; q < 19; ++q) {
@@ -534,7 +552,7 @@ __global__ void k_wrap_phi(Dev d, int wrap_y, int wrap_z) {
     if (gz && !wrap_z) return;
     const int js = gy ? (j < 1 ? j + L.ny : j - L.ny) : j;
     const int ks = gz ? (k < 1 ? k + L.nz : k - L.nz) : k;
-    d.phi[L.at(i, j, k)] = d.phi[L.at(i, js, ks)];
+    d.phi[L.at(i, j, k)] = phi_now(d, L.at(i, js, ks));
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -672,7 +690,7 @@ __device__ __forceinline__ void bc_outlet_convective_before(const Dev& d, int t,
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
     double* ph = d.phi;
-    const double v = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    const double v = ((d.cphi[pc] + u_convec * phi_now(d, L.at(i, j, nz))) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
     d.cphi[pc] = v;
     ph[L.at(i, j, nz + 1)] = v;
     ph[L.at(i, j, nz + 2)] = v;
@@ -708,7 +726,7 @@ __device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, 
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
     double* ph = d.phi;
-    const double v = ((d.cphi[pc] + u_convec * ph[L.at(i, j, nz)]) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    const double v = ((d.cphi[pc] + u_convec * phi_now(d, L.at(i, j, nz))) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
     d.cphi[pc] = v;
     ph[L.at(i, j, nz + 1)] = v;
     ph[L.at(i, j, nz + 2)] = v;
@@ -739,7 +757,7 @@ __device__ __forceinline__ void bc_outlet_pressure_before(const Dev& d, int t, d
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
     double* ph = d.phi;
-    const double pz = ph[L.at(i, j, nz)];
+    const double pz = phi_now(d, L.at(i, j, nz));
     ph[L.at(i, j, nz + 1)] = pz;
     ph[L.at(i, j, nz + 2)] = pz;
     ph[L.at(i, j, nz + 3)] = pz;
@@ -778,7 +796,7 @@ __device__ __forceinline__ void bc_outlet_pressure_after(const Dev& d, int t, do
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = 1.0 - (double)wi, wd = (double)wi;
     double* ph = d.phi;
-    const double pz = ph[L.at(i, j, nz)];
+    const double pz = phi_now(d, L.at(i, j, nz));
     ph[L.at(i, j, nz + 1)] = pz;
     ph[L.at(i, j, nz + 2)] = pz;
     ph[L.at(i, j, nz + 3)] = pz;
@@ -1038,13 +1056,13 @@ __device__ __forceinline__ double quiet_curvature() { return curvature_f(ZeroAt{
 
 // phi extrapolated into a solid boundary node from its fluid neighbours (Phase_gradient.F90:12-30).  Solid list
 // nodes lie within [-2, n+3]^3, so all 18 neighbours are inside the padded grid: they are loaded unconditionally
-// (one round trip instead of one per direction) and the fluid ones summed in the reference's order.
+// (phi_now: the current phi also where the dense phi lags) and the fluid ones summed in the reference's order.
 __device__ __forceinline__ void phi_extrapolate_at(const Dev& d, const SolidList& s, int t) {
     const int n = s.pos[t];
     const unsigned mk = s.mask[t];
     double v[19];
 #pragma unroll
-    for (int q = 1; q < 19; ++q) v[q] = d.phi[n + d.L.off(q)];
+    for (int q = 1; q < 19; ++q) v[q] = phi_now(d, n + d.L.off(q));
     double acc = 0.0;
 #pragma unroll
     for (int q = 1; q < 19; ++q) acc = (mk & (1u << q)) ? acc + v[q] * lat::W[q] : acc;
@@ -1089,8 +1107,9 @@ __device__ __forceinline__ void wet(double& cx, double& cy, double& cz, double n
 // work list (groups in the XCD-aware block order, so a die works on neighbouring bricks), and the grid is fixed:
 // the list length is known on the device only.
 struct BrickWork {
-    const int4* list;   // bricks to visit: (brick, first entry, end entry, brick quiet)
-    const int* cnt;     // their number
+    const int4* list;   // bricks to visit: (brick, first entry, end entry, brick quiet), NXCD sub-lists of cap items
+    const int* cnt;     // their numbers (NXCD)
+    int cap;
 };
 #ifndef MFLBM_CHAIN_BLOCKS
 #define MFLBM_CHAIN_BLOCKS 2048
@@ -1098,18 +1117,19 @@ struct BrickWork {
 constexpr int CHAIN_BLOCKS = MFLBM_CHAIN_BLOCKS;   // grid of the brick-list kernels: what the GPU holds at once
 constexpr int CHAIN_PER = CHAIN_BLOCKS / NXCD;     // work-groups per XCD
 static_assert(CHAIN_BLOCKS % NXCD == 0, "whole work-groups per XCD");
-// The work items of a group of G lanes.  The list is cut into NXCD contiguous ranges, one per XCD (consecutive
-// items, appended by one plan work-group, are neighbouring bricks: they share the die's L2), and each die's groups
-// take its items in turn.  Work-groups are dispatched to the dies round-robin (blockIdx.x % NXCD).
+// The work items of a group of G lanes.  A work list is kept as NXCD sub-lists (the plan appends a tile's bricks
+// to sub-list tile % NXCD, so no counter takes more than its share of the appends); the groups of XCD x take the
+// items of sub-list x in turn (work-groups are dispatched to the dies round-robin, blockIdx.x % NXCD).
 template <int G>
 struct GroupItems {
     static constexpr int NG = BLOCK / G;
     int i, hi, lane;
-    __device__ __forceinline__ GroupItems(const BrickWork& w) {
-        const int n = *w.cnt;
-        const int x = (int)(blockIdx.x % NXCD);
-        i = (int)(((long)n * x) / NXCD) + (int)(blockIdx.x / NXCD) * NG + (int)threadIdx.x / G;
-        hi = (int)(((long)n * (x + 1)) / NXCD);
+    const int4* list;
+    __device__ __forceinline__ GroupItems(const BrickWork& w, int blk = (int)blockIdx.x) {
+        const int x = blk % NXCD;
+        hi = w.cnt[x];
+        list = w.list + (size_t)x * w.cap;
+        i = (blk / NXCD) * NG + (int)threadIdx.x / G;
         lane = (int)threadIdx.x % G;
     }
     static constexpr int STRIDE = CHAIN_PER * NG;
@@ -1132,7 +1152,7 @@ __device__ __forceinline__ void mark_nonzero(const Dev& d, int b, bool nz, int l
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate_b(Dev d, SolidList s, BrickWork w) {
     GroupItems<16> g(w);
     for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
-        const int4 it = w.list[g.i];
+        const int4 it = g.list[g.i];
         for (int t = it.y + g.lane; t < it.z; t += 16) phi_extrapolate_at(d, s, t);
     }
 }
@@ -1148,7 +1168,7 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     const Layout L = d.L;
     Rec4<NT>* const N = normals<NT>(d);
     for (; g.i < g.hi; g.i += GroupItems<64>::STRIDE) {
-        const int4 it = w.list[g.i];
+        const int4 it = g.list[g.i];
         const int t = it.y + g.lane;
         bool nz = false;
         if (t < it.z) {
@@ -1156,9 +1176,10 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
             if (it.w) {
                 N[n] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
             } else {
-                double cx = dx4(d.phi, n, L.px, L.pxy);
-                double cy = dy4(d.phi, n, L.px, L.pxy);
-                double cz = dz4(d.phi, n, L.px, L.pxy);
+                const PhiNowAt P{&d, n};
+                double cx = dx4f(P);
+                double cy = dy4f(P);
+                double cz = dz4f(P);
                 double cn;
                 normalise(cx, cy, cz, cn);
                 const int fb = wb[t];
@@ -1180,7 +1201,7 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     GroupItems<16> g(w);
     Rec4<NT>* const N = normals<NT>(d);
     for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
-        const int4 it = w.list[g.i];
+        const int4 it = g.list[g.i];
         bool nz = false;
         for (int t = it.y + g.lane; t < it.z; t += 16) {
             const int n = s.pos[t];
@@ -1219,17 +1240,23 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     }
 }
 // The plan of one chain.  A work-group owns TBX x TBY x TBZ bricks and stages the band flags of them and a halo
-// of two bricks in LDS.  From the band flags the lattice update left (and bstatic) it derives q1b (every brick
-// within one quiet with one sign; no brick when `full`) over the tile and a one-brick halo, then q2b (q1b over the
-// 27-brick neighbourhood); zfill := nzb; nzb, nzd and the next band buffers are cleared (the chain stages set nzb
-// and nzd again where they store a normal with a bit set).  The bricks each stage visits are appended to its work
-// list with their entry range: normals where the brick has normal nodes and (!q1b or zfill), normal extrapolation
-// where it has solid nodes within one halo layer and (!q2b or zfill), phi extrapolation where it has solid nodes
-// and !q2b.  The other set of list counters is cleared for the next chain.
+// of two bricks in LDS.  From the band flags the lattice update left it derives q1b (every brick within one quiet
+// with one sign; no brick when `full`) over the tile and a one-brick halo, then q2b (q1b over the 27-brick
+// neighbourhood).  nzb and nzd are cleared (the chain stages set them again where they store a normal with a bit
+// set) and the next band buffers reset to bstatic.  The bricks each stage visits are appended to its work list with
+// their entry range: normals where the brick has normal nodes and (!q1b or zfill), normal extrapolation where it
+// has solid nodes within one halo layer and (!q2b or zfill), phi extrapolation where it has solid nodes and !q2b;
+// zfill is nzb before the clearing: a skipped stage writes +0 where a stored normal has a bit set.  The other set
+// of list counters is cleared for the next chain.
+// IN != 0: the first 2 gface work-groups run the inlet and outlet conditions of the step instead (k_open_faces):
+// they touch no brick flag and read interior phi through phi_now, and the phi they write in the halo planes is
+// read only by the chain stages after the plan, so the latency-bound face work overlaps the plan.  ntx, nty: tiles
+// per axis.
 constexpr int TBX = 8, TBY = 8, TBZ = 4;                     // TBX * TBY * TBZ == BLOCK
 constexpr int TSX = TBX + 2, TSY = TBY + 2, TSZ = TBZ + 2;   // q1b region: the tile and a one-brick halo
 constexpr int TUX = TBX + 4, TUY = TBY + 4, TUZ = TBZ + 4;   // band region: the tile and a two-brick halo
 static_assert(TBX * TBY * TBZ == BLOCK, "one brick per thread");
+constexpr int NWORK = 3;   // work lists: normals, normal extrapolation, phi extrapolation
 struct ChainPlan {
     const unsigned char* notm;   // band flags of the lattice update since the last chain
     const unsigned char* notp;
@@ -1239,21 +1266,17 @@ struct ChainPlan {
     const int* off_n;            // brick offsets of the sorted lists (nbricks + 1 each)
     const int* off_s;
     const int* off_1;
-    int4* list_n;                // work lists and their counters
-    int4* list_x;
-    int4* list_p;
-    int* cnt;                    // normals, normal extrapolation, phi extrapolation
+    int4* list[NWORK];           // work lists (NXCD sub-lists of cap items each) and their counters (NXCD each)
+    int cap;
+    int* cnt;
     int* cnt_next;               // the counters of the next chain
 };
-// IN != 0: the first 2 gface work-groups run the inlet and outlet conditions of the step instead (k_open_faces):
-// they touch no brick flag, and the phi they write in the halo planes is read only by the chain stages after the
-// plan, so the latency-bound face work overlaps the plan.  ntx, nty: tiles per axis.
 template <class T, int IN, int OUT, bool AFTER>
 __global__ void __launch_bounds__(BLOCK) k_brick_plan(Dev d, ChainPlan p, int full, FaceArgs a, int gface, int ntx, int nty) {
     __shared__ unsigned char band[TUX * TUY * TUZ];
     __shared__ unsigned char q1s[TSX * TSY * TSZ];
-    __shared__ int wcount[3][BLOCK / 64];
-    __shared__ int wbase[3];
+    __shared__ int wcount[NWORK][BLOCK / 64];
+    __shared__ int wbase[NWORK];
     int tile = (int)blockIdx.x;
     if (IN != 0) {
         if (tile < 2 * gface) {
@@ -1263,41 +1286,51 @@ __global__ void __launch_bounds__(BLOCK) k_brick_plan(Dev d, ChainPlan p, int fu
         tile -= 2 * gface;
     }
     const int bx0 = (tile % ntx) * TBX, by0 = ((tile / ntx) % nty) * TBY, bz0 = (tile / (ntx * nty)) * TBZ;
-    for (int t = threadIdx.x; t < TUX * TUY * TUZ; t += BLOCK) {
-        const int x = bx0 + t % TUX - 2, y = by0 + (t / TUX) % TUY - 2, z = bz0 + t / (TUX * TUY) - 2;
-        unsigned char v = 0;
-        if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) {
-            const int c = x + d.nbx * (y + d.nby * z);
-            const bool st = d.bstatic[c] != 0;
-            v = (unsigned char)((st || p.notm[c] ? 1 : 0) | (st || p.notp[c] ? 2 : 0));
+    auto inside = [&](int x, int y, int z) { return x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz; };
+    const int tx = threadIdx.x % TBX, ty = (threadIdx.x / TBX) % TBY, tz = threadIdx.x / (TBX * TBY);
+    const int x = bx0 + tx, y = by0 + ty, z = bz0 + tz;
+    const bool valid = x < d.nbx && y < d.nby && z < d.nbz;
+    const int b = valid ? x + d.nbx * (y + d.nby * z) : 0;
+    // this brick's own flags, loaded with the staging
+    const unsigned char zf = valid ? d.nzb[b] : 0, h = valid ? p.hasb[b] : 0, st = valid ? d.bstatic[b] : 0;
+    {   // every load of the staging issued before the first is used
+        constexpr int NSTAGE = (TUX * TUY * TUZ + BLOCK - 1) / BLOCK;
+        unsigned char vm[NSTAGE], vp[NSTAGE];
+#pragma unroll
+        for (int k = 0; k < NSTAGE; ++k) {
+            const int t = threadIdx.x + k * BLOCK;
+            const int xx = bx0 + t % TUX - 2, yy = by0 + (t / TUX) % TUY - 2, zz = bz0 + t / (TUX * TUY) - 2;
+            const bool in = t < TUX * TUY * TUZ && inside(xx, yy, zz);
+            const int c = in ? xx + d.nbx * (yy + d.nby * zz) : 0;
+            vm[k] = in ? p.notm[c] : 0;
+            vp[k] = in ? p.notp[c] : 0;
         }
-        band[t] = v;
+#pragma unroll
+        for (int k = 0; k < NSTAGE; ++k) {
+            const int t = threadIdx.x + k * BLOCK;
+            if (t < TUX * TUY * TUZ) band[t] = (unsigned char)((vm[k] ? 1 : 0) | (vp[k] ? 2 : 0));
+        }
     }
     __syncthreads();
     // q1b of the region brick (lx - 1, ly - 1, lz - 1) of the tile: its neighbourhood is band (lx..lx+2, ...)
     for (int t = threadIdx.x; t < TSX * TSY * TSZ; t += BLOCK) {
         const int lx = t % TSX, ly = (t / TSX) % TSY, lz = t / (TSX * TSY);
-        const int x = bx0 + lx - 1, y = by0 + ly - 1, z = bz0 + lz - 1;
         unsigned char q = 1;   // outside the grid: quiet
-        if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) {
-            unsigned char a = 0;
+        if (inside(bx0 + lx - 1, by0 + ly - 1, bz0 + lz - 1)) {
+            unsigned char o = 0;
 #pragma unroll
             for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-                    for (int dx = 0; dx < 3; ++dx) a |= band[(lx + dx) + TUX * ((ly + dy) + TUY * (lz + dz))];
-            const bool m = full != 0 || (a & 1), pl = full != 0 || (a & 2);
+                    for (int dx = 0; dx < 3; ++dx) o |= band[(lx + dx) + TUX * ((ly + dy) + TUY * (lz + dz))];
+            const bool m = full != 0 || (o & 1), pl = full != 0 || (o & 2);
             q = (!m || !pl) ? 1 : 0;
         }
         q1s[t] = q;
     }
     __syncthreads();
-    const int tx = threadIdx.x % TBX, ty = (threadIdx.x / TBX) % TBY, tz = threadIdx.x / (TBX * TBY);
-    const int x = bx0 + tx, y = by0 + ty, z = bz0 + tz;
-    const bool valid = x < d.nbx && y < d.nby && z < d.nbz;
-    const int b = valid ? x + d.nbx * (y + d.nby * z) : 0;
-    bool take[3] = {false, false, false};
+    bool take[NWORK] = {false, false, false};
     unsigned char q1 = 0, q2 = 0;
     if (valid) {
         const int r = tx + TSX * (ty + TSY * tz);   // this brick's neighbourhood corner in the q1b region
@@ -1309,45 +1342,50 @@ __global__ void __launch_bounds__(BLOCK) k_brick_plan(Dev d, ChainPlan p, int fu
             for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                 for (int dx = 0; dx < 3; ++dx) q2 &= q1s[r + dx + TSX * (dy + TSY * dz)];
-        const unsigned char zf = d.nzb[b];
-        const unsigned char h = p.hasb[b];
-        d.q1b[b] = q1;
-        d.q2b[b] = q2;
-        d.zfill[b] = zf;
         d.nzb[b] = 0;
         d.nzd[b] = 0;
-        p.next_m[b] = 0;
-        p.next_p[b] = 0;
+        p.next_m[b] = st;   // the next lattice update adds its band flags to the static ones
+        p.next_p[b] = st;
         take[0] = (h & 1) && (!q1 || zf);
         take[1] = (h & 4) && (!q2 || zf);
         take[2] = (h & 2) && !q2;
     }
-    // one atomic per list and work-group: wave counts, then the group's base, then each brick's slot
+    // one atomic per list and work-group (on the tile's sub-list): wave counts, the group's base, each brick's slot
     const int lane = (int)(threadIdx.x & 63u), wv = (int)threadIdx.x >> 6;
-    unsigned long long m[3];
+    unsigned long long m[NWORK];
 #pragma unroll
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < NWORK; ++l) {
         m[l] = __ballot(take[l]);
         if (lane == 0) wcount[l][wv] = __popcll(m[l]);
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    const int sub = tile % NXCD;
+    if (threadIdx.x < NWORK) {
         int tot = 0;
         for (int v = 0; v < BLOCK / 64; ++v) tot += wcount[threadIdx.x][v];
-        wbase[threadIdx.x] = tot ? atomicAdd(p.cnt + threadIdx.x, tot) : 0;
+        wbase[threadIdx.x] = tot ? atomicAdd(p.cnt + threadIdx.x * NXCD + sub, tot) : 0;
     }
     __syncthreads();
-    int4* const lists[3] = {p.list_n, p.list_x, p.list_p};
-    const int* const offs[3] = {p.off_n, p.off_1, p.off_s};
-    const unsigned char quiet[3] = {q1, q2, 0};
+    const int* const offs[NWORK] = {p.off_n, p.off_1, p.off_s};
+    const unsigned char quiet[NWORK] = {q1, q2, 0};
 #pragma unroll
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < NWORK; ++l) {
         if (!take[l]) continue;
         int slot = wbase[l] + __popcll(m[l] & ((1ull << lane) - 1ull));
         for (int v = 0; v < wv; ++v) slot += wcount[l][v];
-        lists[l][slot] = int4{b, offs[l][b], offs[l][b + 1], (int)quiet[l]};
+        p.list[l][(size_t)sub * p.cap + slot] = int4{b, offs[l][b], offs[l][b + 1], (int)quiet[l]};
     }
-    if (tile == 0 && threadIdx.x < 3) p.cnt_next[threadIdx.x] = 0;
+    if (tile == 0 && threadIdx.x < NWORK * NXCD) p.cnt_next[threadIdx.x] = 0;
+}
+// the whole dense phi at the fluid nodes from the compact phi (before a reader outside the chain)
+__global__ void k_phi_dense(Dev d) {
+    const int m = blockIdx.x * BLOCK + threadIdx.x;
+    if (m < d.nf) d.phi[d.cpos[m]] = d.phic[m];
+}
+// the compact phi from the dense phi (after an upload or a device initialisation)
+__global__ void k_phic_gather(Dev d) {
+    const int m = blockIdx.x * BLOCK + threadIdx.x;
+    if (m < d.nf) d.phic[m] = d.phi[d.cpos[m]];
 }
 // component C (0 x, 1 y, 2 z) of the normal records around node n
 template <class NT, int C>
@@ -1770,9 +1808,13 @@ struct mflbm_ctx {
     double* fields = nullptr;   // phi, the normal records (4 arrays' worth), curv, u, v, w, rho, phi_old
     float* fields32 = nullptr;  // FP32 storage modes: the normal records in FP32
     int* role = nullptr;        // dense: fluid boundary index (>= 0) or -(extrapolation list index) - 2, else -1
-    unsigned char* qbuf = nullptr;   // brick flags: notm, notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd, hasb
-    int* cbuf = nullptr;             // chain lists: 3 work lists (nbricks int4 items each), the brick offsets of the
-                                     // 3 sorted lists (nbricks + 1 each), 2 sets of 3 work-list counters
+    unsigned char* qbuf = nullptr;   // brick flags: notm, notp (two buffers each), bstatic, nzb, nzd, hasb
+    int* cbuf = nullptr;             // chain lists: NWORK work lists (NXCD sub-lists of lcap int4 items each), the
+                                     // brick offsets of the 3 sorted lists (nbricks + 1 each), 2 sets of NWORK x NXCD
+                                     // counters
+    int lcap = 0;
+    double* phic = nullptr;          // compact phi (Dev::phic)
+    bool phi_stale = false;          // the dense phi at fluid nodes may lag the compact phi (outside the chain's bricks)
     long nbricks = 0;
     int band_cur = 0;                // band buffers the lattice update fills until the next chain
     bool full_chain = true;          // the next chain evaluates every brick (new geometry or state)
@@ -1792,7 +1834,7 @@ struct mflbm_ctx {
     void* compact = nullptr;    // 38 population slices (compact slots + link slots), FP64 or FP32
     bool fp32 = false;          // precision 1 or 2: FP32 storage (s, a) of collide.h
     double* force = nullptr;    // 4 compact force arrays
-    int* map = nullptr;
+    int* map = nullptr;         // Dev::map, then Dev::pmap (2 x L.total)
     int* ibuf = nullptr;        // cpos, normal list, solid lists, fluid list positions
     int* wbuf = nullptr;        // per normal-list node: index into the fluid boundary list or -1
     unsigned* mbuf = nullptr;   // solid list masks
@@ -1830,8 +1872,10 @@ Fold fold_of(const mflbm_params& p) {
 // work list of one chain stage (0 normals, 1 normal extrapolation, 2 phi extrapolation) and its counter in set `set`
 BrickWork brick_work(const mflbm_ctx* c, int work, int set) {
     const long nb = c->nbricks;
+    const size_t lw = (size_t)NXCD * c->lcap;   // items per work list
     const int4* lists = (const int4*)c->cbuf;
-    return BrickWork{lists + (size_t)work * nb, c->cbuf + 12 * (size_t)nb + 3 * (size_t)(nb + 1) + 3 * set + work};
+    return BrickWork{lists + (size_t)work * lw,
+                     c->cbuf + 4 * NWORK * lw + 3 * (size_t)(nb + 1) + NXCD * (NWORK * set + work), c->lcap};
 }
 #define CHAIN_LAUNCH(c, kern, ...)                                                                           \
     do {                                                                                                     \
@@ -1845,6 +1889,13 @@ template <class NT>
 int launch_normals_listed(mflbm_ctx* c, int set) {
     CHAIN_LAUNCH(c, k_normals<NT>, c->d, c->nrm, (const int*)c->wbuf, c->fl, brick_work(c, 0, set));
     CHAIN_LAUNCH(c, k_normal_extrapolate<NT>, c->d, c->sl1, brick_work(c, 1, set));
+    return MFLBM_OK;
+}
+
+// the dense phi at every fluid node (the lattice update writes the compact phi; the chain keeps no dense copy)
+int ensure_dense_phi(mflbm_ctx* c) {
+    if (c->phi_stale) LAUNCH(c, k_phi_dense, nblocks(c->d.nf), c->d);
+    c->phi_stale = false;
     return MFLBM_OK;
 }
 
@@ -1890,17 +1941,18 @@ int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
         p.notp = c->qbuf + (size_t)(2 + set) * nb;
         p.next_m = c->qbuf + (size_t)(set ^ 1) * nb;
         p.next_p = c->qbuf + (size_t)(2 + (set ^ 1)) * nb;
-        p.hasb = c->qbuf + 10 * (size_t)nb;
-        p.off_n = c->cbuf + 12 * nb;
-        p.off_s = c->cbuf + 12 * nb + (nb + 1);
-        p.off_1 = c->cbuf + 12 * nb + 2 * (nb + 1);
+        p.hasb = c->qbuf + 7 * (size_t)nb;
+        const size_t lw = (size_t)NXCD * c->lcap;
+        const int* offs = c->cbuf + 4 * NWORK * lw;
+        p.off_n = offs;
+        p.off_s = offs + (nb + 1);
+        p.off_1 = offs + 2 * (nb + 1);
         int4* lists = (int4*)c->cbuf;
-        p.list_n = lists;
-        p.list_x = lists + nb;
-        p.list_p = lists + 2 * nb;
-        int* cnts = c->cbuf + 12 * (size_t)nb + 3 * (size_t)(nb + 1);
-        p.cnt = cnts + 3 * set;
-        p.cnt_next = cnts + 3 * (set ^ 1);
+        for (int l = 0; l < NWORK; ++l) p.list[l] = lists + (size_t)l * lw;
+        p.cap = c->lcap;
+        int* cnts = c->cbuf + 4 * NWORK * lw + 3 * (size_t)(nb + 1);
+        p.cnt = cnts + NWORK * NXCD * set;
+        p.cnt_next = cnts + NWORK * NXCD * (set ^ 1);
         // no brick quiet after new geometry or state, and none with the fused chain, which does not use the flags
         const int full = (c->full_chain || c->chain_fused) ? 1 : 0;
         const int rp = c->fp32 ? launch_plan<float>(c, p, full, faces) : launch_plan<double>(c, p, full, faces);
@@ -1912,7 +1964,9 @@ int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
         c->solid_zeroed = false;
     }
     CHAIN_LAUNCH(c, k_phi_extrapolate_b, c->d, c->sl, brick_work(c, 2, set));
-    if (c->chain_fused) {
+    if (c->chain_fused) {   // (its LDS planes are loaded from the dense phi)
+        const int re = ensure_dense_phi(c);
+        if (re) return re;
         const Layout& L = c->L;
         const int ntx = (L.nx + CT - 1) / CT, nty = (L.ny + CT - 1) / CT, nseg = (L.nz + CSEG - 1) / CSEG;
         const long nblk = (long)ntx * nty * nseg;
@@ -1934,6 +1988,7 @@ int launch_faces(mflbm_ctx* c, bool odd, FaceWork* defer);
 
 int launch_step(mflbm_ctx* c, int ntime) {
     c->stepped = true;
+    c->phi_stale = true;
     const mflbm_params& p = c->p;
     const CollideParams P = collide_params(p);
     const bool odd = (ntime % 2) != 0;
@@ -2057,6 +2112,8 @@ int gather_state(mflbm_ctx* c) {
     LAUNCH(c, (k_links<false, T>), nblocks(c->d.nf), c->d, c->dense_pdf, c->ps);
     if (c->fold.y || c->fold.z)
         LAUNCH(c, k_fold_upload<T>, fold_blocks(c->L), c->d, c->fold, (const double*)c->dense_pdf, c->ps);
+    LAUNCH(c, k_phic_gather, nblocks(c->d.nf), c->d);
+    c->phi_stale = false;
     c->stepped = false;
     return MFLBM_OK;
 }
@@ -2068,7 +2125,7 @@ int scatter_state(mflbm_ctx* c) {
 }
 
 void free_geometry(mflbm_ctx* c) {
-    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role, c->qbuf, c->cbuf};
+    void* bufs[] = {c->compact, c->force, c->map, c->ibuf, c->wbuf, c->mbuf, c->dbuf, c->wmask, c->wbase, c->role, c->qbuf, c->cbuf, c->phic};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     c->compact = nullptr;
@@ -2083,6 +2140,8 @@ void free_geometry(mflbm_ctx* c) {
     c->role = nullptr;
     c->qbuf = nullptr;
     c->cbuf = nullptr;
+    c->phic = nullptr;
+    c->d.phic = nullptr;
     c->have_geom = false;
     c->have_state = false;
 }
@@ -2320,7 +2379,7 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     const size_t nd = (size_t)ns + s1w.size() + fd.size() + 1;
     HIPCHK(c, hipMalloc(&c->compact, esz * soff[19]));
     HIPCHK(c, hipMalloc(&c->force, sizeof(double) * fs * 4));
-    HIPCHK(c, hipMalloc(&c->map, sizeof(int) * L.total));
+    HIPCHK(c, hipMalloc(&c->map, 2 * sizeof(int) * L.total));   // map, then pmap
     HIPCHK(c, hipMalloc(&c->ibuf, sizeof(int) * ni));
     HIPCHK(c, hipMalloc(&c->wbuf, sizeof(int) * (wb.size() + 1)));
     if (!wb.empty()) HIPCHK(c, hipMemcpy(c->wbuf, wb.data(), sizeof(int) * wb.size(), hipMemcpyHostToDevice));
@@ -2352,10 +2411,15 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     hd.insert(hd.end(), fd.begin(), fd.end());
     hd.push_back(0.0);
     HIPCHK(c, hipMemcpy(c->map, cidx.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
+    {
+        std::vector<int> pm(L.total, -1);
+        for (int m = 0; m < nfl; ++m) pm[cpos[m]] = m;
+        HIPCHK(c, hipMemcpy(c->map + L.total, pm.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMalloc(&c->role, sizeof(int) * L.total));
     HIPCHK(c, hipMemcpy(c->role, role.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
-    // brick flags (Dev): notm and notp (two buffers each), bstatic, q1b, q2b, zfill, nzb, nzd.  The band flags,
-    // zfill, nzb and nzd start set, so the first chain and lattice update evaluate everything; bstatic from the
+    // brick flags (Dev): notm and notp (two buffers each), bstatic, nzb, nzd.  The band flags, nzb and nzd start
+    // set, so the first chain and lattice update evaluate everything; bstatic from the
     // walls: bricks with a non-solid node outside the interior, and bricks with an interior node within 4 of a
     // periodic face
     std::vector<unsigned char> hstatic(nbr, 0);
@@ -2375,12 +2439,15 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     }
     // chain lists: the brick offsets of the sorted lists, which of them a brick has entries in (hasb), and the
     // work lists with their counters (zero: the first plan fills set 0)
-    std::vector<int> hc(12 * nbr + 3 * (nbr + 1) + 6, 0);
+    const int ntiles = ((nbx + TBX - 1) / TBX) * ((nby + TBY - 1) / TBY) * ((nbz + TBZ - 1) / TBZ);
+    const int lcap = (ntiles + NXCD - 1) / NXCD * BLOCK;   // items per sub-list: the bricks of its tiles
+    const size_t lw = (size_t)NXCD * lcap;
+    std::vector<int> hc(4 * NWORK * lw + 3 * (nbr + 1) + 2 * NWORK * NXCD, 0);
     std::vector<unsigned char> hasb(nbr, 0);
     {
         const std::vector<int>* lists[3] = {&nrm, &spos, &s1pos};
         for (int l = 0; l < 3; ++l) {
-            int* off = hc.data() + 12 * nbr + (size_t)l * (nbr + 1);
+            int* off = hc.data() + 4 * NWORK * lw + (size_t)l * (nbr + 1);
             for (int o : *lists[l]) ++off[brick(o) + 1];
             for (size_t b = 0; b < nbr; ++b) {
                 if (off[b + 1] > 64) return fail(c, MFLBM_ERR_ARG, "brick with more than 64 chain entries");
@@ -2389,13 +2456,17 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
             }
         }
     }
-    HIPCHK(c, hipMalloc(&c->qbuf, 11 * nbr));
-    HIPCHK(c, hipMemset(c->qbuf, 1, 10 * nbr));
+    HIPCHK(c, hipMalloc(&c->qbuf, 8 * nbr));
+    HIPCHK(c, hipMemset(c->qbuf, 1, 7 * nbr));
     HIPCHK(c, hipMemcpy(c->qbuf + 4 * nbr, hstatic.data(), nbr, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->qbuf + 10 * nbr, hasb.data(), nbr, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->qbuf + 7 * nbr, hasb.data(), nbr, hipMemcpyHostToDevice));
     HIPCHK(c, hipMalloc(&c->cbuf, sizeof(int) * hc.size()));
     HIPCHK(c, hipMemcpy(c->cbuf, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->phic, sizeof(double) * ((size_t)nfl + 1)));
+    HIPCHK(c, hipMemset(c->phic, 0, sizeof(double) * ((size_t)nfl + 1)));
+    c->phi_stale = false;
     c->nbricks = (long)nbr;
+    c->lcap = lcap;
     c->band_cur = 0;
     c->full_chain = true;
     c->solid_zeroed = false;
@@ -2412,6 +2483,8 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
         d.S.p[19 + q] = (double*)((char*)c->compact + esz * (soff[q] + 1));
     }
     d.map = c->map;
+    d.pmap = c->map + L.total;
+    d.phic = c->phic;
     d.cpos = c->ibuf;
     d.nf = nfl;
     d.nc = nc;
@@ -2422,11 +2495,8 @@ extern "C" int mflbm_upload_geometry(mflbm_ctx* c, const int8_t* walls_h2, int32
     d.notm = c->qbuf;
     d.notp = c->qbuf + 2 * nbr;
     d.bstatic = c->qbuf + 4 * nbr;
-    d.q1b = c->qbuf + 5 * nbr;
-    d.q2b = c->qbuf + 6 * nbr;
-    d.zfill = c->qbuf + 7 * nbr;
-    d.nzb = c->qbuf + 8 * nbr;
-    d.nzd = c->qbuf + 9 * nbr;
+    d.nzb = c->qbuf + 5 * nbr;
+    d.nzd = c->qbuf + 6 * nbr;
     d.nbx = nbx;
     d.nby = nby;
     d.nbz = nbz;
@@ -2510,6 +2580,8 @@ extern "C" int mflbm_init_state_device(mflbm_ctx* c, int32_t option, double inte
 // pass has run since that chain (compute_macro_vars, Misc.F90:423)
 int refresh_solid_phi(mflbm_ctx* c) {
     if (!c->have_geom || !c->have_state) return MFLBM_OK;
+    const int re = ensure_dense_phi(c);
+    if (re) return re;
     LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
     if (c->solid_zeroed) LAUNCH(c, k_zero_solid_phi, nblocks((long)c->L.nx * c->L.ny * c->L.nz), c->d);
     return MFLBM_OK;
@@ -2577,6 +2649,8 @@ extern "C" int mflbm_run(mflbm_ctx* c, int32_t ntime0, int32_t nsteps) {
 extern "C" int mflbm_compute_macro(mflbm_ctx* c) {
     if (!c) return MFLBM_ERR_ARG;
     if (!c->have_state) return fail(c, MFLBM_ERR_STATE, "no state uploaded");
+    const int re = ensure_dense_phi(c);   // the monitor and the downloads read the dense phi
+    if (re) return re;
     // the force input of the macroscopic velocity comes from the current normal field
     // (the fused chain keeps the force arrays current; the list chain leaves only the dense normals)
     if (!c->chain_fused) {
@@ -2666,6 +2740,8 @@ extern "C" int mflbm_download_gradient(mflbm_ctx* c, double* cn_x, double* cn_y,
     // reference holds is evaluated from the current normals on request
     if (c->chain_fused) {   // the fused chain does not keep the dense normal field: evaluate it now, over the
                             // lists of the last chain (every brick: the fused chain plans a full chain)
+        const int re = ensure_dense_phi(c);
+        if (re) return re;
         const int set = c->band_cur ^ 1;
         const int rc = c->fp32 ? launch_normals_listed<float>(c, set) : launch_normals_listed<double>(c, set);
         if (rc) return rc;
