@@ -36,7 +36,8 @@ constexpr int NXCD = 8;   // gfx950: 8 accelerator complex dies, work-groups dis
 
 // Logical block index with each XCD taking one contiguous range of blocks: node lists are in x-fastest
 // order, so a die then works on a z-slab and stencil neighbours stay in its own L2.  The grid is launched
-// as NXCD * per blocks.
+// as NXCD * per blocks.  (The lattice updates use the plain block order instead: with every die streaming its own
+// distant range, they ran 2 % slower at 256^3, the populations having no reuse to gain from the L2.)
 __device__ __forceinline__ int xcd_block(int per) { return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD); }
 
 inline unsigned nblocks(long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }

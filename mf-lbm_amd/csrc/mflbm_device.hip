@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(2, 8)))
 k_collide(Dev d, CollideParams P, int per) {
     const SlotTable& S = d.S;
-    const int m = xcd_block(per) * BLOCK + threadIdx.x;
+    const int m = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
     double cnx, cny, cnz, tf;
@@ -314,7 +314,7 @@ k_collide(Dev d, CollideParams P, int per) {
 template <bool ODD, class C, bool STENCIL>
 __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> P, double gamma, int per) {
     const SlotTable& S = d.S;
-    const int m = xcd_block(per) * BLOCK + threadIdx.x;
+    const int m = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
     if (m >= d.nf) return;
     const int nn = d.cpos[m];
     C cnx, cny, cnz, tf;

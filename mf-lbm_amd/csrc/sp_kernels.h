@@ -128,9 +128,10 @@ __device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollide
     }
 }
 
+// block order: plain for FP64 storage, XCD-aware for FP32 storage (each measured the faster at 256^3, by 2 %)
 template <bool ODD, class T, class C>
 __global__ void __launch_bounds__(BLOCK) k_sp_collide(SDev<T> d, SPCollideParamsT<C> P, int per) {
-    const int m = xcd_block(per) * BLOCK + threadIdx.x;
+    const int m = (sizeof(T) == sizeof(double) ? (int)blockIdx.x : xcd_block(per)) * BLOCK + (int)threadIdx.x;
     if (m >= d.nf) return;
     sp_update_node<ODD, T, C>(d, P, m);
 }
