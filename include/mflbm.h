@@ -310,7 +310,8 @@ int mflbm_spb_download_macro(mflbm_spb_ctx* ctx, int32_t sample, double* u, doub
 int mflbm_nn_weights_to_bf16(const float* w, void* wf, void* wb, int32_t cin, int32_t cout, void* stream);
 int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* bias, void* y, int32_t n, int32_t d, int32_t h,
                         int32_t w, int32_t cin, int32_t cout, int32_t relu, void* stream);
-/* accumulates the weight gradient (FP32, rows of 27 cin) and, when dbias is not NULL, the bias gradient */
+/* accumulates the weight gradient (FP32, rows of 27 cin) and, when dbias is not NULL, the bias gradient
+ * (fewer than 2^31 positions n d h w, else MFLBM_ERR_ARG) */
 int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
                        int32_t w, int32_t cin, int32_t cout, void* stream);
 /* the same gradients summed in a fixed order, so that repeated calls give identical bits: the positions are cut

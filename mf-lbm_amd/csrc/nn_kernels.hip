@@ -124,75 +124,103 @@ __global__ void __launch_bounds__(NT) k_conv_fwd(const __bf16* __restrict__ x, c
 }
 
 // Weight gradient: dw[co][k] += sum_p dy[p][co] * x[p + off(k)][ci(k)] (rows of 27 cin, the layout of the FP32
-// master weights), the positions split into chunks of `chunk` (a multiple of 32) over grid z; one wave computes
-// 32 output channels x 32 k.  The chunk sums go to dw with FP32 atomics, or (ORDERED) are stored to the
-// chunk's own slice dw + z * cout * K, every element written by exactly one lane, for a fixed-order sum after.
+// master weights), the positions split into chunks of `chunk` (a multiple of 32) over grid z.  A work-group computes
+// 32 output channels x 128 k (one wave per 32 k) and walks its chunk 32 positions at a time: the dy tile (32
+// positions x 32 channels) and the im2col tile of x (32 positions x 128 k) are loaded with 16-byte accesses (8
+// channels of one position) and stored transposed in LDS, two buffers in turn (the next tile's loads are in flight
+// while the current one is multiplied), and each wave reads its fragments from there.  The fragments, and so the
+// sums, are those of a lane-by-lane gather: lane l takes 8 consecutive positions for one channel (A) or one k (B).
+// The chunk sums go to dw with FP32 atomics, or (ORDERED) are stored to the chunk's own slice dw + z * cout * K,
+// every element written by exactly one lane, for a fixed-order sum after.
+constexpr int DW_KT = 128;          // k per work-group
+constexpr int DW_LD = 40;           // LDS row length (32 positions + 8: 80-byte rows keep 16-byte fragment reads aligned)
 template <bool ORDERED>
 __global__ void __launch_bounds__(NT) k_conv_dw(const __bf16* __restrict__ x, const __bf16* __restrict__ dy,
                                                 float* __restrict__ dw, Shape S, int cin, int cout, long chunk) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][32][DW_LD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][DW_KT][DW_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r = lane & 15, q = lane >> 4;
     const int co0 = blockIdx.x * 32;
-    const int kk0 = (blockIdx.y * 4 + wv) * 32;
+    const int kb0 = blockIdx.y * DW_KT;
     const int K = 27 * cin;
-    if (kk0 >= K) return;
     const long M = S.count();
     const long pbeg = (long)blockIdx.z * chunk;
     const long pend = pbeg + chunk < M ? pbeg + chunk : M;
-    // this lane's B columns (k = kk0 + g*16 + r) -> offset and channel
-    int bdz[2], bdy[2], bdx[2], bci[2];
-    bool bv[2];
+    const int ntile = (int)((pend - pbeg + 31) / 32);
+    const bf16x8 zero = {};
+    // this thread's loads of a tile: one dy chunk (threads < 128: position tid / 4, channels 8 (tid % 4) ..) and two
+    // x chunks (item u = tid + 256 v: position u / 16, k 8 (u % 16) .. of the work-group's 128)
+    auto load = [&](long pk, bf16x8& ra, bf16x8 (&rb)[2]) {
+        ra = zero;
+        if (tid < 128) {
+            const long p = pk + (tid >> 2);
+            const int co = co0 + 8 * (tid & 3);
+            if (p < pend && co < cout) ra = *(const bf16x8*)(dy + p * cout + co);
+        }
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-        const int kcol = kk0 + g * 16 + r;
-        bv[g] = kcol < K;
-        const int off = bv[g] ? kcol / cin : 0;
-        bci[g] = bv[g] ? kcol - off * cin : 0;
-        bdz[g] = off / 9 - 1;
-        bdy[g] = (off / 3) % 3 - 1;
-        bdx[g] = off % 3 - 1;
-    }
+        for (int v = 0; v < 2; ++v) {
+            const int u = tid + NT * v;
+            const long p = pk + (u >> 4);
+            const int k = kb0 + 8 * (u & 15);
+            rb[v] = zero;
+            if (p < pend && k < K) {
+                const int off = k / cin, ci = k - off * cin;
+                int t = (int)p;   // (positions are below 2^31: 32-bit decode)
+                const int xq = t % S.w;
+                t /= S.w;
+                const int yq = t % S.h;
+                t /= S.h;
+                const int zq = t % S.d;
+                const int nq = t / S.d;
+                const int zz = zq + off / 9 - 1, yy = yq + (off / 3) % 3 - 1, xx = xq + off % 3 - 1;
+                if (zz >= 0 && zz < S.d && yy >= 0 && yy < S.h && xx >= 0 && xx < S.w)
+                    rb[v] = *(const bf16x8*)(x + ((((long)nq * S.d + zz) * S.h + yy) * S.w + xx) * cin + ci);
+            }
+        }
+    };
+    auto store = [&](int buf, const bf16x8& ra, const bf16x8 (&rb)[2]) {
+        if (tid < 128) {
+            const int pl = tid >> 2, c = 8 * (tid & 3);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) As[buf][c + j][pl] = ra[j];
+        }
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const int u = tid + NT * v;
+            const int pl = u >> 4, kl = 8 * (u & 15);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Bs[buf][kl + j][pl] = rb[v][j];
+        }
+    };
     f32x4 acc[2][2];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int g = 0; g < 2; ++g) acc[h][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (long pk = pbeg; pk < pend; pk += 32) {
-        const long p = pk + q * 8;   // this lane's 8 positions p .. p+7 (same row: w is a multiple of 8)
-        bf16x8 a[2] = {}, b[2] = {};
-        if (p < pend) {
-            long t = p;
-            const int xq = (int)(t % S.w);
-            t /= S.w;
-            const int yq = (int)(t % S.h);
-            t /= S.h;
-            const int zq = (int)(t % S.d);
-            const int nq = (int)(t / S.d);
+    bf16x8 ra, rb[2];
+    if (ntile > 0) {
+        load(pbeg, ra, rb);
+        store(0, ra, rb);
+    }
+    __syncthreads();
+    for (int it = 0; it < ntile; ++it) {
+        const int buf = it & 1;
+        const bool more = it + 1 < ntile;
+        if (more) load(pbeg + 32L * (it + 1), ra, rb);
+        bf16x8 fa[2], fb[2];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int co = co0 + h * 16 + r;
-                if (co < cout)
+        for (int h = 0; h < 2; ++h) fa[h] = *(const bf16x8*)&As[buf][h * 16 + r][q * 8];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) a[h][j] = dy[(p + j) * cout + co];
-            }
-#pragma unroll
-            for (int g = 0; g < 2; ++g) {
-                if (!bv[g]) continue;
-                const int zz = zq + bdz[g], yy = yq + bdy[g];
-                if (zz < 0 || zz >= S.d || yy < 0 || yy >= S.h) continue;
-                const long rowb = (((long)nq * S.d + zz) * S.h + yy) * S.w;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int xx = xq + j + bdx[g];
-                    if (xx >= 0 && xx < S.w) b[g][j] = x[(rowb + xx) * cin + bci[g]];
-                }
-            }
-        }
+        for (int g = 0; g < 2; ++g) fb[g] = *(const bf16x8*)&Bs[buf][wv * 32 + g * 16 + r][q * 8];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int g = 0; g < 2; ++g) acc[h][g] = mfma(a[h], b[g], acc[h][g]);
+            for (int g = 0; g < 2; ++g) acc[h][g] = mfma(fa[h], fb[g], acc[h][g]);
+        if (more) store(buf ^ 1, ra, rb);
+        __syncthreads();
     }
+    const int kk0 = kb0 + wv * 32;
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -386,7 +414,8 @@ extern "C" int mflbm_nn_conv3d_fwd(const void* x, const void* wf, const float* b
 
 extern "C" int mflbm_nn_conv3d_dw(const void* x, const void* dy, float* dw, float* dbias, int32_t n, int32_t d, int32_t h,
                                   int32_t w, int32_t cin, int32_t cout, void* stream) {
-    if (!x || !dy || !dw || n < 1 || d < 1 || h < 1 || w < 8 || w % 8 || cin < 8 || cin % 8 || cout < 1)
+    if (!x || !dy || !dw || n < 1 || d < 1 || h < 1 || w < 8 || w % 8 || cin < 8 || cin % 8 || cout < 8 || cout % 8 ||
+        (long)n * d * h * w >= (1L << 31))
         return MFLBM_ERR_ARG;
     const Shape S{n, d, h, w};
     const long M = S.count();
@@ -457,7 +486,8 @@ int scratch_of_current_device(float** out) {
 
 bool dw_args_ok(const void* x, const void* dy, const float* dw, int32_t n, int32_t d, int32_t h, int32_t w, int32_t cin,
                 int32_t cout) {
-    return x && dy && dw && n >= 1 && d >= 1 && h >= 1 && w >= 8 && w % 8 == 0 && cin >= 8 && cin % 8 == 0 && cout >= 1;
+    return x && dy && dw && n >= 1 && d >= 1 && h >= 1 && w >= 8 && w % 8 == 0 && cin >= 8 && cin % 8 == 0 && cout >= 8 &&
+           cout % 8 == 0 && (long)n * d * h * w < (1L << 31);
 }
 
 }  // namespace
