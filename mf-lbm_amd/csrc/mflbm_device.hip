@@ -121,6 +121,8 @@ struct Pop<double> {
     const Dev* d;
     int c0;   // 0 fluid 1, 19 fluid 2
     __device__ __forceinline__ double& operator()(int q, int o) const { return d->S.p[c0 + q][2 * slot_index(*d, q, o)]; }
+    // the slot of direction q with a slot index resolved before (slot_index)
+    __device__ __forceinline__ double& at(int q, int i) const { return d->S.p[c0 + q][2 * i]; }
 };
 struct CRef {
     float* s;
@@ -144,8 +146,8 @@ template <>
 struct Pop<float> {
     const Dev* d;
     int c0;
-    __device__ __forceinline__ CRef operator()(int q, int o) const {
-        const int i = slot_index(*d, q, o);
+    __device__ __forceinline__ CRef operator()(int q, int o) const { return at(q, slot_index(*d, q, o)); }
+    __device__ __forceinline__ CRef at(int q, int i) const {
         return CRef{(float*)d->S.p[q] + 2 * i, (float*)d->S.p[19 + q] + 2 * i, q, c0 != 0};
     }
 };
@@ -583,15 +585,20 @@ __device__ __forceinline__ void bc_inlet_velocity_before(const Dev& d, int t, do
     const double tmp1 = tmp2 * sa_inject;
     tmp2 = tmp2 - tmp1;
     const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
+    // slot written at z = 0 (direction QW, offset DI, DJ) from the slot QR of (i, j, 1); every slot index is
+    // resolved first, so the latency-bound index lookups overlap
+    const int QW[5] = {5, 11, 12, 15, 16}, QR[5] = {6, 14, 13, 18, 17}, DI[5] = {0, -1, 1, 0, 0}, DJ[5] = {0, 0, 0, -1, 1};
+    int iw[5], ir[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        iw[m] = slot_index(d, QW[m], L.at(i + DI[m], j + DJ[m], 0));
+        ir[m] = slot_index(d, QR[m], L.at(i, j, 1));
+    }
     for (int c = 0; c < 2; ++c) {
         const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double s = c == 0 ? tmp1 : tmp2;
-        int o;
-        o = L.at(i, j, 0);         A(5, o) = (A(6, L.at(i, j, 1)) + c1 * s) * om + A(5, o) * wd;
-        o = L.at(i - 1, j, 0);     A(11, o) = (A(14, L.at(i, j, 1)) + c2 * s) * om + A(11, o) * wd;
-        o = L.at(i + 1, j, 0);     A(12, o) = (A(13, L.at(i, j, 1)) + c2 * s) * om + A(12, o) * wd;
-        o = L.at(i, j - 1, 0);     A(15, o) = (A(18, L.at(i, j, 1)) + c2 * s) * om + A(15, o) * wd;
-        o = L.at(i, j + 1, 0);     A(16, o) = (A(17, L.at(i, j, 1)) + c2 * s) * om + A(16, o) * wd;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.at(QW[m], iw[m]) = (A.at(QR[m], ir[m]) + (m == 0 ? c1 : c2) * s) * om + A.at(QW[m], iw[m]) * wd;
     }
 }
 // inlet_bounce_back_velocity_BC_after_odd (Boundary_multiphase_inlet.F90:56-102)
@@ -608,14 +615,19 @@ __device__ __forceinline__ void bc_inlet_velocity_after(const Dev& d, int t, dou
     tmp2 = tmp2 - tmp1;
     const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     const int o = L.at(i, j, 1);
+    // slot QW of (i, j, 1) from the slot QR at z = 0 (offset DI, DJ); slot indices resolved first
+    const int QW[5] = {6, 13, 14, 17, 18}, QR[5] = {5, 12, 11, 16, 15}, DI[5] = {0, 1, -1, 0, 0}, DJ[5] = {0, 0, 0, 1, -1};
+    int iw[5], ir[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        iw[m] = slot_index(d, QW[m], o);
+        ir[m] = slot_index(d, QR[m], L.at(i + DI[m], j + DJ[m], 0));
+    }
     for (int c = 0; c < 2; ++c) {
         const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double s = c == 0 ? tmp1 : tmp2;
-        A(6, o) = (A(5, L.at(i, j, 0)) + c1 * s) * om + A(6, o) * wd;
-        A(13, o) = (A(12, L.at(i + 1, j, 0)) + c2 * s) * om + A(13, o) * wd;
-        A(14, o) = (A(11, L.at(i - 1, j, 0)) + c2 * s) * om + A(14, o) * wd;
-        A(17, o) = (A(16, L.at(i, j + 1, 0)) + c2 * s) * om + A(17, o) * wd;
-        A(18, o) = (A(15, L.at(i, j - 1, 0)) + c2 * s) * om + A(18, o) * wd;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) A.at(QW[m], iw[m]) = (A.at(QR[m], ir[m]) + (m == 0 ? c1 : c2) * s) * om + A.at(QW[m], iw[m]) * wd;
     }
 }
 // inlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_inlet.F90:115-209)
@@ -700,18 +712,20 @@ __device__ __forceinline__ void bc_outlet_convective_before(const Dev& d, int t,
     const int QS[5] = {6, 13, 14, 17, 18};
     const int DI[5] = {0, -1, 1, 0, 0};
     const int DJ[5] = {0, 0, 0, -1, 1};
+    int io[5], ii[5];   // slot indices, resolved first
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        io[m] = slot_index(d, QS[m], L.at(i + DI[m], j + DJ[m], nz + 1));
+        ii[m] = slot_index(d, QS[m], L.at(i + DI[m], j + DJ[m], nz));
+    }
     for (int c = 0; c < 2; ++c) {
         double* hist = c == 0 ? d.cf : d.cg;
-        for (int m = 0; m < 5; ++m) {
-            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
-            const int o = L.at(i + DI[m], j + DJ[m], nz + 1);
-            const int in = L.at(i + DI[m], j + DJ[m], nz);
-            A(QS[m], o) = ((hist[pc + QS[m] * pl] + u_convec * A(QS[m], in)) * temp) * om + A(QS[m], o) * wd;
-        }
-        for (int m = 0; m < 5; ++m) {
-            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
-            hist[pc + QS[m] * pl] = A(QS[m], L.at(i + DI[m], j + DJ[m], nz + 1));
-        }
+        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            A.at(QS[m], io[m]) = ((hist[pc + QS[m] * pl] + u_convec * A.at(QS[m], ii[m])) * temp) * om + A.at(QS[m], io[m]) * wd;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) hist[pc + QS[m] * pl] = A.at(QS[m], io[m]);
     }
 }
 // outlet_convective_BC_after_odd (Boundary_multiphase_outlet.F90:68-127)
@@ -736,16 +750,20 @@ __device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, 
     const int QW[5] = {5, 11, 12, 15, 16};
     const int QH[5] = {6, 14, 13, 18, 17};
     const int o = L.at(i, j, nz), om1 = L.at(i, j, nz - 1);
+    int iw[5], im[5];   // slot indices, resolved first
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        iw[m] = slot_index(d, QW[m], o);
+        im[m] = slot_index(d, QW[m], om1);
+    }
     for (int c = 0; c < 2; ++c) {
         double* hist = c == 0 ? d.cf : d.cg;
-        for (int m = 0; m < 5; ++m) {
-            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
-            A(QW[m], o) = ((hist[pc + QH[m] * pl] + u_convec * A(QW[m], om1)) * temp) * om + A(QW[m], o) * wd;
-        }
-        for (int m = 0; m < 5; ++m) {
-            const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
-            hist[pc + QH[m] * pl] = A(QW[m], o);
-        }
+        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            A.at(QW[m], iw[m]) = ((hist[pc + QH[m] * pl] + u_convec * A.at(QW[m], im[m])) * temp) * om + A.at(QW[m], iw[m]) * wd;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) hist[pc + QH[m] * pl] = A.at(QW[m], iw[m]);
     }
 }
 // outlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_outlet.F90:139-228)
