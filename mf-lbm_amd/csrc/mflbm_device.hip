@@ -1151,6 +1151,16 @@ struct GroupItems {
         lane = (int)threadIdx.x % G;
     }
     static constexpr int STRIDE = CHAIN_PER * NG;
+    // the item at i, with the load of the group's next item started (it overlaps this item's loads)
+    int4 pre;
+    __device__ __forceinline__ void start() {
+        if (i < hi) pre = list[i];
+    }
+    __device__ __forceinline__ int4 take() {
+        const int4 it = pre;
+        if (i + STRIDE < hi) pre = list[i + STRIDE];
+        return it;
+    }
 };
 // a normal of brick b has a bit set when any lane of the group says so: nzb of b and nzd of its 27-brick
 // neighbourhood (which the lattice update reads) are set.  All lanes of the wave call this together.
@@ -1169,8 +1179,8 @@ __device__ __forceinline__ void mark_nonzero(const Dev& d, int b, bool nz, int l
 // phi extrapolation at the solid nodes of the listed bricks (those whose phi an evaluated normal may read)
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate_b(Dev d, SolidList s, BrickWork w) {
     GroupItems<16> g(w);
-    for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
-        const int4 it = g.list[g.i];
+    for (g.start(); g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
+        const int4 it = g.take();
         for (int t = it.y + g.lane; t < it.z; t += 16) phi_extrapolate_at(d, s, t);
     }
 }
@@ -1185,8 +1195,8 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
     GroupItems<64> g(w);
     const Layout L = d.L;
     Rec4<NT>* const N = normals<NT>(d);
-    for (; g.i < g.hi; g.i += GroupItems<64>::STRIDE) {
-        const int4 it = g.list[g.i];
+    for (g.start(); g.i < g.hi; g.i += GroupItems<64>::STRIDE) {
+        const int4 it = g.take();
         const int t = it.y + g.lane;
         bool nz = false;
         if (t < it.z) {
@@ -1218,8 +1228,8 @@ template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, BrickWork w) {
     GroupItems<16> g(w);
     Rec4<NT>* const N = normals<NT>(d);
-    for (; g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
-        const int4 it = g.list[g.i];
+    for (g.start(); g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
+        const int4 it = g.take();
         bool nz = false;
         for (int t = it.y + g.lane; t < it.z; t += 16) {
             const int n = s.pos[t];
