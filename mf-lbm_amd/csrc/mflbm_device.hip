@@ -1190,10 +1190,56 @@ __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate_b(Dev d, SolidList s,
 // boundary list or is -1.  Solid nodes keep 0 or their extrapolated value, as after the reference's
 // zero-then-extrapolate sequence.  Listed bricks are evaluated, or, when q1b (every phi their normals depend on
 // lies within the band: the normals are +0), cleared.
+// the normal of node n (entry t of the normal list): stored, and whether a bit of it is set
+template <class NT>
+__device__ __forceinline__ bool eval_normal(const Dev& d, int n, int fb, const FluidList& fl) {
+    const PhiNowAt P{&d, n};
+    double cx = dx4f(P);
+    double cy = dy4f(P);
+    double cz = dz4f(P);
+    double cn;
+    normalise(cx, cy, cz, cn);
+    if (fb >= 0 && cn > 1e-6)
+        wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
+    const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
+    normals<NT>(d)[n] = Rec4<NT>{sx, sy, sz, sn};
+    return bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn);
+}
+// the extrapolated normal of solid list entry t: the neighbours loaded nine directions at a time, unconditionally
+// (inside the padded grid, as in phi_extrapolate_at), and the fluid ones summed in the reference's order; the record
+// is stored whole (c_norm of a solid node is never evaluated and stays +0)
+template <class NT>
+__device__ __forceinline__ bool eval_normal_extrapolate(const Dev& d, const SolidList& s, int t) {
+    Rec4<NT>* const N = normals<NT>(d);
+    const int n = s.pos[t];
+    const unsigned mk = s.mask[t];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        NT vx[9], vy[9], vz[9];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            const Rec4<NT> r = N[n + d.L.off(1 + 9 * h + u)];
+            vx[u] = r.x;
+            vy[u] = r.y;
+            vz[u] = r.z;
+        }
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            const int q = 1 + 9 * h + u;
+            const bool f = (mk & (1u << q)) != 0;
+            ax = f ? ax + (double)vx[u] * lat::W[q] : ax;
+            ay = f ? ay + (double)vy[u] * lat::W[q] : ay;
+            az = f ? az + (double)vz[u] * lat::W[q] : az;
+        }
+    }
+    const NT sx = (NT)(ax / s.w[t]), sy = (NT)(ay / s.w[t]), sz = (NT)(az / s.w[t]);
+    N[n] = Rec4<NT>{sx, sy, sz, (NT)0.0};
+    return bits_set(sx) || bits_set(sy) || bits_set(sz);
+}
 template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl, BrickWork w) {
     GroupItems<64> g(w);
-    const Layout L = d.L;
     Rec4<NT>* const N = normals<NT>(d);
     for (g.start(); g.i < g.hi; g.i += GroupItems<64>::STRIDE) {
         const int4 it = g.take();
@@ -1201,29 +1247,14 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
         bool nz = false;
         if (t < it.z) {
             const int n = nl.pos[t];
-            if (it.w) {
-                N[n] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
-            } else {
-                const PhiNowAt P{&d, n};
-                double cx = dx4f(P);
-                double cy = dy4f(P);
-                double cz = dz4f(P);
-                double cn;
-                normalise(cx, cy, cz, cn);
-                const int fb = wb[t];
-                if (fb >= 0 && cn > 1e-6)
-                    wet(cx, cy, cz, fl.nw[3 * fb], fl.nw[3 * fb + 1], fl.nw[3 * fb + 2], fl.cos_t[fb], fl.sin_t[fb]);
-                const NT sx = (NT)cx, sy = (NT)cy, sz = (NT)cz, sn = (NT)cn;
-                N[n] = Rec4<NT>{sx, sy, sz, sn};
-                nz = bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn);
-            }
+            if (it.w) N[n] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
+            else nz = eval_normal<NT>(d, n, wb[t], fl);
         }
         mark_nonzero<64>(d, it.x, nz, g.lane);
     }
 }
 // normal extrapolated into the solid boundary nodes within one halo layer (Phase_gradient.F90:84-110); listed
-// bricks are evaluated, or, when q2b (every normal read is +0, so is the weighted mean), cleared.  The record is
-// stored whole: c_norm of a solid node is never evaluated and stays +0.
+// bricks are evaluated, or, when q2b (every normal read is +0, so is the weighted mean), cleared
 template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, BrickWork w) {
     GroupItems<16> g(w);
@@ -1232,40 +1263,24 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
         const int4 it = g.take();
         bool nz = false;
         for (int t = it.y + g.lane; t < it.z; t += 16) {
-            const int n = s.pos[t];
-            if (it.w) {
-                N[n] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
-                continue;
-            }
-            // the neighbours loaded nine directions at a time, unconditionally (inside the padded grid, as in
-            // phi_extrapolate_at), and the fluid ones summed in the reference's order
-            const unsigned mk = s.mask[t];
-            double ax = 0.0, ay = 0.0, az = 0.0;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                NT vx[9], vy[9], vz[9];
-#pragma unroll
-                for (int u = 0; u < 9; ++u) {
-                    const Rec4<NT> r = N[n + d.L.off(1 + 9 * h + u)];
-                    vx[u] = r.x;
-                    vy[u] = r.y;
-                    vz[u] = r.z;
-                }
-#pragma unroll
-                for (int u = 0; u < 9; ++u) {
-                    const int q = 1 + 9 * h + u;
-                    const bool f = (mk & (1u << q)) != 0;
-                    ax = f ? ax + (double)vx[u] * lat::W[q] : ax;
-                    ay = f ? ay + (double)vy[u] * lat::W[q] : ay;
-                    az = f ? az + (double)vz[u] * lat::W[q] : az;
-                }
-            }
-            const NT sx = (NT)(ax / s.w[t]), sy = (NT)(ay / s.w[t]), sz = (NT)(az / s.w[t]);
-            N[n] = Rec4<NT>{sx, sy, sz, (NT)0.0};
-            nz = nz || bits_set(sx) || bits_set(sy) || bits_set(sz);
+            if (it.w) N[s.pos[t]] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
+            else nz = eval_normal_extrapolate<NT>(d, s, t) || nz;
         }
         mark_nonzero<16>(d, it.x, nz, g.lane);
     }
+}
+// The whole chain without brick lists (FP32 collision, whose rounding keeps phi out of the band almost everywhere):
+// one thread per entry of the normal list and of the solid list within one halo layer, in brick order.  nzd stays
+// set everywhere (the lattice update reads the stencil at every node, as the flags would have it).
+template <class NT>
+__global__ void __launch_bounds__(BLOCK) k_normals_all(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl) {
+    const int t = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
+    if (t < nl.n) (void)eval_normal<NT>(d, nl.pos[t], wb[t], fl);
+}
+template <class NT>
+__global__ void __launch_bounds__(BLOCK) k_normal_extrapolate_all(Dev d, SolidList s) {
+    const int t = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
+    if (t < s.n) (void)eval_normal_extrapolate<NT>(d, s, t);
 }
 // The plan of one chain.  A work-group owns TBX x TBY x TBZ bricks and stages the band flags of them and a halo
 // of two bricks in LDS.  From the band flags the lattice update left it derives q1b (every brick within one quiet
@@ -1850,6 +1865,7 @@ struct mflbm_ctx {
     Fold fold;                       // periodic axes folded into the neighbour map
     bool stepped = false;            // a step ran since the state was uploaded or built
     bool chain_fused = false;   // one fused pass for normals, extrapolation and curvature (MFLBM_CHAIN=fused)
+    bool chain_all = false;     // the chain over the whole lists, without the brick plan (MFLBM_CHAIN=all)
     int8_t* walls = nullptr;
     double* planes = nullptr;   // w_in, cphi, cf(19), cg(19)
     double* u = nullptr;
@@ -1961,6 +1977,14 @@ int launch_plan(mflbm_ctx* c, const ChainPlan& p, int full, const FaceWork& f) {
 // flags the last lattice update filled and clears the other buffers, which the next lattice update fills.
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
+    if (c->chain_all) {   // every entry of every list; nzd stays set as the upload left it
+        LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
+        LAUNCH(c, k_normals_all<NT>, nblocks(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl);
+        LAUNCH(c, k_normal_extrapolate_all<NT>, nblocks(c->sl1.n), c->d, c->sl1);
+        c->full_chain = false;
+        c->solid_zeroed = false;
+        return MFLBM_OK;
+    }
     const long nb = c->nbricks;
     const int set = c->band_cur;
     {
@@ -2059,7 +2083,8 @@ int launch_step(mflbm_ctx* c, int ntime) {
 #undef MFLBM_COLLIDE32
     if (t) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     FaceWork faces;
-    const int rc = c->fp32 ? launch_faces<float>(c, odd, &faces) : launch_faces<double>(c, odd, &faces);
+    FaceWork* defer = c->chain_all ? nullptr : &faces;   // (no plan launch to take the faces)
+    const int rc = c->fp32 ? launch_faces<float>(c, odd, defer) : launch_faces<double>(c, odd, defer);
     if (rc) return rc;
     if (t) HIPCHK(c, hipEventRecord(ev[2], c->stream));
     const int rg = launch_gradient(c, faces);
@@ -2198,8 +2223,12 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     {
         // the fused chain is bit-identical but slower on the 256^3 benchmark (0.78 vs 0.42 ms; the lattice
         // update it feeds is 8 % faster): the list kernels stay the default
+        // FP32 collision: its rounding leaves phi outside the band at nearly every node, so the plan would list
+        // nearly every brick and the whole-list chain is the faster (MFLBM_CHAIN=bricks keeps the plan)
         const char* e = std::getenv("MFLBM_CHAIN");
-        c->chain_fused = e && std::strcmp(e, "fused") == 0;
+        const std::string m = e ? e : "";
+        c->chain_fused = m == "fused";
+        c->chain_all = m == "all" || (m.empty() && prm->precision == 1);
     }
     Layout& L = c->L;
     L = make_layout(prm->nx, prm->ny, prm->nz);

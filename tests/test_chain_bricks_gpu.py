@@ -1,13 +1,14 @@
-"""The colour-gradient chain skips bricks (4^3 nodes) whose neighbourhood holds phi within 3e-7 of one of +-1
-(there every normal is +0), leaves the extrapolated phi of solid nodes there as it was until a download needs
+"""The colour-gradient chain skips bricks (16 x 2 x 2 nodes) whose neighbourhood holds phi within 3e-7 of one of
++-1 (there every normal is +0), leaves the extrapolated phi of solid nodes there as it was until a download needs
 it, and the lattice update skips the normal stencil in bricks with no set bit nearby.  These tests run
 domains large enough for most bricks to be skipped, with monitor calls in between and at the last step (the
 macroscopic pass zeroes phi at solid nodes), and check:
 
 * FP64: every population and phi bit for bit against the oracle, which evaluates every node every step;
-* FP32 storage (FP32 and mixed collision): bit for bit against the same run with a full chain forced after
-  every step (mflbm_color_gradient evaluates every brick), so a skipped brick that should have been
-  evaluated shows as a difference.
+* FP32 storage (FP32 and mixed collision, brick plan forced with MFLBM_CHAIN=bricks): bit for bit against the
+  same run with a full chain forced after every step (mflbm_color_gradient evaluates every brick), so a skipped
+  brick that should have been evaluated shows as a difference, and against the chain over whole lists
+  (MFLBM_CHAIN=all, the FP32-collision default), which evaluates every entry every step.
 """
 import numpy as np
 import pytest
@@ -76,11 +77,18 @@ def test_brick_skipping_chain_is_bit_identical_to_the_oracle(make, nsteps):
 
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
 @pytest.mark.parametrize("make,nsteps", [(_drainage, 200), (_periodic_slug, 160)])
-def test_brick_skipping_chain_equals_a_full_chain_with_fp32_storage(make, nsteps, precision):
+def test_brick_skipping_chain_equals_a_full_chain_with_fp32_storage(make, nsteps, precision, monkeypatch):
     print(f"[bricks {precision}] {make.__name__} {nsteps} steps", flush=True)
+    monkeypatch.setenv("MFLBM_CHAIN", "bricks")
     a, b = Simulation(make(precision)), Simulation(make(precision))
+    monkeypatch.setenv("MFLBM_CHAIN", "all")
+    w = Simulation(make(precision))
+    monkeypatch.delenv("MFLBM_CHAIN")
     _run(a, nsteps)
     _run(b, nsteps, full_every_step=True)
-    _same(a.state(), b.state(), f"{make.__name__} {precision}")
-    a.close()
-    b.close()
+    _run(w, nsteps)
+    sa = a.state()
+    _same(sa, b.state(), f"{make.__name__} {precision}")
+    _same(sa, w.state(), f"{make.__name__} {precision} whole lists")
+    for s in (a, b, w):
+        s.close()

@@ -91,23 +91,26 @@ def test_gpu_c1_2000_steps_matches_reference_and_oracle():
 @pytest.mark.parametrize("name", ["open_velocity", "periodic_yz_drop"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_fused_gradient_chain_equals_the_list_kernels(name, precision, monkeypatch):
-    """MFLBM_CHAIN=fused: normals, extrapolation and curvature force in one LDS-tiled pass (k_chain_force); the
-    populations and phi after a run are bit-identical to the list kernels' and the downloaded gradient too."""
+    """MFLBM_CHAIN=fused: normals, extrapolation and curvature force in one LDS-tiled pass (k_chain_force), and
+    MFLBM_CHAIN=all: the list kernels over every entry without the brick plan; the populations and phi after a run
+    are bit-identical to the brick-listed kernels' and the downloaded gradient too."""
     from helpers import load_fixture
     from mflbm import Case
     z, ctl = load_fixture(name)
     case = Case(ctl, z["walls"], precision=precision)
     print(f"[fused chain] case={name} precision={precision}", flush=True)
     sims = {}
-    for chain in ("lists", "fused"):
+    for chain in ("bricks", "all", "fused"):
         monkeypatch.setenv("MFLBM_CHAIN", chain)
         sim = Simulation(case)
         sim.run(1, 12)
         sims[chain] = (sim.state(), sim.gradient(), sim.monitor())
         sim.close()
-    a, b = sims["lists"], sims["fused"]
-    for k in ("f", "g", "phi"):
-        assert np.array_equal(getattr(a[0], k), getattr(b[0], k)), k
-    for k in a[1]:
-        assert np.array_equal(a[1][k], b[1][k]), k
-    assert a[2].saturation == b[2].saturation
+    a = sims["bricks"]
+    for chain in ("all", "fused"):
+        b = sims[chain]
+        for k in ("f", "g", "phi"):
+            assert np.array_equal(getattr(a[0], k), getattr(b[0], k)), (chain, k)
+        for k in a[1]:
+            assert np.array_equal(a[1][k], b[1][k]), (chain, k)
+        assert a[2].saturation == b[2].saturation, chain
