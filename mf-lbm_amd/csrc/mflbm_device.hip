@@ -1270,17 +1270,52 @@ __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s
     }
 }
 // The whole chain without brick lists (FP32 collision, whose rounding keeps phi out of the band almost everywhere):
-// one thread per entry of the normal list and of the solid list within one halo layer, in brick order.  nzd stays
-// set everywhere (the lattice update reads the stencil at every node, as the flags would have it).
+// one thread per entry of the normal list and of the solid list within one halo layer.  The lists are in brick
+// order, so a wave holds a few runs of entries of one brick each: the first lane of a run with a set bit sets nzd
+// of the brick's 27-brick neighbourhood (cleared before the chain).  Every lane of the wave calls this; lanes past
+// the end of the list pass b = -1 and nz = false.
+__device__ __forceinline__ void mark_nonzero_runs(const Dev& d, int b, bool nz) {
+    const unsigned long long m = __ballot(nz);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int bp = __shfl(b, lane == 0 ? 0 : lane - 1);
+    const bool lead = lane == 0 || bp != b;
+    const unsigned long long ld = __ballot(lead);
+    const unsigned long long above = lane == 63 ? 0ull : ld & (~0ull << (lane + 1));
+    const unsigned long long run = above ? (above & (~above + 1ull)) - (1ull << lane) : ~0ull << lane;
+    if (!lead || !(m & run)) return;
+    const int bz = b / (d.nbx * d.nby), by = (b / d.nbx) % d.nby, bx = b % d.nbx;
+    for (int u = 0; u < 27; ++u) {
+        const int x = bx + u % 3 - 1, y = by + (u / 3) % 3 - 1, z = bz + u / 9 - 1;
+        if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) d.nzd[x + d.nbx * (y + d.nby * z)] = 1;
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_clear_bytes(unsigned char* p, int n) {
+    const int t = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
+    if (t < n) p[t] = 0;
+}
 template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normals_all(Dev d, NodeList nl, const int* __restrict__ wb, FluidList fl) {
     const int t = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
-    if (t < nl.n) (void)eval_normal<NT>(d, nl.pos[t], wb[t], fl);
+    int b = -1;
+    bool nz = false;
+    if (t < nl.n) {
+        const int n = nl.pos[t];
+        b = brick_of(d, n);
+        nz = eval_normal<NT>(d, n, wb[t], fl);
+    }
+    mark_nonzero_runs(d, b, nz);
 }
 template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate_all(Dev d, SolidList s) {
     const int t = (int)blockIdx.x * BLOCK + (int)threadIdx.x;
-    if (t < s.n) (void)eval_normal_extrapolate<NT>(d, s, t);
+    int b = -1;
+    bool nz = false;
+    if (t < s.n) {
+        b = brick_of(d, s.pos[t]);
+        nz = eval_normal_extrapolate<NT>(d, s, t);
+    }
+    mark_nonzero_runs(d, b, nz);
 }
 // The plan of one chain.  A work-group owns TBX x TBY x TBZ bricks and stages the band flags of them and a halo
 // of two bricks in LDS.  From the band flags the lattice update left it derives q1b (every brick within one quiet
@@ -1977,7 +2012,8 @@ int launch_plan(mflbm_ctx* c, const ChainPlan& p, int full, const FaceWork& f) {
 // flags the last lattice update filled and clears the other buffers, which the next lattice update fills.
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
-    if (c->chain_all) {   // every entry of every list; nzd stays set as the upload left it
+    if (c->chain_all) {   // every entry of every list; nzd cleared and set again by the normals
+        LAUNCH(c, k_clear_bytes, nblocks(c->nbricks), c->d.nzd, (int)c->nbricks);
         LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
         LAUNCH(c, k_normals_all<NT>, nblocks(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl);
         LAUNCH(c, k_normal_extrapolate_all<NT>, nblocks(c->sl1.n), c->d, c->sl1);
