@@ -192,12 +192,17 @@ __device__ __forceinline__ double phi_now(const Dev& d, int n) {
     const int c = d.pmap[n];
     return *((unsigned)c < (unsigned)d.nf ? d.phic + c : d.phi + n);
 }
-// phi_now around node n, for the stencils
+// phi_now around node n, for the stencils (DENSE: the dense phi, where it is current at every node)
+template <bool DENSE = false>
+__device__ __forceinline__ double phi_at(const Dev& d, int n) {
+    return DENSE ? d.phi[n] : phi_now(d, n);
+}
+template <bool DENSE = false>
 struct PhiNowAt {
     const Dev* d;
     int n;
     __device__ __forceinline__ double operator()(int a, int b, int c) const {
-        return phi_now(*d, n + a + b * d->L.px + c * d->L.pxy);
+        return phi_at<DENSE>(*d, n + a + b * d->L.px + c * d->L.pxy);
     }
 };
 
@@ -1075,22 +1080,24 @@ __device__ __forceinline__ double quiet_curvature() { return curvature_f(ZeroAt{
 // phi extrapolated into a solid boundary node from its fluid neighbours (Phase_gradient.F90:12-30).  Solid list
 // nodes lie within [-2, n+3]^3, so all 18 neighbours are inside the padded grid: they are loaded unconditionally
 // (phi_now: the current phi also where the dense phi lags) and the fluid ones summed in the reference's order.
+template <bool DENSE = false>
 __device__ __forceinline__ void phi_extrapolate_at(const Dev& d, const SolidList& s, int t) {
     const int n = s.pos[t];
     const unsigned mk = s.mask[t];
     double v[19];
 #pragma unroll
-    for (int q = 1; q < 19; ++q) v[q] = phi_now(d, n + d.L.off(q));
+    for (int q = 1; q < 19; ++q) v[q] = phi_at<DENSE>(d, n + d.L.off(q));
     double acc = 0.0;
 #pragma unroll
     for (int q = 1; q < 19; ++q) acc = (mk & (1u << q)) ? acc + v[q] * lat::W[q] : acc;
     d.phi[n] = acc / s.w[t];
 }
-// every entry of the list (before a download: the chain skips the bricks no evaluated normal reads)
+// every entry of the list, from the dense phi (made current first): before a download (the brick chain skips the
+// bricks no evaluated normal reads) and in the whole-list chain
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate(Dev d, SolidList s, int per) {
     const int t = xcd_block(per) * BLOCK + threadIdx.x;
     if (t >= s.n) return;
-    phi_extrapolate_at(d, s, t);
+    phi_extrapolate_at<true>(d, s, t);
 }
 // wetting: turn the normal at a fluid node next to the wall to the contact angle (Akai et al. 2018;
 // alter_color_gradient_solid_surface, Phase_gradient.F90:210-265).  cos/sin of the angle come from the host.
@@ -1191,9 +1198,9 @@ __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate_b(Dev d, SolidList s,
 // zero-then-extrapolate sequence.  Listed bricks are evaluated, or, when q1b (every phi their normals depend on
 // lies within the band: the normals are +0), cleared.
 // the normal of node n (entry t of the normal list): stored, and whether a bit of it is set
-template <class NT>
+template <class NT, bool DENSE = false>
 __device__ __forceinline__ bool eval_normal(const Dev& d, int n, int fb, const FluidList& fl) {
-    const PhiNowAt P{&d, n};
+    const PhiNowAt<DENSE> P{&d, n};
     double cx = dx4f(P);
     double cy = dy4f(P);
     double cz = dz4f(P);
@@ -1302,7 +1309,7 @@ __global__ void __launch_bounds__(BLOCK) k_normals_all(Dev d, NodeList nl, const
     if (t < nl.n) {
         const int n = nl.pos[t];
         b = brick_of(d, n);
-        nz = eval_normal<NT>(d, n, wb[t], fl);
+        nz = eval_normal<NT, true>(d, n, wb[t], fl);
     }
     mark_nonzero_runs(d, b, nz);
 }
@@ -2012,7 +2019,9 @@ int launch_plan(mflbm_ctx* c, const ChainPlan& p, int full, const FaceWork& f) {
 // flags the last lattice update filled and clears the other buffers, which the next lattice update fills.
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
-    if (c->chain_all) {   // every entry of every list; nzd cleared and set again by the normals
+    if (c->chain_all) {   // every entry of every list, from the dense phi; nzd cleared and set again by the normals
+        const int re = ensure_dense_phi(c);
+        if (re) return re;
         LAUNCH(c, k_clear_bytes, nblocks(c->nbricks), c->d.nzd, (int)c->nbricks);
         LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
         LAUNCH(c, k_normals_all<NT>, nblocks(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl);
