@@ -1078,15 +1078,17 @@ struct ZeroAt {
 __device__ __forceinline__ double quiet_curvature() { return curvature_f(ZeroAt{}, ZeroAt{}, ZeroAt{}); }
 
 // phi extrapolated into a solid boundary node from its fluid neighbours (Phase_gradient.F90:12-30).  Solid list
-// nodes lie within [-2, n+3]^3, so all 18 neighbours are inside the padded grid: they are loaded unconditionally
-// (phi_now: the current phi also where the dense phi lags) and the fluid ones summed in the reference's order.
+// nodes lie within [-2, n+3]^3, so all 18 neighbours are inside the padded grid.  The 18 loads are issued together,
+// without branches (phi_now: the current phi also where the dense phi lags); a direction without a fluid neighbour
+// loads the node's own value instead (a line already fetched, the value unused), and the fluid ones are summed in
+// the reference's order.
 template <bool DENSE = false>
 __device__ __forceinline__ void phi_extrapolate_at(const Dev& d, const SolidList& s, int t) {
     const int n = s.pos[t];
     const unsigned mk = s.mask[t];
     double v[19];
 #pragma unroll
-    for (int q = 1; q < 19; ++q) v[q] = phi_at<DENSE>(d, n + d.L.off(q));
+    for (int q = 1; q < 19; ++q) v[q] = phi_at<DENSE>(d, (mk & (1u << q)) ? n + d.L.off(q) : n);
     double acc = 0.0;
 #pragma unroll
     for (int q = 1; q < 19; ++q) acc = (mk & (1u << q)) ? acc + v[q] * lat::W[q] : acc;
@@ -1212,9 +1214,10 @@ __device__ __forceinline__ bool eval_normal(const Dev& d, int n, int fb, const F
     normals<NT>(d)[n] = Rec4<NT>{sx, sy, sz, sn};
     return bits_set(sx) | bits_set(sy) | bits_set(sz) | bits_set(sn);
 }
-// the extrapolated normal of solid list entry t: the neighbours loaded nine directions at a time, unconditionally
-// (inside the padded grid, as in phi_extrapolate_at), and the fluid ones summed in the reference's order; the record
-// is stored whole (c_norm of a solid node is never evaluated and stays +0)
+// the extrapolated normal of solid list entry t: the neighbours loaded nine directions at a time, without branches
+// (inside the padded grid, as in phi_extrapolate_at; a direction without a fluid neighbour loads the node's own
+// record), and the fluid ones summed in the reference's order; the record is stored whole (c_norm of a solid node is
+// never evaluated and stays +0)
 template <class NT>
 __device__ __forceinline__ bool eval_normal_extrapolate(const Dev& d, const SolidList& s, int t) {
     Rec4<NT>* const N = normals<NT>(d);
@@ -1226,7 +1229,8 @@ __device__ __forceinline__ bool eval_normal_extrapolate(const Dev& d, const Soli
         NT vx[9], vy[9], vz[9];
 #pragma unroll
         for (int u = 0; u < 9; ++u) {
-            const Rec4<NT> r = N[n + d.L.off(1 + 9 * h + u)];
+            const int q = 1 + 9 * h + u;
+            const Rec4<NT> r = N[(mk & (1u << q)) ? n + d.L.off(q) : n];
             vx[u] = r.x;
             vy[u] = r.y;
             vz[u] = r.z;
