@@ -53,6 +53,30 @@ __device__ __forceinline__ int link_slot(int nc, const int* wbase, const unsigne
     return nc + wbase[e] + __popcll(wm & ((1ull << l) - 1ull));
 }
 
+// The odd step's neighbour slots of fluid node m (compact index; m & 63 its lane, every lane of the wave in one link
+// mask word) at dense offset n: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r), which is
+// the link slot of (m, r) where the wave's link mask has m's bit, else the compact index of the fluid node x + e_r.
+// The wave-uniform masks and bases are scalar loads issued together; x + e_r lies inside the padded grid, so the
+// 18 dense map entries are loaded unconditionally, all in flight at once, and selected afterwards (a conditional
+// load per direction compiles to a branch that waits for every earlier load).
+__device__ __forceinline__ void odd_neighbour_slots(const int* __restrict__ map, const Layout& L, int nc,
+                                                    const int* __restrict__ wbase,
+                                                    const unsigned long long* __restrict__ wmask, int m, int n,
+                                                    unsigned nb[19]) {
+    const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
+    const unsigned lane = (unsigned)m & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned mv[19];
+#pragma unroll
+    for (int r = 1; r < 19; ++r) mv[r] = (unsigned)map[n + L.off(r)];
+#pragma unroll
+    for (int r = 1; r < 19; ++r) {
+        const unsigned long long wm = wmask[e0 + r];
+        const unsigned li = (unsigned)(nc + wbase[e0 + r]) + (unsigned)__popcll(wm & below);
+        nb[r] = ((wm >> lane) & 1ull) ? li : mv[r];
+    }
+}
+
 // Slot access with a scalar slice base and a 32-bit per-lane byte offset (one global load/store with an SGPR
 // base instead of a 64-bit vector address per slot); slices are kept below 2^32 bytes.
 template <class T>

@@ -278,17 +278,7 @@ k_collide(Dev d, CollideParams P, int per) {
         b[0] = v.y;
     }
     if (ODD) {
-        const int n = nn;
-        // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
-        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
-        const unsigned lane = (unsigned)m & 63u;
-        const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-        for (int r = 1; r < 19; ++r) {
-            const unsigned long long wm = d.wmask[e0 + r];
-            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
-            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[n + d.L.off(r)];
-        }
+        odd_neighbour_slots(d.map, d.L, d.nc, d.wbase, d.wmask, m, nn, nb);
 #pragma unroll
         for (int q = 1; q < 19; ++q) {   // slot q of the upwind node x - e_q = x + e_opp(q)
             const double2 v = ld_pair(S.p[q], nb[lat::OPP[q]]);
@@ -353,15 +343,7 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
         a[0] = v.y;
     }
     if (ODD) {
-        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
-        const unsigned lane = (unsigned)m & 63u;
-        const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-        for (int r = 1; r < 19; ++r) {
-            const unsigned long long wm = d.wmask[e0 + r];
-            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
-            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[nn + d.L.off(r)];
-        }
+        odd_neighbour_slots(d.map, d.L, d.nc, d.wbase, d.wmask, m, nn, nb);
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
             const float2 v = ld_pair32(S.p[q], nb[lat::OPP[q]]);

@@ -93,17 +93,7 @@ __device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollide
     unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
     a[0] = slot_ld(d.S[0], m);
     if (ODD) {
-        const int n = d.cpos[m];
-        // wave-uniform link masks and bases (scalar loads); m & 63 is the lane
-        const int e0 = 18 * __builtin_amdgcn_readfirstlane(m >> 6) - 1;
-        const unsigned lane = (unsigned)m & 63u;
-        const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-        for (int r = 1; r < 19; ++r) {
-            const unsigned long long wm = d.wmask[e0 + r];
-            const unsigned li = (unsigned)(d.nc + d.wbase[e0 + r]) + (unsigned)__popcll(wm & below);
-            nb[r] = ((wm >> lane) & 1ull) ? li : (unsigned)d.map[n + d.L.off(r)];
-        }
+        odd_neighbour_slots(d.map, d.L, d.nc, d.wbase, d.wmask, m, d.cpos[m], nb);
 #pragma unroll
         for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[q], nb[lat::OPP[q]]);   // slot q of the upwind node x - e_q
     } else {
