@@ -1167,12 +1167,14 @@ __device__ __forceinline__ void mark_nonzero(const Dev& d, int b, bool nz, int l
         if (x >= 0 && x < d.nbx && y >= 0 && y < d.nby && z >= 0 && z < d.nbz) d.nzd[x + d.nbx * (y + d.nby * z)] = 1;
     }
 }
+// lanes per brick of the two extrapolation stages (a brick has up to 64 solid entries; 16 and 64 lanes measured slower)
+constexpr int SOLID_GROUP = 32;
 // phi extrapolation at the solid nodes of the listed bricks (those whose phi an evaluated normal may read)
 __global__ void __launch_bounds__(BLOCK) k_phi_extrapolate_b(Dev d, SolidList s, BrickWork w) {
-    GroupItems<16> g(w);
-    for (g.start(); g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
+    GroupItems<SOLID_GROUP> g(w);
+    for (g.start(); g.i < g.hi; g.i += GroupItems<SOLID_GROUP>::STRIDE) {
         const int4 it = g.take();
-        for (int t = it.y + g.lane; t < it.z; t += 16) phi_extrapolate_at(d, s, t);
+        for (int t = it.y + g.lane; t < it.z; t += SOLID_GROUP) phi_extrapolate_at(d, s, t);
     }
 }
 // interface normal and |grad phi| at the non-solid nodes of [-1, n+2]^3 (Phase_gradient.F90:33-79), with the
@@ -1250,16 +1252,16 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Dev d, NodeList nl, const int
 // bricks are evaluated, or, when q2b (every normal read is +0, so is the weighted mean), cleared
 template <class NT>
 __global__ void __launch_bounds__(BLOCK) k_normal_extrapolate(Dev d, SolidList s, BrickWork w) {
-    GroupItems<16> g(w);
+    GroupItems<SOLID_GROUP> g(w);
     Rec4<NT>* const N = normals<NT>(d);
-    for (g.start(); g.i < g.hi; g.i += GroupItems<16>::STRIDE) {
+    for (g.start(); g.i < g.hi; g.i += GroupItems<SOLID_GROUP>::STRIDE) {
         const int4 it = g.take();
         bool nz = false;
-        for (int t = it.y + g.lane; t < it.z; t += 16) {
+        for (int t = it.y + g.lane; t < it.z; t += SOLID_GROUP) {
             if (it.w) N[s.pos[t]] = rec4<NT>(0.0, 0.0, 0.0, 0.0);
             else nz = eval_normal_extrapolate<NT>(d, s, t) || nz;
         }
-        mark_nonzero<16>(d, it.x, nz, g.lane);
+        mark_nonzero<SOLID_GROUP>(d, it.x, nz, g.lane);
     }
 }
 // The whole chain without brick lists (FP32 collision, whose rounding keeps phi out of the band almost everywhere):
