@@ -229,6 +229,68 @@ __device__ __forceinline__ float2 ld_pair32(const double* slice, unsigned m) {
 __device__ __forceinline__ void st_pair32(double* slice, unsigned m, float s, float a) {
     *(float2*)((char*)slice + m * 8u) = float2{s, a};
 }
+// One slot (direction q, index i) of both colours held in registers.  The velocity inlet and the convective outlet
+// load every slot they touch before their first store (a load after a store through another pointer of the same
+// type must wait for it, and the slices cannot be told apart at compile time), update the registers in the order of
+// the colour-by-colour code, and store each slot once.  get/set mirror Pop<T>::at: FP64 holds (f, g); FP32 holds
+// (s, a) and a colour's value is read and written in FP64 exactly as CRef does.
+template <class T>
+struct SlotReg;
+template <>
+struct SlotReg<double> {
+    double v[2];
+    __device__ __forceinline__ void load(const Dev& d, int q, int i) {
+        const double2 p = *(const double2*)(d.S.p[q] + 2 * (long)i);
+        v[0] = p.x;
+        v[1] = p.y;
+    }
+    __device__ __forceinline__ void store(const Dev& d, int q, int i) const {
+        *(double2*)(d.S.p[q] + 2 * (long)i) = double2{v[0], v[1]};
+    }
+    __device__ __forceinline__ double get(int c, int) const { return v[c]; }
+    __device__ __forceinline__ void set(int c, int, double x) { v[c] = x; }
+};
+template <>
+struct SlotReg<float> {
+    float s, a;
+    __device__ __forceinline__ void load(const Dev& d, int q, int i) {
+        const float2 p = *(const float2*)((const float*)d.S.p[q] + 2 * (long)i);
+        s = p.x;
+        a = p.y;
+    }
+    __device__ __forceinline__ void store(const Dev& d, int q, int i) const {
+        *(float2*)((float*)d.S.p[q] + 2 * (long)i) = float2{s, a};
+    }
+    __device__ __forceinline__ double get(int c, int q) const {
+        const double F = (double)s + rest_weight(q), A = (double)a;
+        return c ? 0.5 * (F - A) : 0.5 * (F + A);
+    }
+    __device__ __forceinline__ void set(int c, int q, double x) {
+        const double F = (double)s + rest_weight(q), A = (double)a;
+        const double other = c ? 0.5 * (F + A) : 0.5 * (F - A);
+        s = (float)(x + other - rest_weight(q));
+        a = (float)(c ? other - x : x - other);
+    }
+};
+// A slot of the velocity inlet or the convective outlet.  HOIST: held in registers (SlotReg); otherwise read and
+// written in place through Pop<T> in the same order, with fewer registers: the conditions that run in extra
+// work-groups of the chain's plan launch would otherwise lower the occupancy of the whole launch.
+template <class T, bool HOIST>
+struct FaceSlot;
+template <class T>
+struct FaceSlot<T, true> : SlotReg<T> {};
+template <class T>
+struct FaceSlot<T, false> {
+    const Dev* d;
+    int i;
+    __device__ __forceinline__ void load(const Dev& dd, int, int ii) {
+        d = &dd;
+        i = ii;
+    }
+    __device__ __forceinline__ void store(const Dev&, int, int) const {}
+    __device__ __forceinline__ double get(int c, int q) const { return (double)Pop<T>{d, c ? 19 : 0}.at(q, i); }
+    __device__ __forceinline__ void set(int c, int q, double x) const { Pop<T>{d, c ? 19 : 0}.at(q, i) = x; }
+};
 
 template <class NT>
 __device__ __forceinline__ double curvature_at(const Dev& d, int n);
@@ -560,20 +622,13 @@ __device__ __forceinline__ void set_inlet_phi(const Dev& d, int i, int j, double
 }
 
 // inlet_bounce_back_velocity_BC_before_odd (Boundary_multiphase_inlet.F90:6-53)
-template <class T>
+template <class T, bool HOIST = true>
 __device__ __forceinline__ void bc_inlet_velocity_before(const Dev& d, int t, double phi_inlet, double sa_inject, double relaxation) {
     const Layout L = d.L;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
-    const int wi = d.walls[L.at(i, j, 1)];
-    const double om = (double)(1 - wi), wd = (double)wi;
-    set_inlet_phi(d, i, j, phi_inlet, wi);
-    double tmp2 = d.w_in[plane_at(d, i, j)] * relaxation;
-    const double tmp1 = tmp2 * sa_inject;
-    tmp2 = tmp2 - tmp1;
-    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     // slot written at z = 0 (direction QW, offset DI, DJ) from the slot QR of (i, j, 1); every slot index is
-    // resolved first, so the latency-bound index lookups overlap
+    // resolved first and every slot loaded before the first store, so the latency-bound accesses overlap
     const int QW[5] = {5, 11, 12, 15, 16}, QR[5] = {6, 14, 13, 18, 17}, DI[5] = {0, -1, 1, 0, 0}, DJ[5] = {0, 0, 0, -1, 1};
     int iw[5], ir[5];
 #pragma unroll
@@ -581,28 +636,35 @@ __device__ __forceinline__ void bc_inlet_velocity_before(const Dev& d, int t, do
         iw[m] = slot_index(d, QW[m], L.at(i + DI[m], j + DJ[m], 0));
         ir[m] = slot_index(d, QR[m], L.at(i, j, 1));
     }
+    FaceSlot<T, HOIST> W[5], R[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        W[m].load(d, QW[m], iw[m]);
+        R[m].load(d, QR[m], ir[m]);
+    }
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    double tmp2 = d.w_in[plane_at(d, i, j)] * relaxation;
+    set_inlet_phi(d, i, j, phi_inlet, wi);
+    const double tmp1 = tmp2 * sa_inject;
+    tmp2 = tmp2 - tmp1;
+    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     for (int c = 0; c < 2; ++c) {
-        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double s = c == 0 ? tmp1 : tmp2;
 #pragma unroll
-        for (int m = 0; m < 5; ++m) A.at(QW[m], iw[m]) = (A.at(QR[m], ir[m]) + (m == 0 ? c1 : c2) * s) * om + A.at(QW[m], iw[m]) * wd;
+        for (int m = 0; m < 5; ++m) W[m].set(c, QW[m], (R[m].get(c, QR[m]) + (m == 0 ? c1 : c2) * s) * om + W[m].get(c, QW[m]) * wd);
     }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) W[m].store(d, QW[m], iw[m]);
 }
 // inlet_bounce_back_velocity_BC_after_odd (Boundary_multiphase_inlet.F90:56-102)
-template <class T>
+template <class T, bool HOIST = true>
 __device__ __forceinline__ void bc_inlet_velocity_after(const Dev& d, int t, double phi_inlet, double sa_inject, double relaxation) {
     const Layout L = d.L;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1;
-    const int wi = d.walls[L.at(i, j, 1)];
-    const double om = (double)(1 - wi), wd = (double)wi;
-    set_inlet_phi(d, i, j, phi_inlet, wi);
-    double tmp2 = d.w_in[plane_at(d, i, j)] * relaxation;
-    const double tmp1 = tmp2 * sa_inject;
-    tmp2 = tmp2 - tmp1;
-    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     const int o = L.at(i, j, 1);
-    // slot QW of (i, j, 1) from the slot QR at z = 0 (offset DI, DJ); slot indices resolved first
+    // slot QW of (i, j, 1) from the slot QR at z = 0 (offset DI, DJ); slot indices resolved and slots loaded first
     const int QW[5] = {6, 13, 14, 17, 18}, QR[5] = {5, 12, 11, 16, 15}, DI[5] = {0, 1, -1, 0, 0}, DJ[5] = {0, 0, 0, 1, -1};
     int iw[5], ir[5];
 #pragma unroll
@@ -610,12 +672,26 @@ __device__ __forceinline__ void bc_inlet_velocity_after(const Dev& d, int t, dou
         iw[m] = slot_index(d, QW[m], o);
         ir[m] = slot_index(d, QR[m], L.at(i + DI[m], j + DJ[m], 0));
     }
+    FaceSlot<T, HOIST> W[5], R[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        W[m].load(d, QW[m], iw[m]);
+        R[m].load(d, QR[m], ir[m]);
+    }
+    const int wi = d.walls[L.at(i, j, 1)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    double tmp2 = d.w_in[plane_at(d, i, j)] * relaxation;
+    set_inlet_phi(d, i, j, phi_inlet, wi);
+    const double tmp1 = tmp2 * sa_inject;
+    tmp2 = tmp2 - tmp1;
+    const double c1 = 6.0 * lat::W1, c2 = 6.0 * lat::W2;
     for (int c = 0; c < 2; ++c) {
-        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
         const double s = c == 0 ? tmp1 : tmp2;
 #pragma unroll
-        for (int m = 0; m < 5; ++m) A.at(QW[m], iw[m]) = (A.at(QR[m], ir[m]) + (m == 0 ? c1 : c2) * s) * om + A.at(QW[m], iw[m]) * wd;
+        for (int m = 0; m < 5; ++m) W[m].set(c, QW[m], (R[m].get(c, QR[m]) + (m == 0 ? c1 : c2) * s) * om + W[m].get(c, QW[m]) * wd);
     }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) W[m].store(d, QW[m], iw[m]);
 }
 // inlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_inlet.F90:115-209)
 template <class T>
@@ -678,54 +754,38 @@ __device__ __forceinline__ void bc_inlet_pressure_after(const Dev& d, int t, dou
 }
 
 // outlet_convective_BC_before_odd (Boundary_multiphase_outlet.F90:7-65)
-template <class T>
+template <class T, bool HOIST = true>
 __device__ __forceinline__ void bc_outlet_convective_before(const Dev& d, int t, double u_convec) {
     const Layout L = d.L;
     if (t >= L.nx * L.ny) return;
     const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
-    const double temp = 1.0 / (1.0 + u_convec);
-    const int wi = d.walls[L.at(i, j, nz)];
-    const double om = (double)(1 - wi), wd = (double)wi;
     const int pc = plane_at(d, i, j);
     const long pl = (long)(L.nx + 2) * (L.ny + 2);
-    double* ph = d.phi;
-    const double v = ((d.cphi[pc] + u_convec * phi_now(d, L.at(i, j, nz))) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
-    d.cphi[pc] = v;
-    ph[L.at(i, j, nz + 1)] = v;
-    ph[L.at(i, j, nz + 2)] = v;
-    ph[L.at(i, j, nz + 3)] = v;
-    ph[L.at(i, j, nz + 4)] = v;
-    // (slot, x offset, y offset) of the five unknown populations at the outlet halo
+    // (slot, x offset, y offset) of the five unknown populations at the outlet halo; slot indices resolved, and
+    // slots and history loaded, before the first store
     const int QS[5] = {6, 13, 14, 17, 18};
     const int DI[5] = {0, -1, 1, 0, 0};
     const int DJ[5] = {0, 0, 0, -1, 1};
-    int io[5], ii[5];   // slot indices, resolved first
+    int io[5], ii[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         io[m] = slot_index(d, QS[m], L.at(i + DI[m], j + DJ[m], nz + 1));
         ii[m] = slot_index(d, QS[m], L.at(i + DI[m], j + DJ[m], nz));
     }
-    for (int c = 0; c < 2; ++c) {
-        double* hist = c == 0 ? d.cf : d.cg;
-        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
+    FaceSlot<T, HOIST> O[5], I[5];
+    double H[2][5];
 #pragma unroll
-        for (int m = 0; m < 5; ++m)
-            A.at(QS[m], io[m]) = ((hist[pc + QS[m] * pl] + u_convec * A.at(QS[m], ii[m])) * temp) * om + A.at(QS[m], io[m]) * wd;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) hist[pc + QS[m] * pl] = A.at(QS[m], io[m]);
+    for (int m = 0; m < 5; ++m) {
+        O[m].load(d, QS[m], io[m]);
+        I[m].load(d, QS[m], ii[m]);
+        if constexpr (HOIST) {
+            H[0][m] = d.cf[pc + QS[m] * pl];
+            H[1][m] = d.cg[pc + QS[m] * pl];
+        }
     }
-}
-// outlet_convective_BC_after_odd (Boundary_multiphase_outlet.F90:68-127)
-template <class T>
-__device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, double u_convec) {
-    const Layout L = d.L;
-    if (t >= L.nx * L.ny) return;
-    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
     const double temp = 1.0 / (1.0 + u_convec);
     const int wi = d.walls[L.at(i, j, nz)];
     const double om = (double)(1 - wi), wd = (double)wi;
-    const int pc = plane_at(d, i, j);
-    const long pl = (long)(L.nx + 2) * (L.ny + 2);
     double* ph = d.phi;
     const double v = ((d.cphi[pc] + u_convec * phi_now(d, L.at(i, j, nz))) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
     d.cphi[pc] = v;
@@ -733,25 +793,75 @@ __device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, 
     ph[L.at(i, j, nz + 2)] = v;
     ph[L.at(i, j, nz + 3)] = v;
     ph[L.at(i, j, nz + 4)] = v;
-    // slot written at z = nz and the history slot it relaxes toward
+    for (int c = 0; c < 2; ++c) {
+        double* hist = c == 0 ? d.cf : d.cg;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            double h;
+            if constexpr (HOIST) h = H[c][m];
+            else h = hist[pc + QS[m] * pl];
+            O[m].set(c, QS[m], ((h + u_convec * I[m].get(c, QS[m])) * temp) * om + O[m].get(c, QS[m]) * wd);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) hist[pc + QS[m] * pl] = O[m].get(c, QS[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) O[m].store(d, QS[m], io[m]);
+}
+// outlet_convective_BC_after_odd (Boundary_multiphase_outlet.F90:68-127)
+template <class T, bool HOIST = true>
+__device__ __forceinline__ void bc_outlet_convective_after(const Dev& d, int t, double u_convec) {
+    const Layout L = d.L;
+    if (t >= L.nx * L.ny) return;
+    const int i = t % L.nx + 1, j = t / L.nx + 1, nz = L.nz;
+    const int pc = plane_at(d, i, j);
+    const long pl = (long)(L.nx + 2) * (L.ny + 2);
+    // slot written at z = nz and the history slot it relaxes toward; slot indices resolved, and slots and history
+    // loaded, before the first store
     const int QW[5] = {5, 11, 12, 15, 16};
     const int QH[5] = {6, 14, 13, 18, 17};
     const int o = L.at(i, j, nz), om1 = L.at(i, j, nz - 1);
-    int iw[5], im[5];   // slot indices, resolved first
+    int iw[5], im[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         iw[m] = slot_index(d, QW[m], o);
         im[m] = slot_index(d, QW[m], om1);
     }
+    FaceSlot<T, HOIST> W[5], M[5];
+    double H[2][5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        W[m].load(d, QW[m], iw[m]);
+        M[m].load(d, QW[m], im[m]);
+        if constexpr (HOIST) {
+            H[0][m] = d.cf[pc + QH[m] * pl];
+            H[1][m] = d.cg[pc + QH[m] * pl];
+        }
+    }
+    const double temp = 1.0 / (1.0 + u_convec);
+    const int wi = d.walls[L.at(i, j, nz)];
+    const double om = (double)(1 - wi), wd = (double)wi;
+    double* ph = d.phi;
+    const double v = ((d.cphi[pc] + u_convec * phi_now(d, L.at(i, j, nz))) * temp) * om + ph[L.at(i, j, nz + 1)] * wd;
+    d.cphi[pc] = v;
+    ph[L.at(i, j, nz + 1)] = v;
+    ph[L.at(i, j, nz + 2)] = v;
+    ph[L.at(i, j, nz + 3)] = v;
+    ph[L.at(i, j, nz + 4)] = v;
     for (int c = 0; c < 2; ++c) {
         double* hist = c == 0 ? d.cf : d.cg;
-        const Pop<T> A = c == 0 ? fpop<T>(d) : gpop<T>(d);
 #pragma unroll
-        for (int m = 0; m < 5; ++m)
-            A.at(QW[m], iw[m]) = ((hist[pc + QH[m] * pl] + u_convec * A.at(QW[m], im[m])) * temp) * om + A.at(QW[m], iw[m]) * wd;
+        for (int m = 0; m < 5; ++m) {
+            double h;
+            if constexpr (HOIST) h = H[c][m];
+            else h = hist[pc + QH[m] * pl];
+            W[m].set(c, QW[m], ((h + u_convec * M[m].get(c, QW[m])) * temp) * om + W[m].get(c, QW[m]) * wd);
+        }
 #pragma unroll
-        for (int m = 0; m < 5; ++m) hist[pc + QH[m] * pl] = A.at(QW[m], iw[m]);
+        for (int m = 0; m < 5; ++m) hist[pc + QH[m] * pl] = W[m].get(c, QW[m]);
     }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) W[m].store(d, QW[m], iw[m]);
 }
 // outlet_Zou_He_pressure_BC_before_odd (Boundary_multiphase_outlet.F90:139-228)
 template <class T>
@@ -868,13 +978,13 @@ __global__ void __launch_bounds__(BLOCK) k_outlet_pressure_after(Dev d, double r
 struct FaceArgs {
     double phi_inlet, sa_inject, relaxation, rho_in, u_convec, rho_out;
 };
-template <class T, int IN, int OUT, bool AFTER>
+template <class T, int IN, int OUT, bool AFTER, bool HOIST>
 __device__ __forceinline__ void open_faces_block(const Dev& d, const FaceArgs& a, int gface, int b) {
     if (b < gface) {
         const int t = b * BLOCK + (int)threadIdx.x;
         if (IN == 1) {
-            if (AFTER) bc_inlet_velocity_after<T>(d, t, a.phi_inlet, a.sa_inject, a.relaxation);
-            else bc_inlet_velocity_before<T>(d, t, a.phi_inlet, a.sa_inject, a.relaxation);
+            if (AFTER) bc_inlet_velocity_after<T, HOIST>(d, t, a.phi_inlet, a.sa_inject, a.relaxation);
+            else bc_inlet_velocity_before<T, HOIST>(d, t, a.phi_inlet, a.sa_inject, a.relaxation);
         } else {
             if (AFTER) bc_inlet_pressure_after<T>(d, t, a.phi_inlet, a.sa_inject, a.rho_in, a.relaxation);
             else bc_inlet_pressure_before<T>(d, t, a.phi_inlet, a.sa_inject, a.rho_in, a.relaxation);
@@ -882,8 +992,8 @@ __device__ __forceinline__ void open_faces_block(const Dev& d, const FaceArgs& a
     } else {
         const int t = (b - gface) * BLOCK + (int)threadIdx.x;
         if (OUT == 1) {
-            if (AFTER) bc_outlet_convective_after<T>(d, t, a.u_convec);
-            else bc_outlet_convective_before<T>(d, t, a.u_convec);
+            if (AFTER) bc_outlet_convective_after<T, HOIST>(d, t, a.u_convec);
+            else bc_outlet_convective_before<T, HOIST>(d, t, a.u_convec);
         } else {
             if (AFTER) bc_outlet_pressure_after<T>(d, t, a.rho_out);
             else bc_outlet_pressure_before<T>(d, t, a.rho_out);
@@ -892,7 +1002,7 @@ __device__ __forceinline__ void open_faces_block(const Dev& d, const FaceArgs& a
 }
 template <class T, int IN, int OUT, bool AFTER>
 __global__ void __launch_bounds__(BLOCK) k_open_faces(Dev d, FaceArgs a, int gface) {
-    open_faces_block<T, IN, OUT, AFTER>(d, a, gface, (int)blockIdx.x);
+    open_faces_block<T, IN, OUT, AFTER, true>(d, a, gface, (int)blockIdx.x);
 }
 
 // porous_plate_BC_before_odd / _after_odd (Boundary_multiphase_other.F90:8-184); `blocked` is the colour
@@ -1353,7 +1463,7 @@ __global__ void __launch_bounds__(BLOCK) k_brick_plan(Dev d, ChainPlan p, int fu
     int tile = (int)blockIdx.x;
     if (IN != 0) {
         if (tile < 2 * gface) {
-            open_faces_block<T, IN, OUT, AFTER>(d, a, gface, tile);
+            open_faces_block<T, IN, OUT, AFTER, false>(d, a, gface, tile);
             return;
         }
         tile -= 2 * gface;
