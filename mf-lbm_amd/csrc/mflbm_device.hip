@@ -2364,8 +2364,8 @@ extern "C" int mflbm_create(const mflbm_params* prm, mflbm_ctx** out) {
     c->p = *prm;
     c->fp32 = prm->precision != 0;
     {
-        // the fused chain is bit-identical but slower on the 256^3 benchmark (0.78 vs 0.42 ms; the lattice
-        // update it feeds is 8 % faster): the list kernels stay the default
+        // the fused chain is bit-identical but evaluates every node, so it is slower than the brick lists on the
+        // 256^3 benchmark although the lattice update it feeds is 8 % faster: the list kernels stay the default.
         // FP32 collision: its rounding leaves phi outside the band at nearly every node, so the plan would list
         // nearly every brick and the whole-list chain is the faster (MFLBM_CHAIN=bricks keeps the plan)
         const char* e = std::getenv("MFLBM_CHAIN");
