@@ -58,7 +58,11 @@ __device__ __forceinline__ int link_slot(int nc, const int* wbase, const unsigne
 // the link slot of (m, r) where the wave's link mask has m's bit, else the compact index of the fluid node x + e_r.
 // The wave-uniform masks and bases are scalar loads issued together; x + e_r lies inside the padded grid, so the
 // 18 dense map entries are loaded unconditionally, all in flight at once, and selected afterwards (a conditional
-// load per direction compiles to a branch that waits for every earlier load).
+// load per direction compiles to a branch that waits for every earlier load).  ROWS: the entries of a row of three
+// directions (ex = -1, 0, +1 at one (ey, ez)) are adjacent and taken by one 12-byte load (10 load instructions
+// instead of 18): 1.6-2.1 % faster single-phase steps at 256^3, 0.4 % slower for the two-phase update, whose
+// curvature stencil keeps more loads in flight.
+template <bool ROWS>
 __device__ __forceinline__ void odd_neighbour_slots(const int* __restrict__ map, const Layout& L, int nc,
                                                     const int* __restrict__ wbase,
                                                     const unsigned long long* __restrict__ wmask, int m, int n,
@@ -67,8 +71,25 @@ __device__ __forceinline__ void odd_neighbour_slots(const int* __restrict__ map,
     const unsigned lane = (unsigned)m & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
     unsigned mv[19];
+    if constexpr (ROWS) {   // rows (ey, ez) = (0, 0), (1, 0), (-1, 0), (0, 1), (0, -1); the four directions with ey
+                            // and ez both nonzero one by one (4-byte aligned 12-byte loads)
+        struct __attribute__((aligned(4))) Row {
+            int v[3];
+        };
+        Row row[5];
+        const int rb[5] = {n - 1, n + L.px - 1, n - L.px - 1, n + L.pxy - 1, n - L.pxy - 1};
 #pragma unroll
-    for (int r = 1; r < 19; ++r) mv[r] = (unsigned)map[n + L.off(r)];
+        for (int k = 0; k < 5; ++k) row[k] = *(const Row*)(map + rb[k]);
+#pragma unroll
+        for (int r = 1; r < 19; ++r) {
+            const int ey = lat::EY[r], ez = lat::EZ[r];
+            const int k = ey == 1 ? 1 : ey == -1 ? 2 : ez == 1 ? 3 : ez == -1 ? 4 : 0;
+            mv[r] = (unsigned)((ey != 0 && ez != 0) ? map[n + L.off(r)] : row[k].v[lat::EX[r] + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 1; r < 19; ++r) mv[r] = (unsigned)map[n + L.off(r)];
+    }
 #pragma unroll
     for (int r = 1; r < 19; ++r) {
         const unsigned long long wm = wmask[e0 + r];

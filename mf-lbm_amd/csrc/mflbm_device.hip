@@ -340,7 +340,7 @@ k_collide(Dev d, CollideParams P, int per) {
         b[0] = v.y;
     }
     if (ODD) {
-        odd_neighbour_slots(d.map, d.L, d.nc, d.wbase, d.wmask, m, nn, nb);
+        odd_neighbour_slots<false>(d.map, d.L, d.nc, d.wbase, d.wmask, m, nn, nb);
 #pragma unroll
         for (int q = 1; q < 19; ++q) {   // slot q of the upwind node x - e_q = x + e_opp(q)
             const double2 v = ld_pair(S.p[q], nb[lat::OPP[q]]);
@@ -405,7 +405,7 @@ __global__ void __launch_bounds__(BLOCK) k_collide32(Dev d, CollideParamsDev<C> 
         a[0] = v.y;
     }
     if (ODD) {
-        odd_neighbour_slots(d.map, d.L, d.nc, d.wbase, d.wmask, m, nn, nb);
+        odd_neighbour_slots<false>(d.map, d.L, d.nc, d.wbase, d.wmask, m, nn, nb);
 #pragma unroll
         for (int q = 1; q < 19; ++q) {
             const float2 v = ld_pair32(S.p[q], nb[lat::OPP[q]]);

@@ -93,7 +93,7 @@ __device__ __forceinline__ void sp_update_node(const SDev<T>& d, const SPCollide
     unsigned nb[19];   // odd: per direction r, the index of the slot (x + e_r, opp(r)) in slice opp(r)
     a[0] = slot_ld(d.S[0], m);
     if (ODD) {
-        odd_neighbour_slots(d.map, d.L, d.nc, d.wbase, d.wmask, m, d.cpos[m], nb);
+        odd_neighbour_slots<true>(d.map, d.L, d.nc, d.wbase, d.wmask, m, d.cpos[m], nb);
 #pragma unroll
         for (int q = 1; q < 19; ++q) a[q] = slot_ld(d.S[q], nb[lat::OPP[q]]);   // slot q of the upwind node x - e_q
     } else {
