@@ -161,12 +161,7 @@ __device__ __forceinline__ int plane_at(const Dev& d, int i, int j) { return i +
 // whole 128-byte lines of a dense FP64 field, so the chain stages that take one brick per wave touch few lines
 // per access).  The brick flags need every brick at least 2 nodes wide in each axis (a normal depends on phi
 // within 2 nodes: its stencil and the fluid neighbours of the stencil's solid nodes).
-#ifndef MFLBM_BRICK_SX
-#define MFLBM_BRICK_SX 4
-#define MFLBM_BRICK_SY 1
-#define MFLBM_BRICK_SZ 1
-#endif
-constexpr int BRICK_SX = MFLBM_BRICK_SX, BRICK_SY = MFLBM_BRICK_SY, BRICK_SZ = MFLBM_BRICK_SZ;
+constexpr int BRICK_SX = 4, BRICK_SY = 1, BRICK_SZ = 1;
 static_assert(BRICK_SX >= 1 && BRICK_SY >= 1 && BRICK_SZ >= 1 && BRICK_SX + BRICK_SY + BRICK_SZ == 6,
               "64-node bricks at least 2 nodes wide");
 __device__ __forceinline__ int brick_of(const Dev& d, int n) {
@@ -1230,10 +1225,7 @@ struct BrickWork {
     const int* cnt;     // their numbers (NXCD)
     int cap;
 };
-#ifndef MFLBM_CHAIN_BLOCKS
-#define MFLBM_CHAIN_BLOCKS 2048
-#endif
-constexpr int CHAIN_BLOCKS = MFLBM_CHAIN_BLOCKS;   // grid of the brick-list kernels: what the GPU holds at once
+constexpr int CHAIN_BLOCKS = 2048;   // grid of the brick-list kernels: what the GPU holds at once (1024-8192 measured alike)
 constexpr int CHAIN_PER = CHAIN_BLOCKS / NXCD;     // work-groups per XCD
 static_assert(CHAIN_BLOCKS % NXCD == 0, "whole work-groups per XCD");
 // The work items of a group of G lanes.  A work list is kept as NXCD sub-lists (the plan appends a tile's bricks
