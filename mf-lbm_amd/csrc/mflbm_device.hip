@@ -1557,6 +1557,12 @@ __global__ void k_phi_dense(Dev d) {
     const int m = blockIdx.x * BLOCK + threadIdx.x;
     if (m < d.nf) d.phi[d.cpos[m]] = d.phic[m];
 }
+// the same, with the n bytes at z cleared in the same launch (the whole-list chain's stencil flags)
+__global__ void k_phi_dense_clear(Dev d, unsigned char* z, int n) {
+    const int m = blockIdx.x * BLOCK + threadIdx.x;
+    if (m < d.nf) d.phi[d.cpos[m]] = d.phic[m];
+    if (m < n) z[m] = 0;
+}
 // the compact phi from the dense phi (after an upload or a device initialisation)
 __global__ void k_phic_gather(Dev d) {
     const int m = blockIdx.x * BLOCK + threadIdx.x;
@@ -2110,9 +2116,10 @@ int launch_plan(mflbm_ctx* c, const ChainPlan& p, int full, const FaceWork& f) {
 template <class NT>
 int launch_gradient_t(mflbm_ctx* c, const FaceWork& faces) {
     if (c->chain_all) {   // every entry of every list, from the dense phi; nzd cleared and set again by the normals
-        const int re = ensure_dense_phi(c);
-        if (re) return re;
-        LAUNCH(c, k_clear_bytes, nblocks(c->nbricks), c->d.nzd, (int)c->nbricks);
+        const int nb = (int)c->nbricks;
+        if (c->phi_stale) LAUNCH(c, k_phi_dense_clear, nblocks(std::max(c->d.nf, nb)), c->d, c->d.nzd, nb);
+        else LAUNCH(c, k_clear_bytes, nblocks(nb), c->d.nzd, nb);
+        c->phi_stale = false;
         LAUNCH(c, k_phi_extrapolate, NXCD * xcd_per(c->sl.n), c->d, c->sl, xcd_per(c->sl.n));
         LAUNCH(c, k_normals_all<NT>, nblocks(c->nrm.n), c->d, c->nrm, (const int*)c->wbuf, c->fl);
         LAUNCH(c, k_normal_extrapolate_all<NT>, nblocks(c->sl1.n), c->d, c->sl1);
